@@ -1,0 +1,202 @@
+"""Benchmark: keyword-PIR respond throughput and latency on one MI355X.
+
+Workload: 1,000,000 ML-KEM-768 keys (entry 1216 B) at the reference's default
+shape (buckets 2,097,152, dim0 2048 -> db_rows 2048, db_cols 720,896,
+352 packed outputs, interpolation degree 32, 11 fold chains).  Synthetic
+device-resident state: encoded DB (2.95 GB) and InspiRING key-switch material
+(35.4 GB) from the counter-based generator; the (32x smaller) cuckoo/encode
+setup is not part of the timed path.  One step = one keyword query (2 hash
+queries, the cuckoo num_hashes) answered end to end on the GPU from a
+serialized query already resident in HBM: unpack, packing, INTT/CRT, GEMV,
+RGSW fold, modulus switch and bit-pack.
+
+Prints one JSON line (rank 0).  The cpu_baseline leg times the CPU oracle on a
+bounded sample of the same workload and scales it to one keyword query.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "inspire-rs_amd"))
+
+METRIC = "PIR queries/sec + p50/p99 respond latency, 1M keys, batched vs single-query"
+HBM_PEAK_GBS = 8000.0
+RECORDS, KEY, VALUE = 1_000_000, 32, 1184
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def make_queries(IA, torch, db_rows, count, seed, nq=2):
+    """Serialized synthetic keyword queries (uniform residues, as from a client
+    with a real ternary key), staged back to back in one device buffer."""
+    import numpy as np
+    N = IA.N
+    blobs = []
+    for i in range(count):
+        s = seed * 1000 + i
+        y = torch.empty(3 * N, dtype=torch.int64, device="cuda")
+        z = torch.empty(3 * N, dtype=torch.int64, device="cuda")
+        r = torch.empty(nq * db_rows, dtype=torch.int64, device="cuda")
+        b = torch.empty(nq * 12 * N, dtype=torch.int64, device="cuda")
+        IA.synth_fill(y, s, IA.STREAM_Y, 3 * N, IA.KIND_CONDENSED)
+        IA.synth_fill(z, s, IA.STREAM_Z, 3 * N, IA.KIND_CONDENSED)
+        IA.synth_fill(r, s, IA.STREAM_ROW, nq * db_rows, IA.KIND_CONDENSED)
+        IA.synth_fill(b, s, IA.STREAM_BODY, nq * 12 * N, IA.KIND_CRT)
+        torch.cuda.synchronize()
+        hv = lambda t: t.cpu().numpy().view(np.uint64)
+        blobs.append(IA.serialize_query(hv(y).reshape(3, N), hv(z).reshape(3, N),
+                                        hv(r).reshape(nq, db_rows), hv(b).reshape(nq, 6, 2, N)))
+    qlen = len(blobs[0])
+    stride = (qlen + 16 + 255) // 256 * 256
+    host = np.zeros(stride * count, dtype=np.uint8)
+    for i, bl in enumerate(blobs):
+        host[i * stride:i * stride + qlen] = np.frombuffer(bl, dtype=np.uint8)
+    dev = torch.from_numpy(host).cuda()
+    return dev, qlen, stride
+
+
+def cpu_baseline(plan, out_samples=32, fold_chains=2):
+    """Times the CPU oracle (restatement of the reference's respond path) on a
+    bounded sample and scales it to one keyword query (2 hash queries)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+    N, inst, d, rows = O.N, plan["instances"], plan["interpolate_degree"], plan["db_rows"]
+    J = min(out_samples, inst)
+    seed = 11
+    bt = O.synth(seed, O.STREAM_BT, J * O.NINNER * N, O.KIND_CONDENSED)
+    btb = O.synth(seed, O.STREAM_BTB, J * O.NINNER * N, O.KIND_CONDENSED)
+    bth = O.synth(seed, O.STREAM_BTH, J * 3 * N, O.KIND_CONDENSED)
+    db = O.synth(seed, O.STREAM_DB, J * N * rows, O.KIND_U16)
+    y, z, qrows, bodies = O.synth_query(seed, rows, nq=2)
+    t0 = time.perf_counter()
+    S = O.packing(bt, btb, bth, y, z, J)
+    t_pack = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.post(S)
+    t_post = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for h in range(2):
+        O.gemv(qrows[h], db, rows, J * N)
+    t_gemv = time.perf_counter() - t0
+    neg_a, _ = O.rgsw_neg_a_ntt()
+    pa = O.synth(seed, O.STREAM_AHAT, d * N, O.KIND_MODQ).reshape(d, N)
+    pb = O.synth(seed + 1, O.STREAM_AHAT, d * N, O.KIND_MODQ).reshape(d, N)
+    t0 = time.perf_counter()
+    for _ in range(fold_chains):
+        O.switch_and_keep(O.fold_chain(pa, pb, neg_a, bodies[0]))
+    t_fold = (time.perf_counter() - t0) / fold_chains
+    scale = inst / J
+    per_query = (t_pack + t_post + t_gemv) * scale + t_fold * 2 * plan["c"]
+    sample = ("packing+INTT for %d of %d outputs, GEMV of %d of %d columns x 2 hash queries, "
+              "fold+modswitch of %d of %d chains; linearly scaled to one keyword query"
+              % (J, inst, J * N, plan["db_cols"], fold_chains, 2 * plan["c"]))
+    return {"value": 1.0 / per_query, "unit": "queries/s", "cores": O.num_threads(), "kind": "port",
+            "sample": sample, "seconds_per_query": per_query,
+            "stage_s": {"packing": t_pack * scale, "post": t_post * scale, "gemv": t_gemv * scale,
+                        "fold": t_fold * 2 * plan["c"]}}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--queries", type=int, default=8, help="distinct resident queries cycled through")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import inspire_amd as IA
+
+    plan = IA.plan_shape(RECORDS, KEY, VALUE)
+    db_rows, inst, d = plan["db_rows"], plan["instances"], plan["interpolate_degree"]
+    log("plan", {k: plan[k] for k in ("num_items", "db_rows", "db_cols", "instances", "interpolate_degree", "c")})
+    srv = IA.Server(db_rows, inst, d)
+    t0 = time.time()
+    srv.fill_synthetic(11)
+    log("state resident in %.1fs" % (time.time() - t0))
+    dq, qlen, stride = make_queries(IA, torch, db_rows, args.queries, 13)
+    out = torch.empty(2 * plan["c"] * IA.RESP_BYTES, dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+
+    def step(k):
+        srv.respond_device(dq.data_ptr() + (k % args.queries) * stride, qlen, 2, out, sh)
+
+    for k in range(args.warmup):
+        step(k)
+    torch.cuda.synchronize()
+    log("warmup done")
+
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    t_begin = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_begin.record(stream)
+    for k in range(args.steps):
+        starts[k].record(stream)
+        step(k)
+        ends[k].record(stream)
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    total_ms = t_begin.elapsed_time(t_end)
+    lat = sorted(s.elapsed_time(e) for s, e in zip(starts, ends))
+    p = lambda q: lat[min(len(lat) - 1, int(round(q * (len(lat) - 1))))]
+    qps = args.steps / (total_ms / 1e3)
+    log("timed %d steps: %.3f ms/step, %.1f q/s" % (args.steps, total_ms / args.steps, qps))
+
+    # Per-stage kernel times (hipEvents between the kernels on the respond stream).
+    srv.profile(True)
+    for k in range(args.profile_steps):
+        step(k)
+    torch.cuda.synchronize()
+    stage_ms, cnt = srv.profile_read()
+    srv.profile(False)
+    names = ["unpack", "packing", "post", "gemv", "fold"]
+    stage = {n: stage_ms[i] / max(cnt, 1) for i, n in enumerate(names)}
+    pack_bytes = plan["bold_t_bytes"] + inst * 3 * IA.N * 8
+    gemv_bytes = plan["db_bytes"]
+    pack_gbs = pack_bytes / (stage["packing"] / 1e3) / 1e9
+    gemv_gbs = gemv_bytes / (stage["gemv"] / 1e3) / 1e9
+    log("stages (ms):", {k: round(v, 4) for k, v in stage.items()})
+
+    result = {
+        "metric": METRIC, "value": qps, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": total_ms / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": "single-query keyword respond (2 hash queries), 1M ML-KEM-768 keys",
+                   "records": RECORDS, "entry_bytes": KEY + VALUE, "buckets": plan["num_items"],
+                   "dim0": plan["dim0"], "db_rows": db_rows, "db_cols": plan["db_cols"], "instances": inst,
+                   "interpolate_degree": d, "chains": plan["c"], "hash_queries": 2,
+                   "parallelism": "1 GPU"},
+        "p50_ms": p(0.5), "p99_ms": p(0.99),
+        "stage_ms": stage,
+        "roofline": {"bound": "hbm", "kernel": "k_packing (InspiRING key-switch inner product)",
+                     "achieved": pack_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": pack_gbs / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes": pack_bytes,
+                     "gemv": {"achieved": gemv_gbs, "frac": gemv_gbs / HBM_PEAK_GBS, "algorithmic_bytes": gemv_bytes}},
+    }
+    srv.close()
+    del out, dq
+    torch.cuda.empty_cache()
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        result["cpu_baseline"] = cpu_baseline(plan)
+        log("cpu baseline %.3f q/s" % result["cpu_baseline"]["value"])
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

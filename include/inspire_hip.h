@@ -1,0 +1,213 @@
+/*
+ * inspire_hip.h -- C ABI of the MI355X (gfx950) keyword-PIR answer path.
+ *
+ * Two groups of entry points:
+ *
+ *  1. The operator surface the reference's Rust FFI binds today
+ *     (src/pir/gpu/{gemv,packing_online,encode}.rs extern "C" blocks,
+ *     implemented in the native/cuda sources).  Same names, argument meaning and
+ *     void/NULL error behaviour, so the Rust wrappers link unchanged.
+ *
+ *  2. Handle-based entry points for the parts of KeywordServer::respond
+ *     (src/pir/server/keyword.rs:534-628) that the reference runs on the CPU
+ *     (post-process, RGSW fold, modulus switch, response assembly) and for the
+ *     whole respond path on device.  These return an int status
+ *     (INSPIRE_OK = 0, negative on error; inspire_last_error() explains).
+ *
+ * All sizes follow the reference: n = poly_len = 2048, two CRT primes
+ * q0 = 268369921, q1 = 249561089 (src/pir/params.rs:9), t_exp_left = 3,
+ * "condensed" u64 = (x mod q0) | (x mod q1) << 32 (src/pir/packing.rs:867-884).
+ */
+#ifndef INSPIRE_HIP_H
+#define INSPIRE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes                                                              */
+/* ------------------------------------------------------------------------ */
+#define INSPIRE_OK 0
+#define INSPIRE_ERR_INVALID (-1)  /* bad argument, shape or malformed query */
+#define INSPIRE_ERR_HIP (-2)      /* HIP runtime error */
+#define INSPIRE_ERR_NOMEM (-3)    /* device allocation failed */
+#define INSPIRE_ERR_STATE (-4)    /* required server state not loaded */
+#define INSPIRE_ERR_CAPACITY (-5) /* caller's output buffer too small */
+
+/* Message for the last error on the calling thread ("" if none). */
+const char* inspire_last_error(void);
+
+/* ======================================================================== */
+/* 1. Reference operator surface (native/cuda replacement)                  */
+/* ======================================================================== */
+
+/* encode_kernel.cu:233-255 (gpu_init_memory_pool): creates the query stream
+ * (default priority) and warms the allocator.  Idempotent. */
+void gpu_init_memory_pool(void);
+/* encode_kernel.cu:258-260: free device memory in bytes (0 on failure). */
+uint64_t gpu_available_memory_bytes(void);
+
+/* gemv.rs:6 / gemv_kernel.cu:116-136: copy a host DB (u16, column-major
+ * [db_cols][db_rows]) to the device; *d_db_out = NULL on failure. */
+void gpu_gemv_upload_db(const uint16_t* h_db, uint16_t** d_db_out, size_t db_size_bytes);
+/* gemv.rs:7 / gemv_kernel.cu:107-113: adopt an existing device DB pointer. */
+void gpu_gemv_set_device_db(uint16_t** d_db_out, const uint16_t* d_db_in, size_t db_size_bytes);
+/* gemv.rs:8 / gemv_kernel.cu:139-141. */
+void gpu_gemv_free_db(uint16_t* d_db);
+/* gemv.rs:9-24 / gemv_kernel.cu:144-194: h_c[j] = sum_k h_a[k]*db[j][k] mod q,
+ * synchronous, host in/out.  The modulus arguments must be the reference's
+ * constants (they are checked; a mismatch leaves h_c untouched). */
+void gpu_gemv_query(uint64_t* h_c, const uint64_t* h_a, const uint16_t* d_db, int db_rows, int b_cols,
+                    uint64_t mod0, uint64_t mod1, uint64_t cr0, uint64_t cr1, uint64_t mod0_inv_mod1,
+                    uint64_t mod1_inv_mod0, uint64_t modulus, uint64_t barrett_cr0_mod,
+                    uint64_t barrett_cr1_mod);
+
+/* packing_online.rs:23-29 / rotation_kernel.cu:71-94. */
+void gpu_rotation_upload_tables(const uint32_t* h_tables, const uint32_t* h_gen_pows, int num_tables,
+                                int num_rotations, int poly_len);
+/* packing_online.rs:31-38 / rotation_kernel.cu:99-137. */
+void gpu_rotation_expand_keys(uint64_t* d_y_all_out, uint64_t* d_y_bar_all_out, const uint64_t* h_y_body,
+                              int num_rotations, int t_exp_left, int poly_len);
+/* packing_online.rs:40 / rotation_kernel.cu:140-144. */
+void gpu_rotation_free(void);
+
+/* packing_online.rs:42-47 / packing_online_kernel.cu:176-211. */
+void gpu_packing_online_ensure_key_buffers(int n_inner, int t_exp_left, int poly_len);
+uint64_t* gpu_packing_online_get_d_y_all(void);
+uint64_t* gpu_packing_online_get_d_y_bar_all(void);
+void gpu_packing_online_upload_z_body(const uint64_t* h_z_body, int t_exp_left, int poly_len);
+/* packing_online.rs:49-57 / packing_online_kernel.cu:110-143. */
+void gpu_packing_online_upload_precomp(const uint64_t* h_bold_t, const uint64_t* h_bold_t_bar,
+                                       const uint64_t* h_bold_t_hat, int num_outputs, int n_inner,
+                                       int t_exp_left, int poly_len);
+/* packing_online.rs:13-21 / packing_online_kernel.cu:258-288 (borrowed pointers). */
+void gpu_packing_online_adopt_precomp(uint64_t* d_bold_t, uint64_t* d_bold_t_bar, uint64_t* d_bold_t_hat,
+                                      int num_outputs, int n_inner, int t_exp_left, int poly_len);
+/* packing_online.rs:59-66 / packing_online_kernel.cu:147-172. */
+void gpu_packing_online_upload_keys(const uint64_t* h_y_all, const uint64_t* h_y_bar_all,
+                                    const uint64_t* h_z_body, int n_inner, int t_exp_left, int poly_len);
+/* packing_online.rs:68-79 / packing_online_kernel.cu:215-251: synchronous;
+ * h_output [num_outputs][2][poly_len] (CRT0 slots, then CRT1 slots). */
+void gpu_packing_online_compute(uint64_t* h_output, int num_outputs, int nphalf_minus_1, int t_exp_left,
+                                int poly_len, int addition_capacity, uint64_t mod0, uint64_t mod1,
+                                uint64_t cr0, uint64_t cr1);
+/* packing_online.rs:83-91 / packing_online_kernel.cu:310-333. */
+void gpu_packing_online_swap_precomp(uint64_t* new_bt, uint64_t* new_bt_bar, uint64_t* new_bt_hat,
+                                     int num_outputs, int n_inner, int t_exp_left, int poly_len);
+/* packing_online.rs:81 / packing_online_kernel.cu:291-306. */
+void gpu_packing_online_free(void);
+
+/* ======================================================================== */
+/* 2. Handle-based answer path                                              */
+/* ======================================================================== */
+
+/* KeywordServerSetupPlan + KeywordServerGpuMemoryEstimate
+ * (keyword.rs:91-233; parameters from protocol/wire.rs:386-477). */
+typedef struct {
+  int64_t num_items;          /* cuckoo buckets */
+  int64_t item_size_bits;     /* 8 * (key_size + value_size) */
+  int64_t dim0;
+  int64_t db_rows;
+  int64_t db_cols;
+  int64_t instances;          /* db_cols_prime = packed RLWE outputs */
+  int64_t interpolate_degree; /* d */
+  int64_t c;                  /* fold chains = instances / d */
+  uint64_t db_bytes, hint_bytes, prep_pack_bytes, r_buffers_bytes, bold_t_bytes, peak_bytes;
+} inspire_plan_t;
+
+/* keyword.rs:145-233 from_shape.  buckets / dim0 <= 0 select the reference
+ * defaults.  On INSPIRE_ERR_INVALID the message (inspire_last_error) is the
+ * reference's error text. */
+int inspire_plan_shape(int64_t record_count, int64_t key_size, int64_t value_size, int64_t buckets,
+                       int64_t dim0, inspire_plan_t* out);
+
+typedef struct {
+  int64_t db_rows;            /* 2^(db_dim_1 + 11), multiple of 2048 */
+  int64_t instances;          /* db_cols / 2048 */
+  int32_t interpolate_degree; /* d, divides instances */
+  int32_t max_queries;        /* hash queries per keyword query (cuckoo num_hashes = 2) */
+} inspire_shape_t;
+
+typedef struct inspire_server inspire_server_t;
+
+/* Allocates device state and per-request buffers on the current device. */
+int inspire_server_create(const inspire_shape_t* shape, inspire_server_t** out);
+void inspire_server_destroy(inspire_server_t* srv);
+
+/* Encoded DB, column-major [db_cols][db_rows] u16 (cuckoo.rs:314-353 after
+ * encode_kernel.cu).  host != 0: copy from host memory; host == 0: copy from
+ * a device pointer. */
+int inspire_server_set_db(inspire_server_t* srv, const uint16_t* db, int host);
+/* InspiRING precomputation (PrecompInsPIR, packing.rs:1718-1736) in the
+ * layout of gpu_packing_online_upload_precomp: bold_t, bold_t_bar
+ * [instances][3069][2048] condensed, bold_t_hat [instances][3][2048]
+ * condensed, a_hat [instances][2048] raw mod q. */
+int inspire_server_set_precomp(inspire_server_t* srv, const uint64_t* bold_t, const uint64_t* bold_t_bar,
+                               const uint64_t* bold_t_hat, const uint64_t* a_hat, int host);
+/* Cuckoo stash entries returned with every response (keyword.rs:624-627). */
+int inspire_server_set_stash(inspire_server_t* srv, const uint8_t* entries, uint32_t entry_size,
+                             uint32_t count);
+/* Fills DB and precomputation on the device from the counter-based synthetic
+ * generator (benchmarks and tests; see inspire_synth_fill). */
+int inspire_server_fill_synthetic(inspire_server_t* srv, uint64_t seed);
+
+/* Serialized response size for a query with nq hash queries. */
+size_t inspire_server_response_bytes(const inspire_server_t* srv, int nq);
+
+/* KeywordServer::respond_to_serialized_query (keyword.rs:636-640):
+ * deserialize (wire.rs:618-705), respond, serialize (wire.rs:708-726). */
+int inspire_respond_serialized(inspire_server_t* srv, const uint8_t* query, size_t len, uint8_t* resp,
+                               size_t cap, size_t* resp_len);
+
+/* Device-resident respond: d_query holds a serialized query in device memory
+ * (len bytes, nq = its first byte, passed by the caller to avoid a read-back),
+ * d_out receives nq * (instances/d) * 12288 response bytes.  Asynchronous on
+ * `stream` (hipStream_t; NULL = the server's own stream). */
+int inspire_respond_device(inspire_server_t* srv, const uint8_t* d_query, size_t len, int nq, uint8_t* d_out,
+                           void* stream);
+/* The server's stream (hipStream_t). */
+void* inspire_server_stream(inspire_server_t* srv);
+
+/* Per-stage GPU timing of the respond pipeline: hipEvents recorded between
+ * the kernels on the respond stream.  While enabled, every respond waits for
+ * its own completion and adds its stage times.  profile_read returns the
+ * summed milliseconds per stage [unpack, packing, post, gemv, fold] and the
+ * number of profiled responds; reset != 0 clears them. */
+int inspire_server_profile_enable(inspire_server_t* srv, int enable);
+int inspire_server_profile_read(inspire_server_t* srv, double* ms5, int64_t* count, int reset);
+
+/* --- Stage entry points (device pointers, asynchronous on `stream`) ------ */
+
+/* Counter-based synthetic fill: kind 0 condensed residues, 1 residues mod q,
+ * 2 u16 in [0, 65534], 3 per-CRT residues of [..][2][2048] polys, 4 small
+ * residues (< 2^20, as from a client with sk = 0). */
+int inspire_synth_fill(void* d_out, uint64_t seed, uint32_t stream_id, uint64_t start, uint64_t count, int kind,
+                       void* stream);
+/* wire.rs:618-705 on device: y_body, z_body [3][2048] condensed; rows
+ * [nq][db_rows] condensed; bodies [nq][6][2][2048] u32. */
+int inspire_unpack_query_device(const uint8_t* d_query, int nq, int64_t db_rows, uint64_t* d_y_body,
+                                uint64_t* d_z_body, uint64_t* d_rows, uint32_t* d_bodies, void* stream);
+/* gemv_kernel.cu:46-103 for nq query rows at once: d_c [nq][db_cols]. */
+int inspire_gemv_device(uint64_t* d_c, const uint64_t* d_rows, int nq, const uint16_t* d_db, int64_t db_rows,
+                        int64_t db_cols, void* stream);
+/* packing_online_kernel.cu:18-92 with the rotations fused:
+ * d_S [instances][2][2048] u32 canonical. */
+int inspire_packing_device(uint32_t* d_S, const uint64_t* d_bold_t, const uint64_t* d_bold_t_bar,
+                           const uint64_t* d_bold_t_hat, const uint64_t* d_y_body, const uint64_t* d_z_body,
+                           int64_t instances, void* stream);
+/* packing_online.rs:359-398: d_P [count][2048] = CRT(INTT(S)) mod q. */
+int inspire_post_device(uint64_t* d_P, const uint32_t* d_S, int64_t count, void* stream);
+/* keyword.rs:543-581 + modulus_switch.rs:45-73 for nq hash queries:
+ * d_out [nq][instances/d][12288]. */
+int inspire_fold_device(uint8_t* d_out, const uint64_t* d_a_hat, const uint64_t* d_c, const uint64_t* d_P,
+                        const uint32_t* d_bodies, int64_t instances, int d, int nq, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* INSPIRE_HIP_H */

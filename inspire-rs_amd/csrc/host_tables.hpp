@@ -1,0 +1,22 @@
+#pragma once
+#include <cstddef>
+#include <cstdint>
+
+#include "modarith.hpp"
+
+namespace inspire {
+
+// out: [2 primes][4][kN] u32: forward w, forward w' = floor(w*2^32/q),
+// inverse w (halved), inverse w'.
+void build_ntt_tables(uint32_t* out);
+
+// In-place forward NTT of one residue polynomial (canonical in, canonical out).
+void host_ntt_forward(uint32_t* a, int c, const uint32_t* tables);
+
+// ChaCha20Rng::from_seed(seed) followed by n u64 draws (low word first).
+void chacha20_u64(const uint8_t seed[32], uint64_t* out, size_t n);
+
+// out: [2][kN] = NTT(-a mod q_c) for the constant RGSW row-0 polynomial a.
+void build_rgsw_neg_a(uint32_t* out, const uint32_t* tables);
+
+}  // namespace inspire

@@ -1,0 +1,573 @@
+// gfx950 kernels for the keyword-PIR answer path (KeywordServer::respond,
+// src/pir/server/keyword.rs:534-628).
+//
+//   k_unpack_query   wire.rs:618-705   bit-unpack of the serialized query
+//   k_gemv           gemv_kernel.cu:46-103 / gpu/kernel.rs:139-237
+//   k_packing        packing_online_kernel.cu:18-92 + rotation_kernel.cu:21-59
+//   k_post           packing_online.rs:359-398 (INTT + CRT)
+//   k_fold           keyword.rs:543-581 + modulus_switch.rs:45-73
+//   k_rotation_legacy / k_packing (legacy form) for the native/cuda shims
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "modarith.hpp"
+#include "ntt_device.hpp"
+
+namespace inspire {
+
+// ---------------------------------------------------------------------------
+// Synthetic fills (benchmark and test inputs; same generator as the oracle)
+// ---------------------------------------------------------------------------
+
+__global__ void k_synth_fill(void* __restrict__ out, uint64_t seed, uint32_t stream, uint64_t start,
+                             uint64_t count, int kind) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+    const uint64_t h = synth_word(seed, stream, start + i);
+    switch (kind) {
+      case kSynthCondensed: {
+        uint64_t lo = ((h & 0xffffffffull) * kQ0) >> 32;
+        uint64_t hi = ((h >> 32) * kQ1) >> 32;
+        ((uint64_t*)out)[i] = lo | (hi << 32);
+        break;
+      }
+      case kSynthModQ:
+        ((uint64_t*)out)[i] = mulhi64(h, kQ);
+        break;
+      case kSynthU16:
+        ((uint16_t*)out)[i] = (uint16_t)((h & 0xffffull) % 65535ull);
+        break;
+      case kSynthCrt: {
+        const int c = (int)(((start + i) / kN) & 1);
+        ((uint64_t*)out)[i] = ((h & 0xffffffffull) * prime(c)) >> 32;
+        break;
+      }
+      case kSynthSmall: {
+        uint64_t v = h & 0xfffffull;
+        ((uint64_t*)out)[i] = (v % kQ0) | ((v % kQ1) << 32);
+        break;
+      }
+      default:
+        break;
+    }
+  }
+}
+
+void launch_synth_fill(void* out, uint64_t seed, uint32_t stream, uint64_t start, uint64_t count, int kind,
+                       hipStream_t s) {
+  if (count == 0) return;
+  uint64_t blocks = (count + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_synth_fill, dim3((unsigned)blocks), dim3(256), 0, s, out, seed, stream, start, count,
+                     kind);
+}
+
+// ---------------------------------------------------------------------------
+// Query unpack (wire.rs:618-705).  Every value is a 56-bit field (28 bits per
+// CRT residue, wire.rs:120-125) starting at bit 8 + 56*f, i.e. at byte 1 + 7*f.
+// ---------------------------------------------------------------------------
+
+__global__ void k_unpack_query(const uint8_t* __restrict__ q, int nq, int db_rows, uint64_t* __restrict__ y_body,
+                               uint64_t* __restrict__ z_body, uint64_t* __restrict__ rows,
+                               uint32_t* __restrict__ bodies) {
+  const int64_t keys = (int64_t)kTExp * kN * 2;
+  const int64_t per_q = (int64_t)db_rows + 6 * kN;
+  const int64_t total = keys + (int64_t)nq * per_q;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const uint64_t m28 = (1ull << 28) - 1;
+  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < total; f += stride) {
+    const uint8_t* p = q + 1 + 7 * f;
+    uint64_t v = 0;
+#pragma unroll
+    for (int b = 0; b < 7; ++b) v |= (uint64_t)p[b] << (8 * b);
+    const uint64_t lo = v & m28, hi = (v >> 28) & m28;
+    if (f < keys) {
+      const int64_t k = f >> 1;  // (i, k) flattened: i*2048 + k
+      if (f & 1)
+        z_body[k] = lo | (hi << 32);
+      else
+        y_body[k] = lo | (hi << 32);
+    } else {
+      const int64_t g = f - keys;
+      const int64_t qi = g / per_q, r = g % per_q;
+      if (r < db_rows) {
+        rows[qi * db_rows + r] = lo | (hi << 32);
+      } else {
+        const int64_t e = r - db_rows;  // poly i = e / N, slot z = e % N
+        const int64_t i = e / kN, z = e % kN;
+        bodies[((qi * 6 + i) * 2 + 0) * kN + z] = (uint32_t)lo;
+        bodies[((qi * 6 + i) * 2 + 1) * kN + z] = (uint32_t)hi;
+      }
+    }
+  }
+}
+
+void launch_unpack_query(const uint8_t* q, int nq, int db_rows, uint64_t* y_body, uint64_t* z_body,
+                         uint64_t* rows, uint32_t* bodies, hipStream_t s) {
+  const int64_t total = (int64_t)kTExp * kN * 2 + (int64_t)nq * (db_rows + 6 * kN);
+  int blocks = (int)((total + 255) / 256);
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_unpack_query, dim3(blocks), dim3(256), 0, s, q, nq, db_rows, y_body, z_body, rows,
+                     bodies);
+}
+
+// ---------------------------------------------------------------------------
+// First pass: c[h][col] = sum_k a_h[k] * db[col][k] mod q (gemv_kernel.cu:46-103).
+// 8 lanes per DB column, 16 B per lane per load, 32 columns per 256-thread
+// block; the query rows are staged in LDS in chunks of 2048 rows.
+// ---------------------------------------------------------------------------
+
+constexpr int kGemvChunk = 2048;
+
+template <int NQ>
+__global__ __launch_bounds__(256) void k_gemv(uint64_t* __restrict__ c, const uint64_t* __restrict__ rows,
+                                              const uint16_t* __restrict__ db, int db_rows, int64_t db_cols) {
+  __shared__ uint32_t a_lds[NQ][2][kGemvChunk];
+  const int lane8 = threadIdx.x & 7;
+  const int64_t col = (int64_t)blockIdx.x * 32 + (threadIdx.x >> 3);
+  uint64_t acc[NQ][2];
+#pragma unroll
+  for (int h = 0; h < NQ; ++h) acc[h][0] = acc[h][1] = 0;
+
+  for (int k0 = 0; k0 < db_rows; k0 += kGemvChunk) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < NQ * kGemvChunk; t += 256) {
+      const int h = t / kGemvChunk, k = t % kGemvChunk;
+      const uint64_t v = rows[(int64_t)h * db_rows + k0 + k];
+      a_lds[h][0][k] = (uint32_t)v;
+      a_lds[h][1][k] = (uint32_t)(v >> 32);
+    }
+    __syncthreads();
+    if (col < db_cols) {
+      const uint4* p = reinterpret_cast<const uint4*>(db + col * db_rows + k0);
+#pragma unroll 4
+      for (int it = 0; it < kGemvChunk / 64; ++it) {
+        const uint4 v = p[it * 8 + lane8];
+        const int kb = it * 64 + lane8 * 8;
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int h = 0; h < NQ; ++h) {
+          const uint4 alo0 = *reinterpret_cast<const uint4*>(&a_lds[h][0][kb]);
+          const uint4 alo1 = *reinterpret_cast<const uint4*>(&a_lds[h][0][kb + 4]);
+          const uint4 ahi0 = *reinterpret_cast<const uint4*>(&a_lds[h][1][kb]);
+          const uint4 ahi1 = *reinterpret_cast<const uint4*>(&a_lds[h][1][kb + 4]);
+          const uint32_t lo[8] = {alo0.x, alo0.y, alo0.z, alo0.w, alo1.x, alo1.y, alo1.z, alo1.w};
+          const uint32_t hi[8] = {ahi0.x, ahi0.y, ahi0.z, ahi0.w, ahi1.x, ahi1.y, ahi1.z, ahi1.w};
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const uint32_t b = (w[e >> 1] >> (16 * (e & 1))) & 0xffffu;
+            acc[h][0] += (uint64_t)lo[e] * b;
+            acc[h][1] += (uint64_t)hi[e] * b;
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < NQ; ++h)
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int off = 4; off >= 1; off >>= 1) acc[h][r] += __shfl_xor(acc[h][r], off, 8);
+  if (lane8 == 0 && col < db_cols) {
+#pragma unroll
+    for (int h = 0; h < NQ; ++h) {
+      const uint32_t r0 = (uint32_t)barrett64(acc[h][0], kCr1Q0, kQ0);
+      const uint32_t r1 = (uint32_t)barrett64(acc[h][1], kCr1Q1, kQ1);
+      c[(int64_t)h * db_cols + col] = crt_compose(r0, r1);
+    }
+  }
+}
+
+int launch_gemv(uint64_t* c, const uint64_t* rows, int nq, const uint16_t* db, int db_rows, int64_t db_cols,
+                hipStream_t s) {
+  if (db_rows % kGemvChunk != 0 || db_rows <= 0 || db_cols <= 0) return -1;
+  const unsigned blocks = (unsigned)((db_cols + 31) / 32);
+  if (nq == 1)
+    hipLaunchKernelGGL(k_gemv<1>, dim3(blocks), dim3(256), 0, s, c, rows, db, db_rows, db_cols);
+  else if (nq == 2)
+    hipLaunchKernelGGL(k_gemv<2>, dim3(blocks), dim3(256), 0, s, c, rows, db, db_rows, db_cols);
+  else {
+    for (int h = 0; h < nq; ++h)
+      hipLaunchKernelGGL(k_gemv<1>, dim3(blocks), dim3(256), 0, s, c + (int64_t)h * db_cols,
+                         rows + (int64_t)h * db_rows, db, db_rows, db_cols);
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Packing inner product (packing_online_kernel.cu:18-92) with the key
+// rotations (rotation_kernel.cu:21-59) fused in.
+//
+// S[j][c][z] = sum_{i<1023,k<3} y_k[perm_{g_i}(z)] * T_j[3i+k][z]
+//            + y_k[perm_{2n-g_i}(z)] * Tbar_j[3i+k][z] + sum_k zb_k[z] * That_j[k][z]  (mod q_c)
+//
+// perm_t(z) is the automorphism table of packing.rs:1423-1468: the NTT slot
+// with exponent (2*rev(z)+1)*t mod 2n.  For t' = 2n - t that exponent is the
+// negation, whose slot is 2047 - perm_t(z), so both gathers share one index.
+//
+// Block = 4 waves; lanes own 64 consecutive slots z, the block owns JG outputs
+// j, and the waves split the 1023 rotations.  Partial sums meet in LDS.
+// ---------------------------------------------------------------------------
+
+template <int JG, bool FUSED>
+__global__ __launch_bounds__(256) void k_packing(uint32_t* __restrict__ S, const uint64_t* __restrict__ bt,
+                                                 const uint64_t* __restrict__ btb,
+                                                 const uint64_t* __restrict__ bth,
+                                                 const uint64_t* __restrict__ y_body,
+                                                 const uint64_t* __restrict__ z_body,
+                                                 const uint64_t* __restrict__ y_all,
+                                                 const uint64_t* __restrict__ y_bar_all) {
+  // y_body (48 KB) while accumulating; the same LDS then holds the partial sums.
+  constexpr int kPartWords = 4 * 2 * JG * 64;  // u32 residues
+  constexpr int kSmemU64 = FUSED ? kTExp * kN : kPartWords / 2;
+  static_assert(kPartWords / 2 <= kSmemU64, "partials must fit in the y_body buffer");
+  __shared__ uint64_t smem[kSmemU64];
+  uint64_t (*ylds)[kN] = reinterpret_cast<uint64_t (*)[kN]>(smem);
+  uint32_t (*part)[2][JG][64] = reinterpret_cast<uint32_t (*)[2][JG][64]>(smem);
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int z = blockIdx.x * 64 + lane;
+  const int j0 = blockIdx.y * JG;
+
+  if (FUSED) {
+    for (int t = threadIdx.x; t < kTExp * kN; t += 256) ylds[t / kN][t % kN] = y_body[t];
+    __syncthreads();
+  }
+
+  const int i_begin = wave * 256;
+  const int i_end = min(i_begin + 256, kNumRot);
+  // g = 5^i mod 4096 (packing.rs:1639-1652).
+  uint32_t g = 1;
+  for (int e = 0; e < i_begin; ++e) g = (g * 5u) & (2 * kN - 1);
+  const uint32_t ez = 2u * (__brev((uint32_t)z) >> 21) + 1u;
+
+  uint64_t acc_lo[JG], acc_hi[JG];
+#pragma unroll
+  for (int jj = 0; jj < JG; ++jj) acc_lo[jj] = acc_hi[jj] = 0;
+
+  const size_t jstride = (size_t)kNInner * kN;
+  const uint64_t* btj = bt + (size_t)j0 * jstride + z;
+  const uint64_t* btbj = btb + (size_t)j0 * jstride + z;
+
+  int since_reduce = 0;
+  for (int i = i_begin; i < i_end; ++i) {
+    uint64_t yv[kTExp], ybv[kTExp];
+    if (FUSED) {
+      const uint32_t e1 = (ez * g) & (2 * kN - 1);
+      const uint32_t s1 = __brev(e1 >> 1) >> 21;
+      const uint32_t s2 = (kN - 1) - s1;
+#pragma unroll
+      for (int k = 0; k < kTExp; ++k) {
+        yv[k] = ylds[k][s1];
+        ybv[k] = ylds[k][s2];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kTExp; ++k) {
+        yv[k] = y_all[((size_t)i * kTExp + k) * kN + z];
+        ybv[k] = y_bar_all[((size_t)i * kTExp + k) * kN + z];
+      }
+    }
+    g = (g * 5u) & (2 * kN - 1);
+#pragma unroll
+    for (int k = 0; k < kTExp; ++k) {
+      const size_t off = ((size_t)i * kTExp + k) * kN;
+      const uint32_t ylo = (uint32_t)yv[k], yhi = (uint32_t)(yv[k] >> 32);
+      const uint32_t yblo = (uint32_t)ybv[k], ybhi = (uint32_t)(ybv[k] >> 32);
+#pragma unroll
+      for (int jj = 0; jj < JG; ++jj) {
+        const uint64_t b = __builtin_nontemporal_load(btj + jj * jstride + off);
+        const uint64_t bb = __builtin_nontemporal_load(btbj + jj * jstride + off);
+        acc_lo[jj] += (uint64_t)ylo * (uint32_t)b + (uint64_t)yblo * (uint32_t)bb;
+        acc_hi[jj] += (uint64_t)yhi * (uint32_t)(b >> 32) + (uint64_t)ybhi * (uint32_t)(bb >> 32);
+      }
+    }
+    // 6 products < 2^56 per iteration: reduce well before 256 products.
+    if (++since_reduce == 32) {
+      since_reduce = 0;
+#pragma unroll
+      for (int jj = 0; jj < JG; ++jj) {
+        acc_lo[jj] = barrett64(acc_lo[jj], kCr1Q0, kQ0);
+        acc_hi[jj] = barrett64(acc_hi[jj], kCr1Q1, kQ1);
+      }
+    }
+  }
+  __syncthreads();  // every wave is done with y_body
+#pragma unroll
+  for (int jj = 0; jj < JG; ++jj) {
+    part[wave][0][jj][lane] = (uint32_t)barrett64(acc_lo[jj], kCr1Q0, kQ0);
+    part[wave][1][jj][lane] = (uint32_t)barrett64(acc_hi[jj], kCr1Q1, kQ1);
+  }
+  __syncthreads();
+  // Wave w finalises outputs jj = w, w+4, ...
+  for (int jj = wave; jj < JG; jj += 4) {
+    const int j = j0 + jj;
+    uint64_t lo = (uint64_t)part[0][0][jj][lane] + part[1][0][jj][lane] + part[2][0][jj][lane] +
+                  part[3][0][jj][lane];
+    uint64_t hi = (uint64_t)part[0][1][jj][lane] + part[1][1][jj][lane] + part[2][1][jj][lane] +
+                  part[3][1][jj][lane];
+#pragma unroll
+    for (int k = 0; k < kTExp; ++k) {
+      const uint64_t zv = z_body[k * kN + z];
+      const uint64_t hv = bth[((size_t)j * kTExp + k) * kN + z];
+      lo += (uint64_t)(uint32_t)zv * (uint32_t)hv;
+      hi += (uint64_t)(uint32_t)(zv >> 32) * (uint32_t)(hv >> 32);
+    }
+    S[((size_t)j * 2 + 0) * kN + z] = (uint32_t)barrett64(lo, kCr1Q0, kQ0);
+    S[((size_t)j * 2 + 1) * kN + z] = (uint32_t)barrett64(hi, kCr1Q1, kQ1);
+  }
+}
+
+template <bool FUSED>
+static void launch_packing_t(uint32_t* S, const uint64_t* bt, const uint64_t* btb, const uint64_t* bth,
+                             const uint64_t* y_body, const uint64_t* z_body, const uint64_t* y_all,
+                             const uint64_t* y_bar_all, int instances, hipStream_t s) {
+  const dim3 block(256);
+  if (instances % 8 == 0)
+    hipLaunchKernelGGL((k_packing<8, FUSED>), dim3(kN / 64, instances / 8), block, 0, s, S, bt, btb, bth, y_body,
+                       z_body, y_all, y_bar_all);
+  else if (instances % 4 == 0)
+    hipLaunchKernelGGL((k_packing<4, FUSED>), dim3(kN / 64, instances / 4), block, 0, s, S, bt, btb, bth, y_body,
+                       z_body, y_all, y_bar_all);
+  else if (instances % 2 == 0)
+    hipLaunchKernelGGL((k_packing<2, FUSED>), dim3(kN / 64, instances / 2), block, 0, s, S, bt, btb, bth, y_body,
+                       z_body, y_all, y_bar_all);
+  else
+    hipLaunchKernelGGL((k_packing<1, FUSED>), dim3(kN / 64, instances), block, 0, s, S, bt, btb, bth, y_body,
+                       z_body, y_all, y_bar_all);
+}
+
+void launch_packing(uint32_t* S, const uint64_t* bt, const uint64_t* btb, const uint64_t* bth,
+                    const uint64_t* y_body, const uint64_t* z_body, int instances, hipStream_t s) {
+  launch_packing_t<true>(S, bt, btb, bth, y_body, z_body, nullptr, nullptr, instances, s);
+}
+
+void launch_packing_materialized(uint32_t* S, const uint64_t* bt, const uint64_t* btb, const uint64_t* bth,
+                                 const uint64_t* y_all, const uint64_t* y_bar_all, const uint64_t* z_body,
+                                 int instances, hipStream_t s) {
+  launch_packing_t<false>(S, bt, btb, bth, nullptr, z_body, y_all, y_bar_all, instances, s);
+}
+
+// ---------------------------------------------------------------------------
+// Legacy rotation (rotation_kernel.cu:21-59): materialise y_all / y_bar_all
+// from uploaded automorphism tables.
+// ---------------------------------------------------------------------------
+
+__global__ void k_rotation_legacy(uint64_t* __restrict__ y_all, uint64_t* __restrict__ y_bar_all,
+                                  const uint64_t* __restrict__ y_body, const uint32_t* __restrict__ tables,
+                                  const uint32_t* __restrict__ gen_pows, int num_rotations, int t_exp,
+                                  int poly_len) {
+  const int rot = blockIdx.x, k = blockIdx.y;
+  if (rot >= num_rotations || k >= t_exp) return;
+  const uint32_t t = gen_pows[rot];
+  const size_t t1 = (size_t)((t - 1) / 2) * poly_len;
+  const size_t t2 = (size_t)((2 * poly_len - t - 1) / 2) * poly_len;
+  const size_t ob = ((size_t)rot * t_exp + k) * poly_len;
+  for (int z = threadIdx.x; z < poly_len; z += blockDim.x) {
+    y_all[ob + z] = y_body[(size_t)k * poly_len + tables[t1 + z]];
+    y_bar_all[ob + z] = y_body[(size_t)k * poly_len + tables[t2 + z]];
+  }
+}
+
+void launch_rotation_legacy(uint64_t* y_all, uint64_t* y_bar_all, const uint64_t* y_body, const uint32_t* tables,
+                            const uint32_t* gen_pows, int num_rotations, int t_exp, int poly_len,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(k_rotation_legacy, dim3(num_rotations, t_exp), dim3(256), 0, s, y_all, y_bar_all, y_body,
+                     tables, gen_pows, num_rotations, t_exp, poly_len);
+}
+
+// ---------------------------------------------------------------------------
+// Post-process (packing_online.rs:359-398): P_j = CRT(INTT(S_j)) mod q.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(1024) void k_post(uint64_t* __restrict__ P, const uint32_t* __restrict__ S,
+                                               const uint32_t* __restrict__ tw_tab) {
+  __shared__ uint32_t buf[2][kN];
+  const NttTables tw{tw_tab};
+  const int j = blockIdx.x;
+  for (int t = threadIdx.x; t < 2 * kN; t += 1024) buf[t / kN][t % kN] = S[(size_t)j * 2 * kN + t];
+  __syncthreads();
+  uint32_t* const polys[2] = {buf[0], buf[1]};
+  const int pr[2] = {0, 1};
+  ntt_inverse_multi<1024, 2>(polys, pr, tw);
+  for (int z = threadIdx.x; z < kN; z += 1024) P[(size_t)j * kN + z] = crt_compose(buf[0][z], buf[1][z]);
+}
+
+void launch_post(uint64_t* P, const uint32_t* S, const uint32_t* tw, int count, hipStream_t s) {
+  hipLaunchKernelGGL(k_post, dim3(count), dim3(1024), 0, s, P, S, tw);
+}
+
+// ---------------------------------------------------------------------------
+// RGSW fold + modulus switch (keyword.rs:543-581, modulus_switch.rs:45-73).
+//
+// One block per (hash query h, chain w).  Horner over i = d-1 .. 0:
+//   sum <- raw(ct_gsw * NTT(G^-1(sum))) + packed[w*d+i]  (mod q)
+// where ct_gsw row 0 is NTT(-a) in every column and row 1 is the client's
+// RGSW body.  The first step has sum = 0, so it reduces to sum = packed[w*d+d-1].
+// packed[j] = (a_hat_j, c_h[j*n..] + P_j).
+// ---------------------------------------------------------------------------
+
+constexpr int kFoldThreads = 1024;
+
+struct FoldArgs {
+  uint8_t* out;              // [nq][c][12288]
+  const uint64_t* a_hat;     // [instances][N]
+  const uint64_t* cvec;      // [nq][instances*N]
+  const uint64_t* P;         // [instances][N]
+  const uint32_t* bodies;    // [nq][6][2][N]
+  const uint32_t* neg_a;     // [2][N]
+  const uint32_t* tw;        // NTT tables
+  int64_t instances;
+  int d;
+  int c;
+};
+
+__global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
+  extern __shared__ uint32_t lds[];
+  uint32_t* dig = lds;                       // [2][6][N]
+  uint32_t* prod = lds + 2 * 6 * kN;         // [2 (prime)][2 (row)][N]
+  const NttTables tw{args.tw};
+  const int h = blockIdx.x / args.c;
+  const int w = blockIdx.x % args.c;
+  const int tid = threadIdx.x;
+  const int d = args.d;
+  const uint64_t* cv = args.cvec + (size_t)h * args.instances * kN;
+  const uint32_t* body = args.bodies + (size_t)h * 6 * 2 * kN;
+
+  // sum[r][zz], zz in {tid, tid+1024}
+  uint64_t sum[2][2];
+  {
+    const size_t j = (size_t)w * d + (d - 1);
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int z = tid + s * 1024;
+      sum[0][s] = args.a_hat[j * kN + z];
+      sum[1][s] = reduce_small_q(cv[j * kN + z] + args.P[j * kN + z]);
+    }
+  }
+
+  uint32_t* const fpolys[12] = {dig + 0 * kN, dig + 1 * kN, dig + 2 * kN, dig + 3 * kN, dig + 4 * kN, dig + 5 * kN,
+                                dig + 6 * kN, dig + 7 * kN, dig + 8 * kN, dig + 9 * kN, dig + 10 * kN,
+                                dig + 11 * kN};
+  const int fprime[12] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1};
+  uint32_t* const ipolys[4] = {prod + 0 * kN, prod + 1 * kN, prod + 2 * kN, prod + 3 * kN};
+  const int iprime[4] = {0, 0, 1, 1};
+  const uint64_t mask = (1ull << kGadgetBits) - 1;
+
+  for (int i = d - 2; i >= 0; --i) {
+    // G^-1(sum): digit k of row r goes to gadget row r + 2k (gadget.rs:35-61).
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int z = tid + s * 1024;
+#pragma unroll
+      for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const uint32_t dv = (uint32_t)((sum[r][s] >> (kGadgetBits * k)) & mask);
+          dig[(0 * 6 + r + 2 * k) * kN + z] = dv;
+          dig[(1 * 6 + r + 2 * k) * kN + z] = dv;
+        }
+    }
+    __syncthreads();
+    ntt_forward_multi<kFoldThreads, 12>(fpolys, fprime, tw);
+    // ct_gsw * G: row 0 = NTT(-a) * sum_k G_k; row 1 = sum_k body_k * G_k.
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int z = tid + s * 1024;
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        uint32_t gsum = 0;
+        uint64_t acc = 0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          const uint32_t gk = dig[(c * 6 + k) * kN + z];
+          gsum += gk;
+          acc += (uint64_t)body[(k * 2 + c) * kN + z] * gk;
+        }
+        const uint32_t q = prime(c);
+        uint32_t gs = gsum % q;
+        const uint32_t na = args.neg_a[c * kN + z];
+        uint32_t p0, p1;
+        if (c == 0) {
+          p0 = mulmod<0>(na, gs);
+        } else {
+          p0 = mulmod<1>(na, gs);
+        }
+        p1 = (uint32_t)barrett64(acc, cr1(c), q);
+        prod[(c * 2 + 0) * kN + z] = p0;
+        prod[(c * 2 + 1) * kN + z] = p1;
+      }
+    }
+    __syncthreads();
+    ntt_inverse_multi<kFoldThreads, 4>(ipolys, iprime, tw);
+    const size_t j = (size_t)w * d + i;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int z = tid + s * 1024;
+      const uint64_t x0 = crt_compose(prod[0 * kN + z], prod[2 * kN + z]);
+      const uint64_t x1 = crt_compose(prod[1 * kN + z], prod[3 * kN + z]);
+      sum[0][s] = reduce_small_q(x0 + args.a_hat[j * kN + z]);
+      sum[1][s] = reduce_small_q(x1 + cv[j * kN + z] + args.P[j * kN + z]);
+    }
+    // The next iteration writes dig only; prod is rewritten after two barriers.
+  }
+
+  // switch_and_keep(2^20, 268369921, 2048): row 0 -> 28 bits, row 1 -> 20 bits.
+  __syncthreads();
+  uint32_t* r0 = dig;
+  uint32_t* r1 = dig + kN;
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int z = tid + s * 1024;
+    r0[z] = (uint32_t)rescale_from_q(sum[0][s], kQPrime2);
+    r1[z] = (uint32_t)rescale_from_q(sum[1][s], kQPrime1);
+  }
+  __syncthreads();
+  uint32_t* out32 = reinterpret_cast<uint32_t*>(args.out + ((size_t)h * args.c + w) * kRespBytes);
+  constexpr int kRow0Bits = 28 * kN;
+  constexpr int kTotalBits = 28 * kN + 20 * kN;
+  for (int wd = tid; wd < kRespBytes / 4; wd += kFoldThreads) {
+    const int lo = wd * 32, hi = lo + 32;
+    uint32_t word = 0;
+    if (lo < kRow0Bits) {
+      const int vlast = (min(hi, kRow0Bits) - 1) / 28;
+      for (int v = lo / 28; v <= vlast; ++v) {
+        const int sh = 28 * v - lo;
+        const uint32_t val = r0[v];
+        word |= sh >= 0 ? (val << sh) : (val >> (-sh));
+      }
+    }
+    if (hi > kRow0Bits && lo < kTotalBits) {
+      const int blo = max(lo, kRow0Bits) - kRow0Bits;
+      const int bhi = min(hi, kTotalBits) - kRow0Bits;
+      const int vlast = (bhi - 1) / 20;
+      for (int v = blo / 20; v <= vlast; ++v) {
+        const int sh = kRow0Bits + 20 * v - lo;
+        const uint32_t val = r1[v];
+        word |= sh >= 0 ? (val << sh) : (val >> (-sh));
+      }
+    }
+    out32[wd] = word;
+  }
+}
+
+size_t fold_lds_bytes() { return (size_t)(2 * 6 * kN + 4 * kN) * sizeof(uint32_t); }
+
+int launch_fold(uint8_t* out, const uint64_t* a_hat, const uint64_t* cvec, const uint64_t* P,
+                const uint32_t* bodies, const uint32_t* neg_a, const uint32_t* tw, int64_t instances, int d,
+                int nq, hipStream_t s) {
+  if (d <= 0 || instances % d != 0) return -1;
+  FoldArgs a{out, a_hat, cvec, P, bodies, neg_a, tw, instances, d, (int)(instances / d)};
+  static bool attr_set = false;
+  if (!attr_set) {
+    // Raises the dynamic-LDS ceiling for this kernel (128 KB of the 160 KB per CU).
+    (void)hipFuncSetAttribute((const void*)k_fold, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)fold_lds_bytes());
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(k_fold, dim3(nq * a.c), dim3(kFoldThreads), fold_lds_bytes(), s, a);
+  return 0;
+}
+
+}  // namespace inspire

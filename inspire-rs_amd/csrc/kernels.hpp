@@ -1,0 +1,41 @@
+// Launch wrappers for the gfx950 kernels (kernels.hip).  All pointers are
+// device pointers; every launch is asynchronous on stream `s`.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+#include <hip/hip_runtime.h>
+
+namespace inspire {
+
+enum SynthKind { kSynthCondensed = 0, kSynthModQ = 1, kSynthU16 = 2, kSynthCrt = 3, kSynthSmall = 4 };
+
+void launch_synth_fill(void* out, uint64_t seed, uint32_t stream, uint64_t start, uint64_t count, int kind,
+                       hipStream_t s);
+
+void launch_unpack_query(const uint8_t* q, int nq, int db_rows, uint64_t* y_body, uint64_t* z_body,
+                         uint64_t* rows, uint32_t* bodies, hipStream_t s);
+
+// c: [nq][db_cols] canonical mod q; rows: [nq][db_rows] condensed; db: [db_cols][db_rows].
+int launch_gemv(uint64_t* c, const uint64_t* rows, int nq, const uint16_t* db, int db_rows, int64_t db_cols,
+                hipStream_t s);
+
+// S: [instances][2][N] u32 canonical residues.
+void launch_packing(uint32_t* S, const uint64_t* bt, const uint64_t* btb, const uint64_t* bth,
+                    const uint64_t* y_body, const uint64_t* z_body, int instances, hipStream_t s);
+void launch_packing_materialized(uint32_t* S, const uint64_t* bt, const uint64_t* btb, const uint64_t* bth,
+                                 const uint64_t* y_all, const uint64_t* y_bar_all, const uint64_t* z_body,
+                                 int instances, hipStream_t s);
+void launch_rotation_legacy(uint64_t* y_all, uint64_t* y_bar_all, const uint64_t* y_body, const uint32_t* tables,
+                            const uint32_t* gen_pows, int num_rotations, int t_exp, int poly_len,
+                            hipStream_t s);
+
+// P: [count][N] canonical mod q.
+void launch_post(uint64_t* P, const uint32_t* S, const uint32_t* tw, int count, hipStream_t s);
+
+size_t fold_lds_bytes();
+// out: [nq][instances/d][12288] response blobs.
+int launch_fold(uint8_t* out, const uint64_t* a_hat, const uint64_t* cvec, const uint64_t* P,
+                const uint32_t* bodies, const uint32_t* neg_a, const uint32_t* tw, int64_t instances, int d,
+                int nq, hipStream_t s);
+
+}  // namespace inspire

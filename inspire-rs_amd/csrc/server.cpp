@@ -1,0 +1,553 @@
+// Handle-based answer path: KeywordServer::respond (src/pir/server/keyword.rs:534-628)
+// with every stage on the GPU, plus the setup plan (keyword.rs:91-233) and the
+// stage entry points declared in include/inspire_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/inspire_hip.h"
+#include "host_tables.hpp"
+#include "kernels.hpp"
+#include "modarith.hpp"
+#include "runtime.hpp"
+
+using namespace inspire;
+
+// ---------------------------------------------------------------------------
+// Errors
+// ---------------------------------------------------------------------------
+
+namespace inspire {
+thread_local std::string g_last_error;
+int set_error(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+int hip_error(hipError_t e, const char* what) {
+  return set_error(INSPIRE_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+}  // namespace inspire
+
+extern "C" const char* inspire_last_error(void) { return g_last_error.c_str(); }
+
+#define HIP_TRY(call, what)                         \
+  do {                                              \
+    hipError_t e_ = (call);                         \
+    if (e_ != hipSuccess) return hip_error(e_, what); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Per-device constants: NTT twiddles and NTT(-a) for the RGSW row 0.
+// ---------------------------------------------------------------------------
+
+namespace inspire {
+
+static std::mutex g_const_mu;
+static DeviceConstants g_consts[64];
+
+const DeviceConstants* device_constants() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  std::lock_guard<std::mutex> lk(g_const_mu);
+  DeviceConstants& dc = g_consts[dev];
+  if (dc.tw) return &dc;
+  std::vector<uint32_t> tw(2 * 4 * kN), na(2 * kN);
+  build_ntt_tables(tw.data());
+  build_rgsw_neg_a(na.data(), tw.data());
+  uint32_t *dtw = nullptr, *dna = nullptr;
+  if (hipMalloc(&dtw, tw.size() * 4) != hipSuccess) return nullptr;
+  if (hipMalloc(&dna, na.size() * 4) != hipSuccess) {
+    (void)hipFree(dtw);
+    return nullptr;
+  }
+  if (hipMemcpy(dtw, tw.data(), tw.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dna, na.data(), na.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(dtw);
+    (void)hipFree(dna);
+    return nullptr;
+  }
+  dc.neg_a = dna;
+  dc.tw = dtw;
+  return &dc;
+}
+
+}  // namespace inspire
+
+// ---------------------------------------------------------------------------
+// Setup plan (keyword.rs:145-233, wire.rs:386-477)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// wire.rs:386-412
+int64_t max_interpolate_degree(double modulus, double d0, double p, double t_exp, double t_rgsw,
+                               double poly_len, bool* ok) {
+  const double sigma_x = 6.4;
+  const double modulus_len = std::log2(modulus);
+  const double z1 = std::pow(2.0, std::ceil(modulus_len / t_exp));
+  const double z2 = std::pow(2.0, std::ceil(modulus_len / t_rgsw));
+  const double s2 = sigma_x * sigma_x;
+  const double term1 = d0 * (p * p) * s2;
+  const double term2 = t_exp * (poly_len * poly_len) * (z1 * z1) * s2 / 4.0;
+  const double term3 = t_rgsw * poly_len * (z2 * z2) * s2 / 2.0;
+  const double total = std::log2(std::sqrt(term1 + term2 + term3));
+  const double ub = std::log2(modulus / (2. * 2. * p)) - std::log2(std::sqrt(2.0 * 41.0 * std::log(2.0)));
+  const double maxlog = 2. * (ub - total);
+  *ok = maxlog >= 0.;
+  const int64_t m = (int64_t)std::floor(std::pow(2.0, maxlog));
+  return m < (int64_t)poly_len ? m : (int64_t)poly_len;
+}
+
+int64_t ceil_div_f(double a, double b) { return (int64_t)std::ceil(a / b); }
+
+struct RgswParams {
+  int64_t db_rows, instances, interpolate_degree;
+};
+
+// wire.rs:414-477 params_rgswpir_given_input_size_and_dim0
+int rgsw_params(int64_t num_items, int64_t item_size_bits, int64_t dim0, RgswParams* out) {
+  const int64_t poly_len = 2048, log_p = 16;
+  const double modulus = (double)kQ;
+  bool ok = true;
+  const int64_t max_interp = max_interpolate_degree(modulus, (double)dim0, 65535.0, 3.0, 3.0, 2048.0, &ok);
+  if (!ok || max_interp <= 1) return set_error(INSPIRE_ERR_INVALID, "dim0 too large for the noise bound");
+  const double bits_per_poly = (double)(poly_len * log_p);
+  int64_t factor = (int64_t)std::floor(bits_per_poly / (double)item_size_bits);
+  if (factor == 0) factor = 1;
+  const int64_t n_items = ceil_div_f((double)num_items, (double)factor);
+  const int64_t item_bits = item_size_bits * factor;
+  const int64_t padded_bits = ceil_div_f((double)item_bits, bits_per_poly) * (int64_t)bits_per_poly;
+  const int64_t padded_pts = ceil_div_f((double)padded_bits, (double)log_p);
+  const int64_t dim1_lb = padded_pts * ceil_div_f((double)n_items, (double)dim0);
+  int64_t cur = padded_pts, interp = 1;
+  while (cur < dim1_lb) {
+    interp *= 2;
+    cur *= 2;
+    if (2 * interp > max_interp) break;
+  }
+  const int64_t new_pts = ceil_div_f((double)dim1_lb, (double)cur) * cur;
+  const int64_t db_rows_in = dim0 >= 8 ? dim0 : 8;
+  const int64_t db_cols_poly = ceil_div_f((double)new_pts, (double)poly_len);
+  int64_t np2 = 1;
+  while (np2 < db_rows_in) np2 <<= 1;
+  out->db_rows = np2;  // 1 << (nu_1 + 11) with nu_1 = log2(next_pow2(db_rows)) - 11
+  if (np2 < 2048) return set_error(INSPIRE_ERR_INVALID, "dim0 must be at least 2048");
+  out->instances = db_cols_poly;
+  out->interpolate_degree = interp;
+  return INSPIRE_OK;
+}
+
+}  // namespace
+
+extern "C" int inspire_plan_shape(int64_t record_count, int64_t key_size, int64_t value_size, int64_t buckets,
+                                  int64_t dim0, inspire_plan_t* out) {
+  if (!out) return set_error(INSPIRE_ERR_INVALID, "null plan");
+  if (record_count <= 0)
+    return set_error(INSPIRE_ERR_INVALID, "keyword record set must contain at least one record");
+  if (key_size <= 0) return set_error(INSPIRE_ERR_INVALID, "keyword key size must be non-zero");
+  if (value_size <= 0) return set_error(INSPIRE_ERR_INVALID, "keyword value size must be non-zero");
+  int64_t num_items = buckets;
+  if (num_items <= 0) {  // default_bucket_count (keyword.rs:29-31)
+    int64_t m = (record_count > 512 ? record_count : 512) * 2, p = 1;
+    while (p < m) p <<= 1;
+    num_items = p;
+  }
+  if (num_items < record_count)
+    return set_error(INSPIRE_ERR_INVALID, "bucket count " + std::to_string(num_items) +
+                                              " must be >= record count " + std::to_string(record_count));
+  const int64_t entry_size = key_size + value_size;
+  const int64_t item_size_bits = entry_size * 8;
+  if (item_size_bits % 16 != 0) return set_error(INSPIRE_ERR_INVALID, "entry size must be a multiple of 16 bits");
+  RgswParams rp;
+  if (dim0 <= 0) {
+    const int64_t def = num_items >= 256000000 ? 32768 : 2048;
+    int rc = rgsw_params(num_items, item_size_bits, def, &rp);
+    if (rc) return rc;
+    dim0 = def >= 8 ? def : 8;  // the derived db_rows of the default dim0
+  }
+  int rc = rgsw_params(num_items, item_size_bits, dim0, &rp);
+  if (rc) return rc;
+  const int64_t db_rows = rp.db_rows;
+  const int64_t db_cols = rp.instances * 2048;
+  if (num_items < db_rows) {
+    const int64_t min_records = db_rows / 2 + 1;
+    return set_error(INSPIRE_ERR_INVALID, "bucket count " + std::to_string(num_items) +
+                                              " is too small for the PIR scheme (needs at least " +
+                                              std::to_string(db_rows) + "); use at least " +
+                                              std::to_string(min_records) + " records, or pass --buckets " +
+                                              std::to_string(db_rows));
+  }
+  out->num_items = num_items;
+  out->item_size_bits = item_size_bits;
+  out->dim0 = dim0;
+  out->db_rows = db_rows;
+  out->db_cols = db_cols;
+  out->instances = rp.instances;
+  out->interpolate_degree = rp.interpolate_degree;
+  out->c = rp.instances / rp.interpolate_degree;
+  // KeywordServerGpuMemoryEstimate::from_plan (keyword.rs:104-142)
+  const uint64_t poly_len = 2048, crt = 2, t = 3, nphalf = 1024;
+  const uint64_t coeff_count = crt * poly_len;
+  out->db_bytes = (uint64_t)db_rows * db_cols * 2;
+  out->hint_bytes = (uint64_t)db_cols * poly_len * 8;
+  out->prep_pack_bytes = (uint64_t)db_cols * coeff_count * 8;
+  out->r_buffers_bytes = 2 * (uint64_t)rp.instances * nphalf * coeff_count * 8;
+  out->bold_t_bytes = 2 * (uint64_t)rp.instances * (nphalf - 1) * t * poly_len * 8;
+  out->peak_bytes = out->db_bytes + out->prep_pack_bytes + out->r_buffers_bytes + out->bold_t_bytes;
+  return INSPIRE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Server
+// ---------------------------------------------------------------------------
+
+struct inspire_server {
+  inspire_shape_t shape{};
+  int64_t db_cols = 0;
+  int c = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  const DeviceConstants* consts = nullptr;
+  // resident state
+  uint16_t* db = nullptr;
+  uint64_t *bt = nullptr, *btb = nullptr, *bth = nullptr, *a_hat = nullptr;
+  bool have_db = false, have_precomp = false;
+  std::vector<uint8_t> stash;  // entries back to back
+  uint32_t stash_entry = 0, stash_count = 0;
+  // request buffers
+  size_t max_query_bytes = 0;
+  uint8_t* qbytes = nullptr;
+  uint64_t *y_body = nullptr, *z_body = nullptr, *rows = nullptr, *P = nullptr, *cvec = nullptr;
+  uint32_t *bodies = nullptr, *S = nullptr;
+  uint8_t* out = nullptr;
+  std::vector<uint8_t> host_out;
+  // optional per-stage timing
+  bool profile = false;
+  hipEvent_t ev[6] = {};
+  double stage_ms[5] = {0, 0, 0, 0, 0};
+  int64_t profiled = 0;
+};
+
+namespace {
+
+size_t query_bytes(int64_t db_rows, int nq) {
+  return 1 + 7 * (size_t)((int64_t)kTExp * kN * 2 + (int64_t)nq * (db_rows + 6 * kN));
+}
+
+template <typename T>
+int dev_alloc(T** p, size_t count, const char* what) {
+  hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return set_error(INSPIRE_ERR_NOMEM, std::string("hipMalloc failed for ") + what + ": " + hipGetErrorString(e));
+  }
+  return INSPIRE_OK;
+}
+
+}  // namespace
+
+extern "C" int inspire_server_create(const inspire_shape_t* shape, inspire_server_t** out) {
+  if (!shape || !out) return set_error(INSPIRE_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (shape->db_rows < 2048 || shape->db_rows % 2048 != 0 || shape->db_rows > (1 << 20))
+    return set_error(INSPIRE_ERR_INVALID, "db_rows must be a multiple of 2048 (at most 2^20)");
+  if (shape->instances <= 0 || shape->interpolate_degree <= 0 ||
+      shape->instances % shape->interpolate_degree != 0)
+    return set_error(INSPIRE_ERR_INVALID, "instances must be a positive multiple of interpolate_degree");
+  if (shape->max_queries <= 0 || shape->max_queries > 255)
+    return set_error(INSPIRE_ERR_INVALID, "max_queries must be in [1, 255]");
+  auto* s = new inspire_server();
+  s->shape = *shape;
+  s->db_cols = shape->instances * kN;
+  s->c = (int)(shape->instances / shape->interpolate_degree);
+  (void)hipGetDevice(&s->device);
+  s->consts = device_constants();
+  if (!s->consts) {
+    delete s;
+    return set_error(INSPIRE_ERR_HIP, "failed to build device constants");
+  }
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete s;
+    return set_error(INSPIRE_ERR_HIP, "hipStreamCreate failed");
+  }
+  const int64_t inst = shape->instances, rows = shape->db_rows, mq = shape->max_queries;
+  s->max_query_bytes = query_bytes(rows, (int)mq);
+  int rc = INSPIRE_OK;
+  if (!rc) rc = dev_alloc(&s->qbytes, s->max_query_bytes + 16, "query bytes");
+  if (!rc) rc = dev_alloc(&s->y_body, (size_t)kTExp * kN, "y_body");
+  if (!rc) rc = dev_alloc(&s->z_body, (size_t)kTExp * kN, "z_body");
+  if (!rc) rc = dev_alloc(&s->rows, (size_t)mq * rows, "query rows");
+  if (!rc) rc = dev_alloc(&s->bodies, (size_t)mq * 6 * 2 * kN, "rgsw bodies");
+  if (!rc) rc = dev_alloc(&s->S, (size_t)inst * 2 * kN, "packing output");
+  if (!rc) rc = dev_alloc(&s->P, (size_t)inst * kN, "post output");
+  if (!rc) rc = dev_alloc(&s->cvec, (size_t)mq * s->db_cols, "gemv output");
+  if (!rc) rc = dev_alloc(&s->out, (size_t)mq * s->c * kRespBytes, "responses");
+  if (!rc) rc = dev_alloc(&s->db, (size_t)rows * s->db_cols, "database");
+  if (!rc) rc = dev_alloc(&s->bt, (size_t)inst * kNInner * kN, "bold_t");
+  if (!rc) rc = dev_alloc(&s->btb, (size_t)inst * kNInner * kN, "bold_t_bar");
+  if (!rc) rc = dev_alloc(&s->bth, (size_t)inst * kTExp * kN, "bold_t_hat");
+  if (!rc) rc = dev_alloc(&s->a_hat, (size_t)inst * kN, "a_hat");
+  if (rc) {
+    std::string msg = g_last_error;
+    inspire_server_destroy(s);
+    return set_error(rc, msg);
+  }
+  s->host_out.resize((size_t)mq * s->c * kRespBytes);
+  *out = s;
+  return INSPIRE_OK;
+}
+
+extern "C" void inspire_server_destroy(inspire_server_t* s) {
+  if (!s) return;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  void* ptrs[] = {s->qbytes, s->y_body, s->z_body, s->rows, s->bodies, s->S, s->P, s->cvec, s->out,
+                  s->db,     s->bt,     s->btb,    s->bth,  s->a_hat};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  for (auto& e : s->ev)
+    if (e) (void)hipEventDestroy(e);
+  delete s;
+}
+
+extern "C" void* inspire_server_stream(inspire_server_t* s) { return s ? (void*)s->stream : nullptr; }
+
+extern "C" int inspire_server_profile_enable(inspire_server_t* s, int enable) {
+  if (!s) return set_error(INSPIRE_ERR_INVALID, "null server");
+  if (enable && !s->ev[0]) {
+    for (auto& e : s->ev) HIP_TRY(hipEventCreate(&e), "hipEventCreate");
+  }
+  s->profile = enable != 0;
+  return INSPIRE_OK;
+}
+
+extern "C" int inspire_server_profile_read(inspire_server_t* s, double* ms5, int64_t* count, int reset) {
+  if (!s || !ms5 || !count) return set_error(INSPIRE_ERR_INVALID, "null argument");
+  for (int i = 0; i < 5; ++i) ms5[i] = s->stage_ms[i];
+  *count = s->profiled;
+  if (reset) {
+    for (double& v : s->stage_ms) v = 0;
+    s->profiled = 0;
+  }
+  return INSPIRE_OK;
+}
+
+extern "C" int inspire_server_set_db(inspire_server_t* s, const uint16_t* db, int host) {
+  if (!s || !db) return set_error(INSPIRE_ERR_INVALID, "null argument");
+  const size_t bytes = (size_t)s->shape.db_rows * s->db_cols * sizeof(uint16_t);
+  HIP_TRY(hipMemcpyAsync(s->db, db, bytes, host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, s->stream),
+          "copy database");
+  HIP_TRY(hipStreamSynchronize(s->stream), "copy database");
+  s->have_db = true;
+  return INSPIRE_OK;
+}
+
+extern "C" int inspire_server_set_precomp(inspire_server_t* s, const uint64_t* bt, const uint64_t* btb,
+                                          const uint64_t* bth, const uint64_t* a_hat, int host) {
+  if (!s || !bt || !btb || !bth || !a_hat) return set_error(INSPIRE_ERR_INVALID, "null argument");
+  const size_t inst = (size_t)s->shape.instances;
+  const hipMemcpyKind k = host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+  HIP_TRY(hipMemcpyAsync(s->bt, bt, inst * kNInner * kN * 8, k, s->stream), "copy bold_t");
+  HIP_TRY(hipMemcpyAsync(s->btb, btb, inst * kNInner * kN * 8, k, s->stream), "copy bold_t_bar");
+  HIP_TRY(hipMemcpyAsync(s->bth, bth, inst * kTExp * kN * 8, k, s->stream), "copy bold_t_hat");
+  HIP_TRY(hipMemcpyAsync(s->a_hat, a_hat, inst * kN * 8, k, s->stream), "copy a_hat");
+  HIP_TRY(hipStreamSynchronize(s->stream), "copy precomp");
+  s->have_precomp = true;
+  return INSPIRE_OK;
+}
+
+extern "C" int inspire_server_set_stash(inspire_server_t* s, const uint8_t* entries, uint32_t entry_size,
+                                        uint32_t count) {
+  if (!s || (count && !entries)) return set_error(INSPIRE_ERR_INVALID, "null argument");
+  s->stash.assign(entries, entries + (size_t)entry_size * count);
+  s->stash_entry = entry_size;
+  s->stash_count = count;
+  return INSPIRE_OK;
+}
+
+// Stream ids of the synthetic generator (the oracle uses the same numbers).
+enum { kStreamDb = 1, kStreamBt = 2, kStreamBtb = 3, kStreamBth = 4, kStreamAhat = 5 };
+
+extern "C" int inspire_server_fill_synthetic(inspire_server_t* s, uint64_t seed) {
+  if (!s) return set_error(INSPIRE_ERR_INVALID, "null server");
+  const uint64_t inst = (uint64_t)s->shape.instances;
+  launch_synth_fill(s->db, seed, kStreamDb, 0, (uint64_t)s->shape.db_rows * s->db_cols, kSynthU16, s->stream);
+  launch_synth_fill(s->bt, seed, kStreamBt, 0, inst * kNInner * kN, kSynthCondensed, s->stream);
+  launch_synth_fill(s->btb, seed, kStreamBtb, 0, inst * kNInner * kN, kSynthCondensed, s->stream);
+  launch_synth_fill(s->bth, seed, kStreamBth, 0, inst * kTExp * kN, kSynthCondensed, s->stream);
+  launch_synth_fill(s->a_hat, seed, kStreamAhat, 0, inst * kN, kSynthModQ, s->stream);
+  HIP_TRY(hipGetLastError(), "synthetic fill launch");
+  HIP_TRY(hipStreamSynchronize(s->stream), "synthetic fill");
+  s->have_db = s->have_precomp = true;
+  return INSPIRE_OK;
+}
+
+extern "C" size_t inspire_server_response_bytes(const inspire_server_t* s, int nq) {
+  if (!s) return 0;
+  size_t n = 4 + (size_t)nq * (4 + (size_t)s->c * kRespBytes) + 4;
+  n += (size_t)s->stash_count * (4 + s->stash_entry);
+  return n;
+}
+
+namespace {
+
+// The respond pipeline on device (keyword.rs:534-628).  The packing inner
+// product depends only on the client's packing keys
+// (packing_online_kernel.cu:46-91), so it runs once per keyword query and
+// serves every hash query; the reference repeats it per hash query with the
+// same result.
+int run_pipeline(inspire_server_t* s, const uint8_t* d_query, int nq, uint8_t* d_out, hipStream_t st) {
+  const int64_t rows = s->shape.db_rows, inst = s->shape.instances;
+  auto mark = [&](int i) {
+    if (s->profile) (void)hipEventRecord(s->ev[i], st);
+  };
+  mark(0);
+  launch_unpack_query(d_query, nq, (int)rows, s->y_body, s->z_body, s->rows, s->bodies, st);
+  mark(1);
+  launch_packing(s->S, s->bt, s->btb, s->bth, s->y_body, s->z_body, (int)inst, st);
+  mark(2);
+  launch_post(s->P, s->S, s->consts->tw, (int)inst, st);
+  mark(3);
+  if (launch_gemv(s->cvec, s->rows, nq, s->db, (int)rows, s->db_cols, st))
+    return set_error(INSPIRE_ERR_INVALID, "unsupported GEMV shape");
+  mark(4);
+  if (launch_fold(d_out, s->a_hat, s->cvec, s->P, s->bodies, s->consts->neg_a, s->consts->tw, inst,
+                  s->shape.interpolate_degree, nq, st))
+    return set_error(INSPIRE_ERR_INVALID, "unsupported fold shape");
+  mark(5);
+  HIP_TRY(hipGetLastError(), "respond launch");
+  if (s->profile) {
+    HIP_TRY(hipEventSynchronize(s->ev[5]), "profile sync");
+    for (int i = 0; i < 5; ++i) {
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, s->ev[i], s->ev[i + 1]), "profile elapsed");
+      s->stage_ms[i] += ms;
+    }
+    ++s->profiled;
+  }
+  return INSPIRE_OK;
+}
+
+}  // namespace
+
+extern "C" int inspire_respond_device(inspire_server_t* s, const uint8_t* d_query, size_t len, int nq,
+                                      uint8_t* d_out, void* stream) {
+  if (!s || !d_query || !d_out) return set_error(INSPIRE_ERR_INVALID, "null argument");
+  if (!s->have_db || !s->have_precomp) return set_error(INSPIRE_ERR_STATE, "database or precomputation not loaded");
+  if (nq < 0 || nq > s->shape.max_queries) return set_error(INSPIRE_ERR_INVALID, "too many hash queries");
+  if (len < query_bytes(s->shape.db_rows, nq)) return set_error(INSPIRE_ERR_INVALID, "keyword query truncated");
+  if (nq == 0) return INSPIRE_OK;
+  hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+  return run_pipeline(s, d_query, nq, d_out, st);
+}
+
+extern "C" int inspire_respond_serialized(inspire_server_t* s, const uint8_t* query, size_t len, uint8_t* resp,
+                                          size_t cap, size_t* resp_len) {
+  if (!s || !query || !resp_len) return set_error(INSPIRE_ERR_INVALID, "null argument");
+  if (!s->have_db || !s->have_precomp) return set_error(INSPIRE_ERR_STATE, "database or precomputation not loaded");
+  if (len < 1) return set_error(INSPIRE_ERR_INVALID, "empty keyword query");
+  const int nq = query[0];
+  if (nq > s->shape.max_queries) return set_error(INSPIRE_ERR_INVALID, "too many hash queries");
+  const size_t need = query_bytes(s->shape.db_rows, nq);
+  if (len < need) return set_error(INSPIRE_ERR_INVALID, "keyword query truncated");
+  const size_t out_bytes = inspire_server_response_bytes(s, nq);
+  *resp_len = out_bytes;
+  if (!resp || cap < out_bytes) return set_error(INSPIRE_ERR_CAPACITY, "response buffer too small");
+  const size_t blob = (size_t)s->c * kRespBytes;
+  if (nq > 0) {
+    HIP_TRY(hipMemcpyAsync(s->qbytes, query, need, hipMemcpyHostToDevice, s->stream), "upload query");
+    int rc = run_pipeline(s, s->qbytes, nq, s->out, s->stream);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(s->host_out.data(), s->out, (size_t)nq * blob, hipMemcpyDeviceToHost, s->stream),
+            "download responses");
+    HIP_TRY(hipStreamSynchronize(s->stream), "respond");
+  }
+  // serialize_keyword_response (wire.rs:708-726)
+  size_t o = 0;
+  auto put32 = [&](uint32_t v) {
+    std::memcpy(resp + o, &v, 4);
+    o += 4;
+  };
+  put32((uint32_t)nq);
+  for (int h = 0; h < nq; ++h) {
+    put32((uint32_t)blob);
+    std::memcpy(resp + o, s->host_out.data() + (size_t)h * blob, blob);
+    o += blob;
+  }
+  put32(s->stash_count);
+  for (uint32_t i = 0; i < s->stash_count; ++i) {
+    put32(s->stash_entry);
+    std::memcpy(resp + o, s->stash.data() + (size_t)i * s->stash_entry, s->stash_entry);
+    o += s->stash_entry;
+  }
+  return INSPIRE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Stage entry points
+// ---------------------------------------------------------------------------
+
+extern "C" int inspire_synth_fill(void* d_out, uint64_t seed, uint32_t stream_id, uint64_t start, uint64_t count,
+                                  int kind, void* stream) {
+  if (!d_out || kind < 0 || kind > 4) return set_error(INSPIRE_ERR_INVALID, "bad synth_fill argument");
+  launch_synth_fill(d_out, seed, stream_id, start, count, kind, (hipStream_t)stream);
+  HIP_TRY(hipGetLastError(), "synth_fill launch");
+  return INSPIRE_OK;
+}
+
+extern "C" int inspire_unpack_query_device(const uint8_t* d_query, int nq, int64_t db_rows, uint64_t* d_y_body,
+                                           uint64_t* d_z_body, uint64_t* d_rows, uint32_t* d_bodies,
+                                           void* stream) {
+  if (!d_query || !d_y_body || !d_z_body || (nq > 0 && (!d_rows || !d_bodies)) || nq < 0 || db_rows <= 0)
+    return set_error(INSPIRE_ERR_INVALID, "bad unpack argument");
+  launch_unpack_query(d_query, nq, (int)db_rows, d_y_body, d_z_body, d_rows, d_bodies, (hipStream_t)stream);
+  HIP_TRY(hipGetLastError(), "unpack launch");
+  return INSPIRE_OK;
+}
+
+extern "C" int inspire_gemv_device(uint64_t* d_c, const uint64_t* d_rows, int nq, const uint16_t* d_db,
+                                   int64_t db_rows, int64_t db_cols, void* stream) {
+  if (!d_c || !d_rows || !d_db || nq <= 0) return set_error(INSPIRE_ERR_INVALID, "bad gemv argument");
+  if (db_rows > (1 << 20)) return set_error(INSPIRE_ERR_INVALID, "db_rows too large");
+  if (launch_gemv(d_c, d_rows, nq, d_db, (int)db_rows, db_cols, (hipStream_t)stream))
+    return set_error(INSPIRE_ERR_INVALID, "db_rows must be a positive multiple of 2048");
+  HIP_TRY(hipGetLastError(), "gemv launch");
+  return INSPIRE_OK;
+}
+
+extern "C" int inspire_packing_device(uint32_t* d_S, const uint64_t* d_bt, const uint64_t* d_btb,
+                                      const uint64_t* d_bth, const uint64_t* d_y_body, const uint64_t* d_z_body,
+                                      int64_t instances, void* stream) {
+  if (!d_S || !d_bt || !d_btb || !d_bth || !d_y_body || !d_z_body || instances <= 0 || instances > (1 << 20))
+    return set_error(INSPIRE_ERR_INVALID, "bad packing argument");
+  launch_packing(d_S, d_bt, d_btb, d_bth, d_y_body, d_z_body, (int)instances, (hipStream_t)stream);
+  HIP_TRY(hipGetLastError(), "packing launch");
+  return INSPIRE_OK;
+}
+
+extern "C" int inspire_post_device(uint64_t* d_P, const uint32_t* d_S, int64_t count, void* stream) {
+  if (!d_P || !d_S || count <= 0) return set_error(INSPIRE_ERR_INVALID, "bad post argument");
+  const DeviceConstants* dc = device_constants();
+  if (!dc) return set_error(INSPIRE_ERR_HIP, "device constants");
+  launch_post(d_P, d_S, dc->tw, (int)count, (hipStream_t)stream);
+  HIP_TRY(hipGetLastError(), "post launch");
+  return INSPIRE_OK;
+}
+
+extern "C" int inspire_fold_device(uint8_t* d_out, const uint64_t* d_a_hat, const uint64_t* d_c,
+                                   const uint64_t* d_P, const uint32_t* d_bodies, int64_t instances, int d,
+                                   int nq, void* stream) {
+  if (!d_out || !d_a_hat || !d_c || !d_P || !d_bodies || nq <= 0)
+    return set_error(INSPIRE_ERR_INVALID, "bad fold argument");
+  const DeviceConstants* dc = device_constants();
+  if (!dc) return set_error(INSPIRE_ERR_HIP, "device constants");
+  if (launch_fold(d_out, d_a_hat, d_c, d_P, d_bodies, dc->neg_a, dc->tw, instances, d, nq, (hipStream_t)stream))
+    return set_error(INSPIRE_ERR_INVALID, "instances must be a positive multiple of d");
+  HIP_TRY(hipGetLastError(), "fold launch");
+  return INSPIRE_OK;
+}

@@ -1,0 +1,243 @@
+"""Python binding of libinspire_hip.so (the C ABI in include/inspire_hip.h).
+
+The answer path of the inspire-rs keyword-PIR server
+(src/pir/server/keyword.rs:534-628) runs in hand-written gfx950 kernels; this
+module only loads the library and marshals pointers.  Device buffers are torch
+tensors (plumbing); every computation happens in the HIP library.  There is no
+CPU fallback: loading fails loudly when the library has not been built.
+"""
+import ctypes
+import os
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libinspire_hip.so")
+HEADER_PATH = os.path.join(os.path.dirname(PKG_DIR), "include", "inspire_hip.h")
+
+N = 2048
+TEXP = 3
+NINNER = 3069
+RESP_BYTES = 12288
+
+OK, ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_STATE, ERR_CAPACITY = 0, -1, -2, -3, -4, -5
+
+# Synthetic generator kinds and stream ids (shared with the oracle).
+KIND_CONDENSED, KIND_MODQ, KIND_U16, KIND_CRT, KIND_SMALL = 0, 1, 2, 3, 4
+STREAM_DB, STREAM_BT, STREAM_BTB, STREAM_BTH, STREAM_AHAT = 1, 2, 3, 4, 5
+STREAM_Y, STREAM_Z, STREAM_ROW, STREAM_BODY = 16, 17, 18, 19
+
+_lib = None
+
+
+class InspireError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("inspire error %d: %s" % (code, msg))
+        self.code = code
+
+
+class PlanT(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in ("num_items", "item_size_bits", "dim0", "db_rows", "db_cols",
+                                               "instances", "interpolate_degree", "c")] + \
+               [(n, ctypes.c_uint64) for n in ("db_bytes", "hint_bytes", "prep_pack_bytes",
+                                               "r_buffers_bytes", "bold_t_bytes", "peak_bytes")]
+
+
+class ShapeT(ctypes.Structure):
+    _fields_ = [("db_rows", ctypes.c_int64), ("instances", ctypes.c_int64),
+                ("interpolate_degree", ctypes.c_int32), ("max_queries", ctypes.c_int32)]
+
+
+def lib():
+    """Load libinspire_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libinspire_hip.so not built: run __graft_entry__.build() "
+                               "(expected at %s)" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i64, i32, u64, u32, sz = (ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_uint64,
+                                      ctypes.c_uint32, ctypes.c_size_t)
+        L.inspire_last_error.restype = ctypes.c_char_p
+        L.inspire_plan_shape.argtypes = [i64, i64, i64, i64, i64, ctypes.POINTER(PlanT)]
+        L.inspire_server_create.argtypes = [ctypes.POINTER(ShapeT), ctypes.POINTER(vp)]
+        L.inspire_server_destroy.argtypes = [vp]
+        L.inspire_server_destroy.restype = None
+        L.inspire_server_set_db.argtypes = [vp, vp, i32]
+        L.inspire_server_set_precomp.argtypes = [vp, vp, vp, vp, vp, i32]
+        L.inspire_server_set_stash.argtypes = [vp, vp, u32, u32]
+        L.inspire_server_fill_synthetic.argtypes = [vp, u64]
+        L.inspire_server_response_bytes.argtypes = [vp, i32]
+        L.inspire_server_response_bytes.restype = sz
+        L.inspire_respond_serialized.argtypes = [vp, vp, sz, vp, sz, ctypes.POINTER(sz)]
+        L.inspire_respond_device.argtypes = [vp, vp, sz, i32, vp, vp]
+        L.inspire_server_profile_enable.argtypes = [vp, i32]
+        L.inspire_server_profile_read.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64), i32]
+        L.inspire_server_stream.argtypes = [vp]
+        L.inspire_server_stream.restype = vp
+        L.inspire_synth_fill.argtypes = [vp, u64, u32, u64, u64, i32, vp]
+        L.inspire_unpack_query_device.argtypes = [vp, i32, i64, vp, vp, vp, vp, vp]
+        L.inspire_gemv_device.argtypes = [vp, vp, i32, vp, i64, i64, vp]
+        L.inspire_packing_device.argtypes = [vp, vp, vp, vp, vp, vp, i64, vp]
+        L.inspire_post_device.argtypes = [vp, vp, i64, vp]
+        L.inspire_fold_device.argtypes = [vp, vp, vp, vp, vp, i64, i32, i32, vp]
+        L.gpu_available_memory_bytes.restype = u64
+        L.gpu_packing_online_get_d_y_all.restype = vp
+        L.gpu_packing_online_get_d_y_bar_all.restype = vp
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != OK:
+        raise InspireError(rc, lib().inspire_last_error().decode())
+    return rc
+
+
+def plan_shape(record_count, key_size, value_size, buckets=None, dim0=None):
+    """KeywordServerSetupPlan::from_shape + memory estimate (keyword.rs:104-233)."""
+    p = PlanT()
+    check(lib().inspire_plan_shape(record_count, key_size, value_size, buckets or 0, dim0 or 0,
+                                   ctypes.byref(p)))
+    return {f: getattr(p, f) for f, _ in PlanT._fields_}
+
+
+def _ptr(t):
+    """Device or host address of a torch tensor / numpy array / int."""
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return t
+    if hasattr(t, "data_ptr"):
+        return t.data_ptr()
+    return t.ctypes.data
+
+
+class Server:
+    """KeywordServer answer path resident on one GPU (keyword.rs:49-63, 534-640)."""
+
+    def __init__(self, db_rows, instances, interpolate_degree, max_queries=2):
+        self.db_rows, self.instances, self.d = db_rows, instances, interpolate_degree
+        self.c = instances // interpolate_degree
+        shape = ShapeT(db_rows, instances, interpolate_degree, max_queries)
+        h = ctypes.c_void_p()
+        check(lib().inspire_server_create(ctypes.byref(shape), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().inspire_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def fill_synthetic(self, seed):
+        check(lib().inspire_server_fill_synthetic(self.h, seed))
+
+    def set_db(self, db, host=True):
+        check(lib().inspire_server_set_db(self.h, _ptr(db), 1 if host else 0))
+
+    def set_precomp(self, bt, btb, bth, a_hat, host=True):
+        check(lib().inspire_server_set_precomp(self.h, _ptr(bt), _ptr(btb), _ptr(bth), _ptr(a_hat),
+                                               1 if host else 0))
+
+    def set_stash(self, entries):
+        entries = list(entries)
+        size = len(entries[0]) if entries else 0
+        blob = b"".join(entries)
+        buf = ctypes.create_string_buffer(blob, len(blob) or 1)
+        check(lib().inspire_server_set_stash(self.h, ctypes.cast(buf, ctypes.c_void_p), size, len(entries)))
+
+    def stream(self):
+        return lib().inspire_server_stream(self.h)
+
+    def respond_serialized(self, query):
+        """respond_to_serialized_query (keyword.rs:636-640): bytes in, bytes out."""
+        nq = query[0] if query else 0
+        cap = lib().inspire_server_response_bytes(self.h, nq)
+        out = ctypes.create_string_buffer(max(cap, 1))
+        qb = ctypes.create_string_buffer(bytes(query), len(query) or 1)
+        n = ctypes.c_size_t()
+        check(lib().inspire_respond_serialized(self.h, ctypes.cast(qb, ctypes.c_void_p), len(query),
+                                               ctypes.cast(out, ctypes.c_void_p), cap, ctypes.byref(n)))
+        return out.raw[:n.value]
+
+    def profile(self, enable=True):
+        check(lib().inspire_server_profile_enable(self.h, 1 if enable else 0))
+
+    def profile_read(self, reset=True):
+        """Summed ms per stage [unpack, packing, post, gemv, fold] and the respond count."""
+        ms = (ctypes.c_double * 5)()
+        cnt = ctypes.c_int64()
+        check(lib().inspire_server_profile_read(self.h, ms, ctypes.byref(cnt), 1 if reset else 0))
+        return list(ms), cnt.value
+
+    def respond_device(self, d_query, length, nq, d_out, stream=None):
+        check(lib().inspire_respond_device(self.h, _ptr(d_query), length, nq, _ptr(d_out), stream))
+
+
+def synth_fill(d_out, seed, stream_id, count, kind, start=0, stream=None):
+    check(lib().inspire_synth_fill(_ptr(d_out), seed, stream_id, start, count, kind, stream))
+
+
+def unpack_query_device(d_query, nq, db_rows, d_y, d_z, d_rows, d_bodies, stream=None):
+    check(lib().inspire_unpack_query_device(_ptr(d_query), nq, db_rows, _ptr(d_y), _ptr(d_z), _ptr(d_rows),
+                                            _ptr(d_bodies), stream))
+
+
+def gemv_device(d_c, d_rows, nq, d_db, db_rows, db_cols, stream=None):
+    check(lib().inspire_gemv_device(_ptr(d_c), _ptr(d_rows), nq, _ptr(d_db), db_rows, db_cols, stream))
+
+
+def packing_device(d_S, d_bt, d_btb, d_bth, d_y, d_z, instances, stream=None):
+    check(lib().inspire_packing_device(_ptr(d_S), _ptr(d_bt), _ptr(d_btb), _ptr(d_bth), _ptr(d_y), _ptr(d_z),
+                                       instances, stream))
+
+
+def post_device(d_P, d_S, count, stream=None):
+    check(lib().inspire_post_device(_ptr(d_P), _ptr(d_S), count, stream))
+
+
+def fold_device(d_out, d_a_hat, d_c, d_P, d_bodies, instances, d, nq, stream=None):
+    check(lib().inspire_fold_device(_ptr(d_out), _ptr(d_a_hat), _ptr(d_c), _ptr(d_P), _ptr(d_bodies),
+                                    instances, d, nq, stream))
+
+
+def serialize_query(y_body, z_body, rows, bodies):
+    """serialize_keyword_query (wire.rs:542-615), the client side of the wire
+    format: a count byte, then 56-bit fields (28 bits per CRT residue) packed
+    little-endian, i.e. field f at bytes [1 + 7f, 8 + 7f).
+
+    y_body, z_body: [3][2048] condensed u64; rows: [nq][db_rows] condensed u64;
+    bodies: [nq][6][2][2048] residues."""
+    import numpy as np
+    nq = rows.shape[0]
+    m28 = np.uint64((1 << 28) - 1)
+
+    def cond(v):
+        v = np.asarray(v, dtype=np.uint64)
+        return (v & m28) | ((v >> np.uint64(32)) & m28) << np.uint64(28)
+
+    keys = np.empty((3, 2048, 2), dtype=np.uint64)
+    keys[:, :, 0] = cond(y_body)
+    keys[:, :, 1] = cond(z_body)
+    parts = [keys.reshape(-1)]
+    for h in range(nq):
+        parts.append(cond(rows[h]))
+        b = np.asarray(bodies[h], dtype=np.uint64)
+        parts.append(((b[:, 0, :] & m28) | ((b[:, 1, :] & m28) << np.uint64(28))).reshape(-1))
+    fields = np.concatenate(parts)
+    raw = fields.view(np.uint8).reshape(-1, 8)[:, :7].reshape(-1)
+    return bytes([nq]) + raw.tobytes()
+
+
+def exported_symbols():
+    """Function names declared in include/inspire_hip.h."""
+    import re
+    text = open(HEADER_PATH).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text)
+    keep = [n for n in names if n.startswith(("gpu_", "inspire_"))]
+    return sorted(set(keep))

@@ -1,0 +1,313 @@
+"""ctypes wrapper around the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py.  It is the checker, never the thing measured or
+shipped.  Each wrapper names the reference function the C code restates.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+N = 2048
+TEXP = 3
+NROT = N // 2 - 1
+NINNER = NROT * TEXP
+Q0 = 268369921
+Q1 = 249561089
+Q = Q0 * Q1
+RESP_BYTES = 12288
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "-B", "liboracle.so"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.orc_synth_word.restype = ctypes.c_uint64
+        _lib.orc_synth_word.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64]
+        _lib.orc_read_bits.restype = ctypes.c_uint64
+        _lib.orc_serialize_query.restype = ctypes.c_size_t
+        _lib.orc_deserialize_query.restype = ctypes.c_int
+        _lib.orc_serialize_response.restype = ctypes.c_size_t
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def constants():
+    """core/params.rs:258-270 + arith.rs:117-131: Barrett and CRT constants."""
+    out = np.zeros(16, dtype=np.uint64)
+    lib().orc_constants(_p(out))
+    keys = ["cr0_q0", "cr1_q0", "cr0_q1", "cr1_q1", "cr0_q", "cr1_q", "mod0_inv_mod1",
+            "mod1_inv_mod0", "modulus", "root0", "root1"]
+    return {k: int(out[i]) for i, k in enumerate(keys)}
+
+
+def ntt_tables():
+    """ntt.rs:74-108 build_ntt_tables -> [2][4][N]."""
+    out = np.zeros((2, 4, N), dtype=np.uint64)
+    lib().orc_ntt_tables(_p(out))
+    return out
+
+
+def ntt_forward(polys):
+    """ntt.rs:211-357 (restated in scalar form). polys: [..., 2, N] canonical."""
+    a = np.ascontiguousarray(polys, dtype=np.uint64).copy()
+    cnt = a.size // (2 * N)
+    lib().orc_ntt_forward(_p(a), ctypes.c_size_t(cnt))
+    return a
+
+
+def ntt_inverse(polys):
+    """ntt.rs:575-728."""
+    a = np.ascontiguousarray(polys, dtype=np.uint64).copy()
+    cnt = a.size // (2 * N)
+    lib().orc_ntt_inverse(_p(a), ctypes.c_size_t(cnt))
+    return a
+
+
+def crt_compose(x, y):
+    x = np.ascontiguousarray(x, dtype=np.uint64)
+    y = np.ascontiguousarray(y, dtype=np.uint64)
+    out = np.zeros_like(x)
+    lib().orc_crt_compose(_p(x), _p(y), _p(out), ctypes.c_size_t(x.size))
+    return out
+
+
+def barrett_u128(vals, cr0, cr1, m):
+    lo = np.array([v & ((1 << 64) - 1) for v in vals], dtype=np.uint64)
+    hi = np.array([v >> 64 for v in vals], dtype=np.uint64)
+    out = np.zeros(len(vals), dtype=np.uint64)
+    lib().orc_barrett_u128(_p(lo), _p(hi), _p(out), ctypes.c_size_t(len(vals)),
+                           ctypes.c_uint64(cr0), ctypes.c_uint64(cr1), ctypes.c_uint64(m))
+    return out
+
+
+def barrett_raw_u64(vals, cr1, m):
+    x = np.ascontiguousarray(vals, dtype=np.uint64)
+    out = np.zeros_like(x)
+    lib().orc_barrett_raw_u64(_p(x), _p(out), ctypes.c_size_t(x.size), ctypes.c_uint64(cr1),
+                              ctypes.c_uint64(m))
+    return out
+
+
+def automorph_table(t):
+    """packing.rs:1423-1468 (closed form)."""
+    out = np.zeros(N, dtype=np.uint32)
+    lib().orc_automorph_table(ctypes.c_uint32(t), _p(out))
+    return out
+
+
+def automorph_raw(a, t, modulus):
+    """poly.rs automorph_poly."""
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    out = np.zeros_like(a)
+    lib().orc_automorph_raw(_p(a), _p(out), ctypes.c_uint32(t), ctypes.c_uint64(modulus))
+    return out
+
+
+def gadget_invert(inp, mx, modulus_log2=56):
+    """gadget.rs:35-65. inp [rdim][N] -> [mx][N]."""
+    inp = np.ascontiguousarray(inp, dtype=np.uint64)
+    rdim = inp.shape[0]
+    out = np.zeros((mx, N), dtype=np.uint64)
+    lib().orc_gadget_invert(_p(inp), _p(out), ctypes.c_int(rdim), ctypes.c_int(mx),
+                            ctypes.c_uint64(modulus_log2))
+    return out
+
+
+def rescale(vals, inp_mod, out_mod):
+    """arith.rs:436-451."""
+    a = np.ascontiguousarray(vals, dtype=np.uint64)
+    out = np.zeros_like(a)
+    lib().orc_rescale(_p(a), _p(out), ctypes.c_size_t(a.size), ctypes.c_uint64(inp_mod),
+                      ctypes.c_uint64(out_mod))
+    return out
+
+
+def chacha20_block(key, words12_15):
+    k = np.frombuffer(bytes(key), dtype=np.uint8).copy()
+    w = np.array(words12_15, dtype=np.uint32)
+    out = np.zeros(16, dtype=np.uint32)
+    lib().orc_chacha20_block(_p(k), _p(w), _p(out))
+    return out
+
+
+def chacha20_u64(seed, n):
+    s = np.frombuffer(bytes(seed), dtype=np.uint8).copy()
+    out = np.zeros(n, dtype=np.uint64)
+    lib().orc_chacha20_u64(_p(s), _p(out), ctypes.c_size_t(n))
+    return out
+
+
+def rgsw_neg_a_ntt():
+    """keyword.rs:546-555: NTT(-a) [2][N] and a (raw mod q)."""
+    out = np.zeros((2, N), dtype=np.uint64)
+    a = np.zeros(N, dtype=np.uint64)
+    lib().orc_rgsw_neg_a_ntt(_p(out), _p(a))
+    return out, a
+
+
+def gemv(a, db, db_rows, db_cols):
+    """gemv_kernel.cu:46-103 / kernel.rs:139-237."""
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    db = np.ascontiguousarray(db, dtype=np.uint16)
+    c = np.zeros(db_cols, dtype=np.uint64)
+    lib().orc_gemv(_p(c), _p(a), _p(db), ctypes.c_int64(db_rows), ctypes.c_int64(db_cols))
+    return c
+
+
+def packing(bold_t, bold_t_bar, bold_t_hat, y_body, z_body, instances, j0=0, j1=None):
+    """packing_online_kernel.cu:18-92 -> S [j1-j0][2][N]."""
+    if j1 is None:
+        j1 = instances
+    S = np.zeros((j1 - j0, 2, N), dtype=np.uint64)
+    lib().orc_packing(_p(S), _p(bold_t), _p(bold_t_bar), _p(bold_t_hat), _p(y_body), _p(z_body),
+                      ctypes.c_int64(instances), ctypes.c_int64(j0), ctypes.c_int64(j1))
+    return S
+
+
+def post(S):
+    """packing_online.rs:359-398: CRT(INTT(S)) -> [count][N] mod q."""
+    S = np.ascontiguousarray(S, dtype=np.uint64)
+    cnt = S.shape[0]
+    P = np.zeros((cnt, N), dtype=np.uint64)
+    lib().orc_post(_p(P), _p(S), ctypes.c_int64(cnt))
+    return P
+
+
+def external_product(sum2, neg_a, body):
+    sum2 = np.ascontiguousarray(sum2, dtype=np.uint64)
+    out = np.zeros((2, N), dtype=np.uint64)
+    lib().orc_external_product(_p(out), _p(sum2), _p(np.ascontiguousarray(neg_a, dtype=np.uint64)),
+                               _p(np.ascontiguousarray(body, dtype=np.uint64)))
+    return out
+
+
+def fold_chain(packed_a, packed_b, neg_a, body):
+    """keyword.rs:558-573 for one chain; packed_a/b: [d][N]."""
+    packed_a = np.ascontiguousarray(packed_a, dtype=np.uint64)
+    packed_b = np.ascontiguousarray(packed_b, dtype=np.uint64)
+    out = np.zeros((2, N), dtype=np.uint64)
+    lib().orc_fold_chain(_p(out), _p(packed_a), _p(packed_b), ctypes.c_int(packed_a.shape[0]),
+                         _p(np.ascontiguousarray(neg_a, dtype=np.uint64)),
+                         _p(np.ascontiguousarray(body, dtype=np.uint64)))
+    return out
+
+
+def switch_and_keep(ct):
+    """modulus_switch.rs:45-73 -> 12288 bytes."""
+    ct = np.ascontiguousarray(ct, dtype=np.uint64)
+    out = np.zeros(RESP_BYTES, dtype=np.uint8)
+    lib().orc_switch_and_keep(_p(ct), _p(out))
+    return out.tobytes()
+
+
+def respond_raw(db, db_rows, instances, d, bold_t, bold_t_bar, bold_t_hat, a_hat,
+                y_body, z_body, rows, bodies):
+    """keyword.rs:534-628 up to the per-hash response blobs (list of bytes)."""
+    nq = rows.shape[0]
+    c = instances // d
+    out = np.zeros(nq * c * RESP_BYTES, dtype=np.uint8)
+    rc = lib().orc_respond_raw(_p(out), _p(db), ctypes.c_int64(db_rows), ctypes.c_int64(instances),
+                               ctypes.c_int(d), _p(bold_t), _p(bold_t_bar), _p(bold_t_hat),
+                               _p(a_hat), ctypes.c_int(nq), _p(y_body), _p(z_body),
+                               _p(np.ascontiguousarray(rows)), _p(np.ascontiguousarray(bodies)))
+    if rc != 0:
+        raise ValueError("orc_respond_raw failed: %d" % rc)
+    blob = out.tobytes()
+    per = c * RESP_BYTES
+    return [blob[i * per:(i + 1) * per] for i in range(nq)]
+
+
+def serialize_query(y_body, z_body, rows, bodies):
+    """wire.rs:542-615."""
+    nq, db_rows = rows.shape
+    cap = 1 + ((TEXP * N * 2 + nq * (db_rows + 6 * N)) * 56 + 63) // 64 * 8 + 16
+    out = np.zeros(cap, dtype=np.uint8)
+    n = lib().orc_serialize_query(_p(out), ctypes.c_size_t(cap), ctypes.c_int(nq),
+                                  ctypes.c_int64(db_rows), _p(y_body), _p(z_body),
+                                  _p(np.ascontiguousarray(rows)), _p(np.ascontiguousarray(bodies)))
+    return out[:n].tobytes()
+
+
+def deserialize_query(buf, db_rows, max_q=8):
+    """wire.rs:618-705."""
+    b = np.frombuffer(buf, dtype=np.uint8).copy()
+    y = np.zeros((TEXP, N), dtype=np.uint64)
+    z = np.zeros((TEXP, N), dtype=np.uint64)
+    rows = np.zeros((max_q, db_rows), dtype=np.uint64)
+    bodies = np.zeros((max_q, 6, 2, N), dtype=np.uint64)
+    nq = lib().orc_deserialize_query(_p(b), ctypes.c_size_t(len(buf)), ctypes.c_int64(db_rows),
+                                     ctypes.c_int(max_q), _p(y), _p(z), _p(rows), _p(bodies))
+    if nq < 0:
+        raise ValueError("malformed keyword query")
+    return y, z, rows[:nq], bodies[:nq]
+
+
+def serialize_response(responses, stash):
+    """wire.rs:708-726."""
+    out = bytearray()
+    out += len(responses).to_bytes(4, "little")
+    for r in responses:
+        out += len(r).to_bytes(4, "little") + r
+    out += len(stash).to_bytes(4, "little")
+    for e in stash:
+        out += len(e).to_bytes(4, "little") + e
+    return bytes(out)
+
+
+# ---------------------------------------------------------------------------
+# Synthetic inputs (same counter-based generator as the HIP library)
+# ---------------------------------------------------------------------------
+
+KIND_CONDENSED, KIND_MODQ, KIND_U16, KIND_CRT, KIND_SMALL = 0, 1, 2, 3, 4
+STREAM_DB, STREAM_BT, STREAM_BTB, STREAM_BTH, STREAM_AHAT = 1, 2, 3, 4, 5
+STREAM_Y, STREAM_Z, STREAM_ROW, STREAM_BODY = 16, 17, 18, 19
+
+
+def synth(seed, stream, count, kind, start=0):
+    dt = np.uint16 if kind == KIND_U16 else np.uint64
+    out = np.zeros(count, dtype=dt)
+    lib().orc_synth_fill(_p(out), ctypes.c_uint64(seed), ctypes.c_uint32(stream),
+                         ctypes.c_uint64(start), ctypes.c_uint64(count), ctypes.c_int(kind))
+    return out
+
+
+def synth_state(seed, db_rows, instances):
+    """Synthetic server state: encoded DB, bold_t/bar/hat, a_hat."""
+    db_cols = instances * N
+    db = synth(seed, STREAM_DB, db_rows * db_cols, KIND_U16)
+    bt = synth(seed, STREAM_BT, instances * NINNER * N, KIND_CONDENSED)
+    btb = synth(seed, STREAM_BTB, instances * NINNER * N, KIND_CONDENSED)
+    bth = synth(seed, STREAM_BTH, instances * TEXP * N, KIND_CONDENSED)
+    ahat = synth(seed, STREAM_AHAT, instances * N, KIND_MODQ)
+    return db, bt, btb, bth, ahat
+
+
+def synth_query(seed, db_rows, nq=2, small=False):
+    """Synthetic keyword query (keys + nq per-hash queries)."""
+    y = synth(seed, STREAM_Y, TEXP * N, KIND_CONDENSED).reshape(TEXP, N)
+    z = synth(seed, STREAM_Z, TEXP * N, KIND_CONDENSED).reshape(TEXP, N)
+    rows = synth(seed, STREAM_ROW, nq * db_rows, KIND_SMALL if small else KIND_CONDENSED)
+    rows = rows.reshape(nq, db_rows)
+    bodies = synth(seed, STREAM_BODY, nq * 6 * 2 * N, KIND_CRT).reshape(nq, 6, 2, N)
+    return y, z, rows, bodies
+
+
+def num_threads():
+    return lib().orc_num_threads()

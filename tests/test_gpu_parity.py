@@ -1,0 +1,261 @@
+"""GPU parity: every stage of the answer path and the whole respond, run through
+the C ABI of libinspire_hip.so on gfx950 and compared bit for bit with the CPU
+oracle (or with the committed golden digests of it)."""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import inspire_amd as IA
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+N = O.N
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _t
+    assert _t.cuda.is_available(), "GPU tests need a visible GPU"
+    return _t
+
+
+def dev_from(torch, arr):
+    a = np.ascontiguousarray(arr)
+    if a.dtype == np.uint64:
+        t = torch.from_numpy(a.view(np.int64))
+    elif a.dtype == np.uint32:
+        t = torch.from_numpy(a.view(np.int32))
+    elif a.dtype == np.uint16:
+        t = torch.from_numpy(a.view(np.int16))
+    else:
+        t = torch.from_numpy(a)
+    return t.cuda()
+
+
+def host(torch, t, dtype):
+    torch.cuda.synchronize()
+    return t.cpu().numpy().view(dtype)
+
+
+def empty(torch, n, dtype):
+    tdt = {np.uint64: torch.int64, np.uint32: torch.int32, np.uint16: torch.int16, np.uint8: torch.uint8}[dtype]
+    return torch.zeros(n, dtype=tdt, device="cuda")
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3, 4])
+def test_synth_fill_matches_oracle(torch, kind):
+    say("synth_fill kind", kind)
+    n, start = 5000, 123456
+    dt = np.uint16 if kind == 2 else np.uint64
+    d = empty(torch, n, dt)
+    IA.synth_fill(d, 77, 3, n, kind, start=start)
+    assert (host(torch, d, dt) == O.synth(77, 3, n, kind, start=start)).all()
+
+
+@pytest.mark.parametrize("db_rows,nq", [(2048, 2), (4096, 1)])
+def test_unpack_query_matches_wire_format(torch, db_rows, nq):
+    say("unpack", db_rows, nq)
+    y, z, rows, bodies = O.synth_query(31, db_rows, nq=nq)
+    buf = O.serialize_query(y, z, rows, bodies)
+    dq = dev_from(torch, np.frombuffer(buf + bytes(16), dtype=np.uint8))
+    dy, dz = empty(torch, 3 * N, np.uint64), empty(torch, 3 * N, np.uint64)
+    dr, db = empty(torch, nq * db_rows, np.uint64), empty(torch, nq * 6 * 2 * N, np.uint32)
+    IA.unpack_query_device(dq, nq, db_rows, dy, dz, dr, db)
+    assert (host(torch, dy, np.uint64).reshape(3, N) == y).all()
+    assert (host(torch, dz, np.uint64).reshape(3, N) == z).all()
+    assert (host(torch, dr, np.uint64).reshape(nq, db_rows) == rows).all()
+    assert (host(torch, db, np.uint32).reshape(nq, 6, 2, N) == bodies).all()
+
+
+@pytest.mark.parametrize("db_rows,db_cols,nq,small", [(2048, 3000, 2, False), (4096, 257, 1, True),
+                                                       (2048, 64, 2, False)])
+def test_gemv_matches_oracle(torch, db_rows, db_cols, nq, small):
+    say("gemv", db_rows, db_cols, nq, small)
+    db = O.synth(41, O.STREAM_DB, db_rows * db_cols, O.KIND_U16)
+    rows = O.synth(42, O.STREAM_ROW, nq * db_rows, O.KIND_SMALL if small else O.KIND_CONDENSED)
+    dc = empty(torch, nq * db_cols, np.uint64)
+    IA.gemv_device(dc, dev_from(torch, rows), nq, dev_from(torch, db), db_rows, db_cols)
+    got = host(torch, dc, np.uint64).reshape(nq, db_cols)
+    for h in range(nq):
+        assert (got[h] == O.gemv(rows[h * db_rows:(h + 1) * db_rows], db, db_rows, db_cols)).all()
+
+
+def test_gemv_extreme_values(torch):
+    say("gemv extreme")
+    db_rows, db_cols = 2048, 40
+    db = np.full(db_rows * db_cols, 65534, dtype=np.uint16)
+    db[::7] = 0
+    a = np.full(db_rows, ((1 << 28) - 1) | (((1 << 28) - 1) << 32), dtype=np.uint64)
+    dc = empty(torch, db_cols, np.uint64)
+    IA.gemv_device(dc, dev_from(torch, a), 1, dev_from(torch, db), db_rows, db_cols)
+    assert (host(torch, dc, np.uint64) == O.gemv(a, db, db_rows, db_cols)).all()
+
+
+@pytest.mark.parametrize("instances", [1, 2, 3, 8, 12])
+def test_packing_matches_oracle(torch, instances):
+    say("packing", instances)
+    bt = O.synth(51, O.STREAM_BT, instances * O.NINNER * N, O.KIND_CONDENSED)
+    btb = O.synth(51, O.STREAM_BTB, instances * O.NINNER * N, O.KIND_CONDENSED)
+    bth = O.synth(51, O.STREAM_BTH, instances * 3 * N, O.KIND_CONDENSED)
+    y, z, _, _ = O.synth_query(52, 2048, nq=1)
+    if instances == 3:  # non-canonical 28-bit key residues from the wire
+        y = y | np.uint64((1 << 28) - 1)
+    dS = empty(torch, instances * 2 * N, np.uint32)
+    IA.packing_device(dS, dev_from(torch, bt), dev_from(torch, btb), dev_from(torch, bth), dev_from(torch, y),
+                      dev_from(torch, z), instances)
+    got = host(torch, dS, np.uint32).reshape(instances, 2, N)
+    exp = O.packing(bt, btb, bth, y, z, instances)
+    assert (got == exp).all()
+
+
+@pytest.mark.parametrize("count", [1, 5])
+def test_post_matches_oracle(torch, count):
+    say("post", count)
+    S = O.synth(61, 7, count * 2 * N, O.KIND_CRT).reshape(count, 2, N)
+    dP = empty(torch, count * N, np.uint64)
+    IA.post_device(dP, dev_from(torch, S.astype(np.uint32)), count)
+    assert (host(torch, dP, np.uint64).reshape(count, N) == O.post(S)).all()
+
+
+@pytest.mark.parametrize("instances,d,nq", [(2, 2, 1), (4, 2, 2), (8, 8, 2), (6, 3, 1)])
+def test_fold_matches_oracle(torch, instances, d, nq):
+    say("fold", instances, d, nq)
+    c = instances // d
+    a_hat = O.synth(71, O.STREAM_AHAT, instances * N, O.KIND_MODQ)
+    cvec = O.synth(72, 8, nq * instances * N, O.KIND_MODQ)
+    P = O.synth(73, 9, instances * N, O.KIND_MODQ)
+    bodies = O.synth(74, O.STREAM_BODY, nq * 6 * 2 * N, O.KIND_CRT)
+    dout = empty(torch, nq * c * O.RESP_BYTES, np.uint8)
+    IA.fold_device(dout, dev_from(torch, a_hat), dev_from(torch, cvec), dev_from(torch, P),
+                   dev_from(torch, bodies.astype(np.uint32)), instances, d, nq)
+    got = host(torch, dout, np.uint8).reshape(nq, c, O.RESP_BYTES)
+    neg_a, _ = O.rgsw_neg_a_ntt()
+    ah = a_hat.reshape(instances, N)
+    Pm = P.reshape(instances, N)
+    for h in range(nq):
+        fb = cvec.reshape(nq, instances, N)[h] + Pm
+        body = bodies.reshape(nq, 6, 2, N)[h]
+        for w in range(c):
+            ct = O.fold_chain(ah[w * d:(w + 1) * d], fb[w * d:(w + 1) * d], neg_a, body)
+            assert got[h, w].tobytes() == O.switch_and_keep(ct), (h, w)
+
+
+def _golden():
+    return json.load(open(os.path.join(os.path.dirname(__file__), "golden", "respond_small.json")))["cases"]
+
+
+@pytest.mark.parametrize("case", _golden(), ids=lambda c: "r%d_i%d_d%d" % (c["db_rows"], c["instances"], c["d"]))
+def test_respond_serialized_matches_golden(torch, case):
+    say("respond golden", case["db_rows"], case["instances"], case["d"], case["nq"])
+    db_rows, inst, d, nq = case["db_rows"], case["instances"], case["d"], case["nq"]
+    srv = IA.Server(db_rows, inst, d)
+    srv.fill_synthetic(case["state_seed"])
+    stash = [bytes([i] * 1216) for i in range(3)]
+    srv.set_stash(stash)
+    y, z, rows, bodies = O.synth_query(case["query_seed"], db_rows, nq=nq, small=case["small"])
+    q = O.serialize_query(y, z, rows, bodies)
+    resp = srv.respond_serialized(q)
+    blob = (inst // d) * O.RESP_BYTES
+    assert len(resp) == 4 + nq * (4 + blob) + 4 + 3 * (4 + 1216)
+    parts = [resp[8 + h * (4 + blob): 8 + h * (4 + blob) + blob] for h in range(nq)]
+    assert [hashlib.sha256(p).hexdigest() for p in parts] == case["sha256"]
+    # and the full serialization equals the oracle's wire format
+    assert resp == O.serialize_response(parts, stash)
+    srv.close()
+
+
+def test_respond_serialized_matches_live_oracle(torch):
+    say("respond live oracle 2048x4 d2")
+    db_rows, inst, d = 2048, 4, 2
+    st = O.synth_state(9, db_rows, inst)
+    srv = IA.Server(db_rows, inst, d)
+    srv.set_db(st[0])
+    srv.set_precomp(st[1], st[2], st[3], st[4])
+    y, z, rows, bodies = O.synth_query(10, db_rows, nq=2)
+    resp = srv.respond_serialized(O.serialize_query(y, z, rows, bodies))
+    exp = O.respond_raw(st[0], db_rows, inst, d, st[1], st[2], st[3], st[4], y, z, rows, bodies)
+    assert resp == O.serialize_response(exp, [])
+    # repeat: the resident state is unchanged by a respond
+    assert srv.respond_serialized(O.serialize_query(y, z, rows, bodies)) == resp
+    srv.close()
+
+
+def test_respond_edge_cases(torch):
+    say("respond edge cases")
+    db_rows, inst, d = 2048, 2, 2
+    srv = IA.Server(db_rows, inst, d)
+    y, z, rows, bodies = O.synth_query(11, db_rows, nq=2)
+    q = O.serialize_query(y, z, rows, bodies)
+    with pytest.raises(IA.InspireError) as e:   # no state loaded yet
+        srv.respond_serialized(q)
+    assert e.value.code == IA.ERR_STATE
+    srv.fill_synthetic(5)
+    with pytest.raises(IA.InspireError) as e:   # truncated
+        srv.respond_serialized(q[:-3])
+    assert e.value.code == IA.ERR_INVALID
+    with pytest.raises(IA.InspireError):        # empty
+        srv.respond_serialized(b"")
+    # zero hash queries: only the counts and the stash come back
+    q0 = O.serialize_query(y, z, rows[:0], bodies[:0])
+    assert srv.respond_serialized(q0) == O.serialize_response([], [])
+    # trailing bytes after a well-formed query are ignored (wire.rs:618-705)
+    assert srv.respond_serialized(q + b"\x00" * 9) == srv.respond_serialized(q)
+    srv.close()
+
+
+def test_legacy_gemv_query(torch):
+    say("legacy gemv")
+    import ctypes
+    L = IA.lib()
+    db_rows, db_cols = 2048, 1000
+    db = O.synth(81, O.STREAM_DB, db_rows * db_cols, O.KIND_U16)
+    a = O.synth(82, O.STREAM_ROW, db_rows, O.KIND_CONDENSED)
+    d_db = ctypes.c_void_p()
+    L.gpu_gemv_upload_db(db.ctypes.data_as(ctypes.c_void_p), ctypes.byref(d_db), ctypes.c_size_t(db.nbytes))
+    assert d_db.value
+    c = np.zeros(db_cols, dtype=np.uint64)
+    k = O.constants()
+    u = ctypes.c_uint64
+    L.gpu_gemv_query(c.ctypes.data_as(ctypes.c_void_p), a.ctypes.data_as(ctypes.c_void_p), d_db, db_rows, db_cols,
+                     u(O.Q0), u(O.Q1), u(k["cr1_q0"]), u(k["cr1_q1"]), u(k["mod0_inv_mod1"]),
+                     u(k["mod1_inv_mod0"]), u(O.Q), u(k["cr0_q"]), u(k["cr1_q"]))
+    L.gpu_gemv_free_db(d_db)
+    assert (c == O.gemv(a, db, db_rows, db_cols)).all()
+
+
+def test_legacy_rotation_and_packing(torch):
+    say("legacy rotation + packing")
+    import ctypes
+    L = IA.lib()
+    vp = ctypes.c_void_p
+    inst = 2
+    tables = np.concatenate([O.automorph_table(t) for t in range(1, 2 * N, 2)]).astype(np.uint32)
+    gen_pows = np.array([pow(5, i, 2 * N) for i in range(O.NROT)], dtype=np.uint32)
+    L.gpu_rotation_upload_tables(tables.ctypes.data_as(vp), gen_pows.ctypes.data_as(vp), N, O.NROT, N)
+    L.gpu_packing_online_ensure_key_buffers(O.NINNER, 3, N)
+    y, z, _, _ = O.synth_query(83, 2048, nq=1)
+    L.gpu_rotation_expand_keys(vp(L.gpu_packing_online_get_d_y_all()), vp(L.gpu_packing_online_get_d_y_bar_all()),
+                               y.ctypes.data_as(vp), O.NROT, 3, N)
+    L.gpu_packing_online_upload_z_body(z.ctypes.data_as(vp), 3, N)
+    bt = O.synth(84, O.STREAM_BT, inst * O.NINNER * N, O.KIND_CONDENSED)
+    btb = O.synth(84, O.STREAM_BTB, inst * O.NINNER * N, O.KIND_CONDENSED)
+    bth = O.synth(84, O.STREAM_BTH, inst * 3 * N, O.KIND_CONDENSED)
+    L.gpu_packing_online_upload_precomp(bt.ctypes.data_as(vp), btb.ctypes.data_as(vp), bth.ctypes.data_as(vp),
+                                        inst, O.NINNER, 3, N)
+    out = np.zeros(inst * 2 * N, dtype=np.uint64)
+    k = O.constants()
+    u = ctypes.c_uint64
+    L.gpu_packing_online_compute(out.ctypes.data_as(vp), inst, O.NROT, 3, N, 128, u(O.Q0), u(O.Q1),
+                                 u(k["cr1_q0"]), u(k["cr1_q1"]))
+    L.gpu_packing_online_free()
+    L.gpu_rotation_free()
+    assert (out.reshape(inst, 2, N) == O.packing(bt, btb, bth, y, z, inst)).all()
