@@ -128,14 +128,18 @@ def main():
     log("state resident in %.1fs" % (time.time() - t0))
     dq, qlen, stride = make_queries(IA, torch, db_rows, args.queries, 13)
     out = torch.empty(2 * plan["c"] * IA.RESP_BYTES, dtype=torch.uint8, device="cuda")
-    stream = torch.cuda.current_stream()
+    # A dedicated stream (default priority): the work and the events that time
+    # it are on the same stream.
+    stream = torch.cuda.Stream()
     sh = stream.cuda_stream
+    assert sh, "need a non-null stream handle"
 
     def step(k):
         srv.respond_device(dq.data_ptr() + (k % args.queries) * stride, qlen, 2, out, sh)
 
     for k in range(args.warmup):
         step(k)
+    stream.synchronize()
     torch.cuda.synchronize()
     log("warmup done")
 
@@ -161,6 +165,7 @@ def main():
     srv.profile(True)
     for k in range(args.profile_steps):
         step(k)
+    stream.synchronize()
     torch.cuda.synchronize()
     stage_ms, cnt = srv.profile_read()
     srv.profile(False)

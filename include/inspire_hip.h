@@ -169,6 +169,12 @@ int inspire_respond_serialized(inspire_server_t* srv, const uint8_t* query, size
  * `stream` (hipStream_t; NULL = the server's own stream). */
 int inspire_respond_device(inspire_server_t* srv, const uint8_t* d_query, size_t len, int nq, uint8_t* d_out,
                            void* stream);
+/* First-pass kernel used by the server: INSPIRE_GEMV_MFMA (default, exact
+ * int8-limb MFMA) or INSPIRE_GEMV_VALU (64-bit multiply-accumulate). */
+#define INSPIRE_GEMV_VALU 0
+#define INSPIRE_GEMV_MFMA 1
+int inspire_server_set_gemv_kernel(inspire_server_t* srv, int kind);
+
 /* The server's stream (hipStream_t). */
 void* inspire_server_stream(inspire_server_t* srv);
 
@@ -194,6 +200,11 @@ int inspire_unpack_query_device(const uint8_t* d_query, int nq, int64_t db_rows,
 /* gemv_kernel.cu:46-103 for nq query rows at once: d_c [nq][db_cols]. */
 int inspire_gemv_device(uint64_t* d_c, const uint64_t* d_rows, int nq, const uint16_t* d_db, int64_t db_rows,
                         int64_t db_cols, void* stream);
+/* The same product on int8 MFMA for nq <= 2; d_scratch holds
+ * inspire_gemv_mfma_scratch_bytes(db_rows) bytes (query fragments). */
+int inspire_gemv_mfma_device(uint64_t* d_c, const uint64_t* d_rows, int nq, const uint16_t* d_db, int64_t db_rows,
+                             int64_t db_cols, void* d_scratch, void* stream);
+size_t inspire_gemv_mfma_scratch_bytes(int64_t db_rows);
 /* packing_online_kernel.cu:18-92 with the rotations fused:
  * d_S [instances][2][2048] u32 canonical. */
 int inspire_packing_device(uint32_t* d_S, const uint64_t* d_bold_t, const uint64_t* d_bold_t_bar,

@@ -1,0 +1,160 @@
+// First-pass GEMV on int8 MFMA (exact):  c[h][col] = sum_k a_h[k] * db[col][k] mod q
+// (gemv_kernel.cu:46-103, gpu/kernel.rs:139-237), for up to two hash queries
+// per launch.
+//
+// Limb split.  Each query residue (28 bits, one per CRT prime) becomes four
+// 7-bit limbs in [0, 127]; each u16 DB value v = 256*hi + lo becomes two
+// signed bytes (lo - 128, hi - 128).  With R_l = sum_k limb_l(a[k]):
+//   sum_k a[k]*v[k] = sum_l 2^(7l) * (256*S_hi,l + S_lo,l + 32896*R_l)
+// where S_x,l = sum_k limb_l(a[k]) * (x[k] - 128) is one int32 MFMA
+// accumulator (|S| <= 127*128*K < 2^31 for K <= 2^17).  The sum is the
+// exact non-negative integer, < 2^55 for K <= 65536, reduced mod q_p last.
+//
+// MFMA tile (v_mfma_i32_16x16x64_i8): rows i = 8h + 4p + l (hash query h,
+// prime p, limb l), columns = 16 DB columns, K = 64 DB rows per step.  Lane
+// (j, g) = (lane % 16, lane / 16) feeds DB column j with the 16 rows
+// {8g..8g+7} and {32+8g..32+8g+7} of the step, in that byte order; the query
+// fragments use the same row order, so the contraction is over matching k.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "modarith.hpp"
+
+namespace inspire {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+
+constexpr int kMfmaChunkRows = 2048;                        // rows staged per LDS chunk
+constexpr int kMfmaStepsPerChunk = kMfmaChunkRows / 64;     // 32
+constexpr int kTilesPerWave = 2;                             // 2 x 16 columns per wave
+constexpr int kColsPerBlock = 4 * kTilesPerWave * 16;        // 128
+
+// Row k of step s held by lane group g at byte t.
+__host__ __device__ inline int frag_row(int s, int g, int t) { return 64 * s + 8 * g + t + (t >= 8 ? 24 : 0); }
+
+// afrag: [db_rows/64][64 lanes][16 B]; rsum: [16] int32 (row i = 8h+4p+l).
+__global__ void k_gemv_prep(const uint64_t* __restrict__ rows, int nq, int db_rows, int8_t* __restrict__ afrag,
+                            int32_t* __restrict__ rsum) {
+  __shared__ int32_t part[16];
+  if (threadIdx.x < 16) part[threadIdx.x] = 0;
+  __syncthreads();
+  const int steps = db_rows / 64;
+  int32_t local[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) local[i] = 0;
+  // One thread per (step, lane): 16 bytes.
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < steps * 64; e += gridDim.x * blockDim.x) {
+    const int s = e / 64, lane = e % 64;
+    const int i = lane % 16, g = lane / 16;
+    const int h = i >> 3, p = (i >> 2) & 1, l = i & 3;
+    uint32_t w[4] = {0, 0, 0, 0};
+    if (h < nq) {
+      for (int t = 0; t < 16; ++t) {
+        const uint64_t v = rows[(int64_t)h * db_rows + frag_row(s, g, t)];
+        const uint32_t r = (uint32_t)(p ? (v >> 32) : v);
+        const uint32_t limb = (r >> (7 * l)) & 127u;
+        w[t >> 2] |= limb << (8 * (t & 3));
+        local[i] += (int32_t)limb;
+      }
+    }
+    reinterpret_cast<uint4*>(afrag)[e] = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+#pragma unroll
+  for (int i = 0; i < 16; ++i)
+    if (local[i]) atomicAdd(&part[i], local[i]);
+  __syncthreads();
+  if (threadIdx.x < 16) atomicAdd(&rsum[threadIdx.x], part[threadIdx.x]);
+}
+
+__device__ inline uint32_t plane_lo(uint32_t a, uint32_t b) {
+  return __builtin_amdgcn_perm(b, a, 0x06040200u) ^ 0x80808080u;
+}
+__device__ inline uint32_t plane_hi(uint32_t a, uint32_t b) {
+  return __builtin_amdgcn_perm(b, a, 0x07050301u) ^ 0x80808080u;
+}
+
+__global__ __launch_bounds__(256) void k_gemv_mfma(uint64_t* __restrict__ c, const int8_t* __restrict__ afrag,
+                                                   const int32_t* __restrict__ rsum,
+                                                   const uint16_t* __restrict__ db, int nq, int db_rows,
+                                                   int64_t db_cols) {
+  __shared__ v4i a_lds[kMfmaStepsPerChunk][64];  // 32 KB
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int j = lane & 15, g = lane >> 4;
+  const int64_t col_base = (int64_t)blockIdx.x * kColsPerBlock + wave * (kTilesPerWave * 16);
+
+  v4i acc_lo[kTilesPerWave], acc_hi[kTilesPerWave];
+  const uint16_t* colp[kTilesPerWave];
+  bool valid[kTilesPerWave];
+#pragma unroll
+  for (int tt = 0; tt < kTilesPerWave; ++tt) {
+    acc_lo[tt] = v4i{0, 0, 0, 0};
+    acc_hi[tt] = v4i{0, 0, 0, 0};
+    const int64_t col = col_base + tt * 16 + j;
+    valid[tt] = col < db_cols;
+    colp[tt] = db + (valid[tt] ? col : 0) * (int64_t)db_rows + 8 * g;
+  }
+
+  for (int k0 = 0; k0 < db_rows; k0 += kMfmaChunkRows) {
+    __syncthreads();
+    const v4i* src = reinterpret_cast<const v4i*>(afrag) + (int64_t)(k0 / 64) * 64;
+    for (int e = threadIdx.x; e < kMfmaStepsPerChunk * 64; e += 256) a_lds[e / 64][e % 64] = src[e];
+    __syncthreads();
+#pragma unroll 4
+    for (int s = 0; s < kMfmaStepsPerChunk; ++s) {
+      const v4i a = a_lds[s][lane];
+#pragma unroll
+      for (int tt = 0; tt < kTilesPerWave; ++tt) {
+        const uint16_t* p = colp[tt] + k0 + 64 * s;
+        u4 d0 = {0, 0, 0, 0}, d1 = {0, 0, 0, 0};
+        if (valid[tt]) {
+          d0 = __builtin_nontemporal_load(reinterpret_cast<const u4*>(p));
+          d1 = __builtin_nontemporal_load(reinterpret_cast<const u4*>(p + 32));
+        }
+        const v4i lo = {(int)plane_lo(d0.x, d0.y), (int)plane_lo(d0.z, d0.w), (int)plane_lo(d1.x, d1.y),
+                        (int)plane_lo(d1.z, d1.w)};
+        const v4i hi = {(int)plane_hi(d0.x, d0.y), (int)plane_hi(d0.z, d0.w), (int)plane_hi(d1.x, d1.y),
+                        (int)plane_hi(d1.z, d1.w)};
+        acc_lo[tt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, lo, acc_lo[tt], 0, 0, 0);
+        acc_hi[tt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, hi, acc_hi[tt], 0, 0, 0);
+      }
+    }
+  }
+
+  // Lane (j, G) holds rows 4G..4G+3 = limbs 0..3 of (h, p) = (G >> 1, G & 1).
+  const int h = g >> 1, p = g & 1;
+  int64_t rs[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) rs[r] = (int64_t)rsum[4 * g + r] * 32896;
+#pragma unroll
+  for (int tt = 0; tt < kTilesPerWave; ++tt) {
+    int64_t v = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      v += ((int64_t)256 * acc_hi[tt][r] + (int64_t)acc_lo[tt][r] + rs[r]) << (7 * r);
+    const uint64_t red = p ? barrett64((uint64_t)v, kCr1Q1, kQ1) : barrett64((uint64_t)v, kCr1Q0, kQ0);
+    const uint64_t other = __shfl_down(red, 16, 64);  // lane group 2h+1 -> 2h
+    const int64_t col = col_base + tt * 16 + j;
+    if (p == 0 && h < nq && col < db_cols) c[(int64_t)h * db_cols + col] = crt_compose((uint32_t)red, (uint32_t)other);
+  }
+}
+
+size_t gemv_mfma_scratch_bytes(int db_rows) { return (size_t)(db_rows / 64) * 64 * 16 + 16 * sizeof(int32_t); }
+
+int launch_gemv_mfma(uint64_t* c, const uint64_t* rows, int nq, const uint16_t* db, int db_rows, int64_t db_cols,
+                     void* scratch, hipStream_t s) {
+  if (nq < 1 || nq > 2 || db_rows <= 0 || db_rows % kMfmaChunkRows != 0 || db_rows > (1 << 17) || db_cols <= 0)
+    return -1;
+  int8_t* afrag = static_cast<int8_t*>(scratch);
+  int32_t* rsum = reinterpret_cast<int32_t*>(afrag + (size_t)(db_rows / 64) * 64 * 16);
+  // rsum is accumulated with atomics: clear it with a kernel (graph-capture safe).
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(rsum), 16);
+  const int prep_threads = (db_rows / 64) * 64;
+  hipLaunchKernelGGL(k_gemv_prep, dim3((prep_threads + 255) / 256), dim3(256), 0, s, rows, nq, db_rows, afrag, rsum);
+  const unsigned blocks = (unsigned)((db_cols + kColsPerBlock - 1) / kColsPerBlock);
+  hipLaunchKernelGGL(k_gemv_mfma, dim3(blocks), dim3(256), 0, s, c, afrag, rsum, db, nq, db_rows, db_cols);
+  return 0;
+}
+
+}  // namespace inspire

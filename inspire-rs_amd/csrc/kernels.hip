@@ -62,6 +62,10 @@ void launch_synth_fill(void* out, uint64_t seed, uint32_t stream, uint64_t start
                      kind);
 }
 
+__global__ void k_zero_u32(uint32_t* __restrict__ p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
 // ---------------------------------------------------------------------------
 // Query unpack (wire.rs:618-705).  Every value is a 56-bit field (28 bits per
 // CRT residue, wire.rs:120-125) starting at bit 8 + 56*f, i.e. at byte 1 + 7*f.

@@ -15,6 +15,14 @@ void launch_synth_fill(void* out, uint64_t seed, uint32_t stream, uint64_t start
 void launch_unpack_query(const uint8_t* q, int nq, int db_rows, uint64_t* y_body, uint64_t* z_body,
                          uint64_t* rows, uint32_t* bodies, hipStream_t s);
 
+__global__ void k_zero_u32(uint32_t* p, int n);
+
+// Exact int8-limb MFMA GEMV for nq <= 2 hash queries; scratch holds the query
+// fragments (gemv_mfma_scratch_bytes(db_rows) bytes, 16-byte aligned).
+size_t gemv_mfma_scratch_bytes(int db_rows);
+int launch_gemv_mfma(uint64_t* c, const uint64_t* rows, int nq, const uint16_t* db, int db_rows, int64_t db_cols,
+                     void* scratch, hipStream_t s);
+
 // c: [nq][db_cols] canonical mod q; rows: [nq][db_rows] condensed; db: [db_cols][db_rows].
 int launch_gemv(uint64_t* c, const uint64_t* rows, int nq, const uint16_t* db, int db_rows, int64_t db_cols,
                 hipStream_t s);

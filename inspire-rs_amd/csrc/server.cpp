@@ -225,6 +225,8 @@ struct inspire_server {
   uint64_t *y_body = nullptr, *z_body = nullptr, *rows = nullptr, *P = nullptr, *cvec = nullptr;
   uint32_t *bodies = nullptr, *S = nullptr;
   uint8_t* out = nullptr;
+  void* gemv_scratch = nullptr;  // MFMA query fragments
+  int gemv_kind = INSPIRE_GEMV_MFMA;
   std::vector<uint8_t> host_out;
   // optional per-stage timing
   bool profile = false;
@@ -287,6 +289,7 @@ extern "C" int inspire_server_create(const inspire_shape_t* shape, inspire_serve
   if (!rc) rc = dev_alloc(&s->P, (size_t)inst * kN, "post output");
   if (!rc) rc = dev_alloc(&s->cvec, (size_t)mq * s->db_cols, "gemv output");
   if (!rc) rc = dev_alloc(&s->out, (size_t)mq * s->c * kRespBytes, "responses");
+  if (!rc) rc = dev_alloc((uint8_t**)&s->gemv_scratch, gemv_mfma_scratch_bytes((int)rows) * 2, "gemv scratch");
   if (!rc) rc = dev_alloc(&s->db, (size_t)rows * s->db_cols, "database");
   if (!rc) rc = dev_alloc(&s->bt, (size_t)inst * kNInner * kN, "bold_t");
   if (!rc) rc = dev_alloc(&s->btb, (size_t)inst * kNInner * kN, "bold_t_bar");
@@ -305,8 +308,8 @@ extern "C" int inspire_server_create(const inspire_shape_t* shape, inspire_serve
 extern "C" void inspire_server_destroy(inspire_server_t* s) {
   if (!s) return;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
-  void* ptrs[] = {s->qbytes, s->y_body, s->z_body, s->rows, s->bodies, s->S, s->P, s->cvec, s->out,
-                  s->db,     s->bt,     s->btb,    s->bth,  s->a_hat};
+  void* ptrs[] = {s->qbytes, s->y_body, s->z_body, s->rows, s->bodies, s->S,     s->P,           s->cvec,
+                  s->out,    s->db,     s->bt,     s->btb,  s->bth,    s->a_hat, s->gemv_scratch};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -316,6 +319,13 @@ extern "C" void inspire_server_destroy(inspire_server_t* s) {
 }
 
 extern "C" void* inspire_server_stream(inspire_server_t* s) { return s ? (void*)s->stream : nullptr; }
+
+extern "C" int inspire_server_set_gemv_kernel(inspire_server_t* s, int kind) {
+  if (!s || (kind != INSPIRE_GEMV_VALU && kind != INSPIRE_GEMV_MFMA))
+    return set_error(INSPIRE_ERR_INVALID, "unknown GEMV kernel");
+  s->gemv_kind = kind;
+  return INSPIRE_OK;
+}
 
 extern "C" int inspire_server_profile_enable(inspire_server_t* s, int enable) {
   if (!s) return set_error(INSPIRE_ERR_INVALID, "null server");
@@ -413,8 +423,17 @@ int run_pipeline(inspire_server_t* s, const uint8_t* d_query, int nq, uint8_t* d
   mark(2);
   launch_post(s->P, s->S, s->consts->tw, (int)inst, st);
   mark(3);
-  if (launch_gemv(s->cvec, s->rows, nq, s->db, (int)rows, s->db_cols, st))
+  if (s->gemv_kind == INSPIRE_GEMV_MFMA) {
+    // two hash queries per MFMA tile; larger nq runs in pairs
+    for (int h = 0; h < nq; h += 2) {
+      const int m = nq - h < 2 ? nq - h : 2;
+      if (launch_gemv_mfma(s->cvec + (size_t)h * s->db_cols, s->rows + (size_t)h * rows, m, s->db, (int)rows,
+                           s->db_cols, s->gemv_scratch, st))
+        return set_error(INSPIRE_ERR_INVALID, "unsupported GEMV shape");
+    }
+  } else if (launch_gemv(s->cvec, s->rows, nq, s->db, (int)rows, s->db_cols, st)) {
     return set_error(INSPIRE_ERR_INVALID, "unsupported GEMV shape");
+  }
   mark(4);
   if (launch_fold(d_out, s->a_hat, s->cvec, s->P, s->bodies, s->consts->neg_a, s->consts->tw, inst,
                   s->shape.interpolate_degree, nq, st))
@@ -518,6 +537,20 @@ extern "C" int inspire_gemv_device(uint64_t* d_c, const uint64_t* d_rows, int nq
     return set_error(INSPIRE_ERR_INVALID, "db_rows must be a positive multiple of 2048");
   HIP_TRY(hipGetLastError(), "gemv launch");
   return INSPIRE_OK;
+}
+
+extern "C" int inspire_gemv_mfma_device(uint64_t* d_c, const uint64_t* d_rows, int nq, const uint16_t* d_db,
+                                        int64_t db_rows, int64_t db_cols, void* d_scratch, void* stream) {
+  if (!d_c || !d_rows || !d_db || !d_scratch || nq < 1 || nq > 2)
+    return set_error(INSPIRE_ERR_INVALID, "bad gemv argument");
+  if (launch_gemv_mfma(d_c, d_rows, nq, d_db, (int)db_rows, db_cols, d_scratch, (hipStream_t)stream))
+    return set_error(INSPIRE_ERR_INVALID, "db_rows must be a multiple of 2048 and at most 2^17");
+  HIP_TRY(hipGetLastError(), "gemv launch");
+  return INSPIRE_OK;
+}
+
+extern "C" size_t inspire_gemv_mfma_scratch_bytes(int64_t db_rows) {
+  return db_rows > 0 && db_rows <= (1 << 17) ? gemv_mfma_scratch_bytes((int)db_rows) : 0;
 }
 
 extern "C" int inspire_packing_device(uint32_t* d_S, const uint64_t* d_bt, const uint64_t* d_btb,

@@ -76,6 +76,10 @@ def lib():
         L.inspire_synth_fill.argtypes = [vp, u64, u32, u64, u64, i32, vp]
         L.inspire_unpack_query_device.argtypes = [vp, i32, i64, vp, vp, vp, vp, vp]
         L.inspire_gemv_device.argtypes = [vp, vp, i32, vp, i64, i64, vp]
+        L.inspire_gemv_mfma_device.argtypes = [vp, vp, i32, vp, i64, i64, vp, vp]
+        L.inspire_gemv_mfma_scratch_bytes.argtypes = [i64]
+        L.inspire_gemv_mfma_scratch_bytes.restype = sz
+        L.inspire_server_set_gemv_kernel.argtypes = [vp, i32]
         L.inspire_packing_device.argtypes = [vp, vp, vp, vp, vp, vp, i64, vp]
         L.inspire_post_device.argtypes = [vp, vp, i64, vp]
         L.inspire_fold_device.argtypes = [vp, vp, vp, vp, vp, i64, i32, i32, vp]
@@ -164,6 +168,9 @@ class Server:
                                                ctypes.cast(out, ctypes.c_void_p), cap, ctypes.byref(n)))
         return out.raw[:n.value]
 
+    def set_gemv_kernel(self, kind):
+        check(lib().inspire_server_set_gemv_kernel(self.h, kind))
+
     def profile(self, enable=True):
         check(lib().inspire_server_profile_enable(self.h, 1 if enable else 0))
 
@@ -189,6 +196,18 @@ def unpack_query_device(d_query, nq, db_rows, d_y, d_z, d_rows, d_bodies, stream
 
 def gemv_device(d_c, d_rows, nq, d_db, db_rows, db_cols, stream=None):
     check(lib().inspire_gemv_device(_ptr(d_c), _ptr(d_rows), nq, _ptr(d_db), db_rows, db_cols, stream))
+
+
+GEMV_VALU, GEMV_MFMA = 0, 1
+
+
+def gemv_mfma_scratch_bytes(db_rows):
+    return lib().inspire_gemv_mfma_scratch_bytes(db_rows)
+
+
+def gemv_mfma_device(d_c, d_rows, nq, d_db, db_rows, db_cols, d_scratch, stream=None):
+    check(lib().inspire_gemv_mfma_device(_ptr(d_c), _ptr(d_rows), nq, _ptr(d_db), db_rows, db_cols,
+                                         _ptr(d_scratch), stream))
 
 
 def packing_device(d_S, d_bt, d_btb, d_bth, d_y, d_z, instances, stream=None):

@@ -100,6 +100,29 @@ def test_gemv_extreme_values(torch):
     assert (host(torch, dc, np.uint64) == O.gemv(a, db, db_rows, db_cols)).all()
 
 
+@pytest.mark.parametrize("db_rows,db_cols,nq,kind", [(2048, 3000, 2, "uniform"), (2048, 128, 1, "small"),
+                                                       (4096, 200, 2, "uniform"), (65536, 40, 2, "extreme"),
+                                                       (2048, 16, 2, "extreme")])
+def test_gemv_mfma_matches_oracle(torch, db_rows, db_cols, nq, kind):
+    say("gemv mfma", db_rows, db_cols, nq, kind)
+    if kind == "extreme":
+        db = np.full(db_rows * db_cols, 65534, dtype=np.uint16)
+        db[::3] = 0
+        db[1::5] = 255
+        db[2::7] = 256
+        rows = np.full(nq * db_rows, ((1 << 28) - 1) | (((1 << 28) - 1) << 32), dtype=np.uint64)
+        rows[::11] = 0
+    else:
+        db = O.synth(43, O.STREAM_DB, db_rows * db_cols, O.KIND_U16)
+        rows = O.synth(44, O.STREAM_ROW, nq * db_rows, O.KIND_SMALL if kind == "small" else O.KIND_CONDENSED)
+    scratch = empty(torch, IA.gemv_mfma_scratch_bytes(db_rows), np.uint8)
+    dc = empty(torch, nq * db_cols, np.uint64)
+    IA.gemv_mfma_device(dc, dev_from(torch, rows), nq, dev_from(torch, db), db_rows, db_cols, scratch)
+    got = host(torch, dc, np.uint64).reshape(nq, db_cols)
+    for h in range(nq):
+        assert (got[h] == O.gemv(rows[h * db_rows:(h + 1) * db_rows], db, db_rows, db_cols)).all(), h
+
+
 @pytest.mark.parametrize("instances", [1, 2, 3, 8, 12])
 def test_packing_matches_oracle(torch, instances):
     say("packing", instances)
@@ -186,6 +209,21 @@ def test_respond_serialized_matches_live_oracle(torch):
     assert resp == O.serialize_response(exp, [])
     # repeat: the resident state is unchanged by a respond
     assert srv.respond_serialized(O.serialize_query(y, z, rows, bodies)) == resp
+    srv.close()
+
+
+def test_respond_gemv_kernels_agree(torch):
+    say("respond valu vs mfma gemv")
+    db_rows, inst, d = 4096, 4, 4
+    srv = IA.Server(db_rows, inst, d)
+    srv.fill_synthetic(12)
+    y, z, rows, bodies = O.synth_query(13, db_rows, nq=2)
+    q = O.serialize_query(y, z, rows, bodies)
+    srv.set_gemv_kernel(IA.GEMV_MFMA)
+    a = srv.respond_serialized(q)
+    srv.set_gemv_kernel(IA.GEMV_VALU)
+    b = srv.respond_serialized(q)
+    assert a == b
     srv.close()
 
 
