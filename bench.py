@@ -102,6 +102,23 @@ def cpu_baseline(plan, out_samples=32, fold_chains=2):
                         "fold": t_fold * 2 * plan["c"]}}
 
 
+def profiled_traffic(kernel_prefix):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
+    or None when no summary for it exists."""
+    path = os.path.join(ROOT, "profiles", "respond_1m_single_query_trace_and_hbm.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.split("|")]
+        if len(cells) > 5 and cells[1].startswith(kernel_prefix):
+            try:
+                return float(cells[4]) + float(cells[5])
+            except ValueError:
+                return None
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +127,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--queries", type=int, default=8, help="distinct resident queries cycled through")
+    ap.add_argument("--slots", type=int, default=2, help="requests in flight (request slots / streams)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
@@ -127,45 +145,61 @@ def main():
     srv.fill_synthetic(11)
     log("state resident in %.1fs" % (time.time() - t0))
     dq, qlen, stride = make_queries(IA, torch, db_rows, args.queries, 13)
-    out = torch.empty(2 * plan["c"] * IA.RESP_BYTES, dtype=torch.uint8, device="cuda")
-    # A dedicated stream (default priority): the work and the events that time
-    # it are on the same stream.
-    stream = torch.cuda.Stream()
-    sh = stream.cuda_stream
-    assert sh, "need a non-null stream handle"
+    # One output buffer and one stream per request slot: requests alternate
+    # between the streams, so one request's fold (22 workgroups) overlaps the
+    # next request's packing.  Streams use the default priority.
+    nslots = max(1, args.slots)
+    srv.set_slots(nslots)
+    outs = [torch.empty(2 * plan["c"] * IA.RESP_BYTES, dtype=torch.uint8, device="cuda") for _ in range(nslots)]
+    streams = [torch.cuda.Stream() for _ in range(nslots)]
+    assert all(st.cuda_stream for st in streams), "need non-null stream handles"
 
     def step(k):
-        srv.respond_device(dq.data_ptr() + (k % args.queries) * stride, qlen, 2, out, sh)
+        i = k % nslots
+        srv.respond_device(dq.data_ptr() + (k % args.queries) * stride, qlen, 2, outs[i], streams[i].cuda_stream)
 
     for k in range(args.warmup):
         step(k)
-    stream.synchronize()
     torch.cuda.synchronize()
     log("warmup done")
 
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    t_begin = torch.cuda.Event(enable_timing=True)
-    t_end = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t_begin.record(stream)
-    for k in range(args.steps):
-        starts[k].record(stream)
-        step(k)
-        ends[k].record(stream)
-    t_end.record(stream)
-    torch.cuda.synchronize()
-    total_ms = t_begin.elapsed_time(t_end)
-    lat = sorted(s.elapsed_time(e) for s, e in zip(starts, ends))
-    p = lambda q: lat[min(len(lat) - 1, int(round(q * (len(lat) - 1))))]
+    def timed(steps, nstreams):
+        starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+        ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+        t_begin = torch.cuda.Event(enable_timing=True)
+        t_end = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t_begin.record(streams[0])
+        for st in streams[1:nstreams]:
+            st.wait_event(t_begin)
+        for k in range(steps):
+            st = streams[k % nstreams]
+            starts[k].record(st)
+            i = k % nstreams
+            srv.respond_device(dq.data_ptr() + (k % args.queries) * stride, qlen, 2, outs[i], st.cuda_stream)
+            ends[k].record(st)
+        for k in range(max(0, steps - nstreams), steps):
+            streams[0].wait_event(ends[k])
+        t_end.record(streams[0])
+        torch.cuda.synchronize()
+        total = t_begin.elapsed_time(t_end)
+        lat = sorted(a.elapsed_time(b) for a, b in zip(starts, ends))
+        pct = lambda q: lat[min(len(lat) - 1, int(round(q * (len(lat) - 1))))]
+        return total, pct(0.5), pct(0.99)
+
+    # serial: one request at a time on one stream
+    ser_ms, ser_p50, ser_p99 = timed(max(10, args.steps // 4), 1)
+    ser_qps = max(10, args.steps // 4) / (ser_ms / 1e3)
+    log("serial: %.3f ms/query, %.1f q/s" % (ser_ms / max(10, args.steps // 4), ser_qps))
+    # pipelined over the slots: the timed region of the JSON line
+    total_ms, p50, p99 = timed(args.steps, nslots)
     qps = args.steps / (total_ms / 1e3)
-    log("timed %d steps: %.3f ms/step, %.1f q/s" % (args.steps, total_ms / args.steps, qps))
+    log("timed %d steps over %d slots: %.3f ms/step, %.1f q/s" % (args.steps, nslots, total_ms / args.steps, qps))
 
     # Per-stage kernel times (hipEvents between the kernels on the respond stream).
     srv.profile(True)
     for k in range(args.profile_steps):
         step(k)
-    stream.synchronize()
     torch.cuda.synchronize()
     stage_ms, cnt = srv.profile_read()
     srv.profile(False)
@@ -186,15 +220,18 @@ def main():
                    "dim0": plan["dim0"], "db_rows": db_rows, "db_cols": plan["db_cols"], "instances": inst,
                    "interpolate_degree": d, "chains": plan["c"], "hash_queries": 2,
                    "parallelism": "1 GPU"},
-        "p50_ms": p(0.5), "p99_ms": p(0.99),
+        "p50_ms": p50, "p99_ms": p99,
+        "serial": {"qps": ser_qps, "ms_per_query": ser_ms / max(10, args.steps // 4), "p50_ms": ser_p50,
+                   "p99_ms": ser_p99},
+        "slots": nslots,
         "stage_ms": stage,
         "roofline": {"bound": "hbm", "kernel": "k_packing (InspiRING key-switch inner product)",
                      "achieved": pack_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": pack_gbs / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes": pack_bytes,
+                     "traffic": profiled_traffic("inspire::k_packing"), "algorithmic_bytes": pack_bytes,
                      "gemv": {"achieved": gemv_gbs, "frac": gemv_gbs / HBM_PEAK_GBS, "algorithmic_bytes": gemv_bytes}},
     }
     srv.close()
-    del out, dq
+    del outs, dq
     torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         log("cpu baseline ...")

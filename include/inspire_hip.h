@@ -169,6 +169,11 @@ int inspire_respond_serialized(inspire_server_t* srv, const uint8_t* query, size
  * `stream` (hipStream_t; NULL = the server's own stream). */
 int inspire_respond_device(inspire_server_t* srv, const uint8_t* d_query, size_t len, int nq, uint8_t* d_out,
                            void* stream);
+/* Number of request slots (default 2, at most 16).  Consecutive respond calls
+ * use the slots in turn, so requests issued on different streams overlap on
+ * the device; a slot is reused only after its previous request completed. */
+int inspire_server_set_slots(inspire_server_t* srv, int count);
+
 /* First-pass kernel used by the server: INSPIRE_GEMV_MFMA (default, exact
  * int8-limb MFMA) or INSPIRE_GEMV_VALU (64-bit multiply-accumulate). */
 #define INSPIRE_GEMV_VALU 0

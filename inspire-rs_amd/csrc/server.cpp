@@ -206,6 +206,19 @@ extern "C" int inspire_plan_shape(int64_t record_count, int64_t key_size, int64_
 // Server
 // ---------------------------------------------------------------------------
 
+// Per-request device buffers.  A server owns several so that consecutive
+// requests overlap on the device (one request's fold with the next one's
+// packing); a slot is reused only after its previous request has finished
+// (its `done` event), whatever stream that request ran on.
+struct Slot {
+  uint8_t* qbytes = nullptr;
+  uint64_t *y_body = nullptr, *z_body = nullptr, *rows = nullptr, *P = nullptr, *cvec = nullptr;
+  uint32_t *bodies = nullptr, *S = nullptr;
+  uint8_t* out = nullptr;
+  void* gemv_scratch = nullptr;  // MFMA query fragments
+  hipEvent_t done = nullptr;
+};
+
 struct inspire_server {
   inspire_shape_t shape{};
   int64_t db_cols = 0;
@@ -219,13 +232,10 @@ struct inspire_server {
   bool have_db = false, have_precomp = false;
   std::vector<uint8_t> stash;  // entries back to back
   uint32_t stash_entry = 0, stash_count = 0;
-  // request buffers
+  // request slots
   size_t max_query_bytes = 0;
-  uint8_t* qbytes = nullptr;
-  uint64_t *y_body = nullptr, *z_body = nullptr, *rows = nullptr, *P = nullptr, *cvec = nullptr;
-  uint32_t *bodies = nullptr, *S = nullptr;
-  uint8_t* out = nullptr;
-  void* gemv_scratch = nullptr;  // MFMA query fragments
+  std::vector<Slot> slots;
+  size_t next_slot = 0;
   int gemv_kind = INSPIRE_GEMV_MFMA;
   std::vector<uint8_t> host_out;
   // optional per-stage timing
@@ -253,6 +263,43 @@ int dev_alloc(T** p, size_t count, const char* what) {
 
 }  // namespace
 
+constexpr int kDefaultSlots = 2;
+
+void free_slots(inspire_server_t* s) {
+  for (Slot& sl : s->slots) {
+    if (sl.done) (void)hipEventSynchronize(sl.done);
+    void* ptrs[] = {sl.qbytes, sl.y_body, sl.z_body, sl.rows, sl.bodies, sl.S, sl.P, sl.cvec, sl.out,
+                    sl.gemv_scratch};
+    for (void* p : ptrs)
+      if (p) (void)hipFree(p);
+    if (sl.done) (void)hipEventDestroy(sl.done);
+  }
+  s->slots.clear();
+  s->next_slot = 0;
+}
+
+int alloc_slots(inspire_server_t* s, int count) {
+  const int64_t inst = s->shape.instances, rows = s->shape.db_rows, mq = s->shape.max_queries;
+  s->slots.resize(count);
+  for (Slot& sl : s->slots) {
+    int rc = INSPIRE_OK;
+    if (!rc) rc = dev_alloc(&sl.qbytes, s->max_query_bytes + 16, "query bytes");
+    if (!rc) rc = dev_alloc(&sl.y_body, (size_t)kTExp * kN, "y_body");
+    if (!rc) rc = dev_alloc(&sl.z_body, (size_t)kTExp * kN, "z_body");
+    if (!rc) rc = dev_alloc(&sl.rows, (size_t)mq * rows, "query rows");
+    if (!rc) rc = dev_alloc(&sl.bodies, (size_t)mq * 6 * 2 * kN, "rgsw bodies");
+    if (!rc) rc = dev_alloc(&sl.S, (size_t)inst * 2 * kN, "packing output");
+    if (!rc) rc = dev_alloc(&sl.P, (size_t)inst * kN, "post output");
+    if (!rc) rc = dev_alloc(&sl.cvec, (size_t)mq * s->db_cols, "gemv output");
+    if (!rc) rc = dev_alloc(&sl.out, (size_t)mq * s->c * kRespBytes, "responses");
+    if (!rc) rc = dev_alloc((uint8_t**)&sl.gemv_scratch, gemv_mfma_scratch_bytes((int)rows), "gemv scratch");
+    if (!rc && hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess)
+      rc = set_error(INSPIRE_ERR_HIP, "hipEventCreate failed");
+    if (rc) return rc;
+  }
+  return INSPIRE_OK;
+}
+
 extern "C" int inspire_server_create(const inspire_shape_t* shape, inspire_server_t** out) {
   if (!shape || !out) return set_error(INSPIRE_ERR_INVALID, "null argument");
   *out = nullptr;
@@ -277,30 +324,21 @@ extern "C" int inspire_server_create(const inspire_shape_t* shape, inspire_serve
     delete s;
     return set_error(INSPIRE_ERR_HIP, "hipStreamCreate failed");
   }
-  const int64_t inst = shape->instances, rows = shape->db_rows, mq = shape->max_queries;
-  s->max_query_bytes = query_bytes(rows, (int)mq);
+  const int64_t inst = shape->instances, rows = shape->db_rows;
+  s->max_query_bytes = query_bytes(rows, shape->max_queries);
   int rc = INSPIRE_OK;
-  if (!rc) rc = dev_alloc(&s->qbytes, s->max_query_bytes + 16, "query bytes");
-  if (!rc) rc = dev_alloc(&s->y_body, (size_t)kTExp * kN, "y_body");
-  if (!rc) rc = dev_alloc(&s->z_body, (size_t)kTExp * kN, "z_body");
-  if (!rc) rc = dev_alloc(&s->rows, (size_t)mq * rows, "query rows");
-  if (!rc) rc = dev_alloc(&s->bodies, (size_t)mq * 6 * 2 * kN, "rgsw bodies");
-  if (!rc) rc = dev_alloc(&s->S, (size_t)inst * 2 * kN, "packing output");
-  if (!rc) rc = dev_alloc(&s->P, (size_t)inst * kN, "post output");
-  if (!rc) rc = dev_alloc(&s->cvec, (size_t)mq * s->db_cols, "gemv output");
-  if (!rc) rc = dev_alloc(&s->out, (size_t)mq * s->c * kRespBytes, "responses");
-  if (!rc) rc = dev_alloc((uint8_t**)&s->gemv_scratch, gemv_mfma_scratch_bytes((int)rows) * 2, "gemv scratch");
   if (!rc) rc = dev_alloc(&s->db, (size_t)rows * s->db_cols, "database");
   if (!rc) rc = dev_alloc(&s->bt, (size_t)inst * kNInner * kN, "bold_t");
   if (!rc) rc = dev_alloc(&s->btb, (size_t)inst * kNInner * kN, "bold_t_bar");
   if (!rc) rc = dev_alloc(&s->bth, (size_t)inst * kTExp * kN, "bold_t_hat");
   if (!rc) rc = dev_alloc(&s->a_hat, (size_t)inst * kN, "a_hat");
+  if (!rc) rc = alloc_slots(s, kDefaultSlots);
   if (rc) {
     std::string msg = g_last_error;
     inspire_server_destroy(s);
     return set_error(rc, msg);
   }
-  s->host_out.resize((size_t)mq * s->c * kRespBytes);
+  s->host_out.resize((size_t)shape->max_queries * s->c * kRespBytes);
   *out = s;
   return INSPIRE_OK;
 }
@@ -308,14 +346,21 @@ extern "C" int inspire_server_create(const inspire_shape_t* shape, inspire_serve
 extern "C" void inspire_server_destroy(inspire_server_t* s) {
   if (!s) return;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
-  void* ptrs[] = {s->qbytes, s->y_body, s->z_body, s->rows, s->bodies, s->S,     s->P,           s->cvec,
-                  s->out,    s->db,     s->bt,     s->btb,  s->bth,    s->a_hat, s->gemv_scratch};
+  free_slots(s);
+  void* ptrs[] = {s->db, s->bt, s->btb, s->bth, s->a_hat};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   for (auto& e : s->ev)
     if (e) (void)hipEventDestroy(e);
   delete s;
+}
+
+extern "C" int inspire_server_set_slots(inspire_server_t* s, int count) {
+  if (!s || count < 1 || count > 16) return set_error(INSPIRE_ERR_INVALID, "slot count must be in [1, 16]");
+  HIP_TRY(hipDeviceSynchronize(), "drain before resizing slots");
+  free_slots(s);
+  return alloc_slots(s, count);
 }
 
 extern "C" void* inspire_server_stream(inspire_server_t* s) { return s ? (void*)s->stream : nullptr; }
@@ -411,35 +456,38 @@ namespace {
 // (packing_online_kernel.cu:46-91), so it runs once per keyword query and
 // serves every hash query; the reference repeats it per hash query with the
 // same result.
-int run_pipeline(inspire_server_t* s, const uint8_t* d_query, int nq, uint8_t* d_out, hipStream_t st) {
+int run_pipeline(inspire_server_t* s, Slot& sl, const uint8_t* d_query, int nq, uint8_t* d_out, hipStream_t st) {
   const int64_t rows = s->shape.db_rows, inst = s->shape.instances;
+  // the slot's previous request (on whatever stream) must be finished
+  HIP_TRY(hipStreamWaitEvent(st, sl.done, 0), "slot wait");
   auto mark = [&](int i) {
     if (s->profile) (void)hipEventRecord(s->ev[i], st);
   };
   mark(0);
-  launch_unpack_query(d_query, nq, (int)rows, s->y_body, s->z_body, s->rows, s->bodies, st);
+  launch_unpack_query(d_query, nq, (int)rows, sl.y_body, sl.z_body, sl.rows, sl.bodies, st);
   mark(1);
-  launch_packing(s->S, s->bt, s->btb, s->bth, s->y_body, s->z_body, (int)inst, st);
+  launch_packing(sl.S, s->bt, s->btb, s->bth, sl.y_body, sl.z_body, (int)inst, st);
   mark(2);
-  launch_post(s->P, s->S, s->consts->tw, (int)inst, st);
+  launch_post(sl.P, sl.S, s->consts->tw, (int)inst, st);
   mark(3);
   if (s->gemv_kind == INSPIRE_GEMV_MFMA) {
     // two hash queries per MFMA tile; larger nq runs in pairs
     for (int h = 0; h < nq; h += 2) {
       const int m = nq - h < 2 ? nq - h : 2;
-      if (launch_gemv_mfma(s->cvec + (size_t)h * s->db_cols, s->rows + (size_t)h * rows, m, s->db, (int)rows,
-                           s->db_cols, s->gemv_scratch, st))
+      if (launch_gemv_mfma(sl.cvec + (size_t)h * s->db_cols, sl.rows + (size_t)h * rows, m, s->db, (int)rows,
+                           s->db_cols, sl.gemv_scratch, st))
         return set_error(INSPIRE_ERR_INVALID, "unsupported GEMV shape");
     }
-  } else if (launch_gemv(s->cvec, s->rows, nq, s->db, (int)rows, s->db_cols, st)) {
+  } else if (launch_gemv(sl.cvec, sl.rows, nq, s->db, (int)rows, s->db_cols, st)) {
     return set_error(INSPIRE_ERR_INVALID, "unsupported GEMV shape");
   }
   mark(4);
-  if (launch_fold(d_out, s->a_hat, s->cvec, s->P, s->bodies, s->consts->neg_a, s->consts->tw, inst,
+  if (launch_fold(d_out, s->a_hat, sl.cvec, sl.P, sl.bodies, s->consts->neg_a, s->consts->tw, inst,
                   s->shape.interpolate_degree, nq, st))
     return set_error(INSPIRE_ERR_INVALID, "unsupported fold shape");
   mark(5);
   HIP_TRY(hipGetLastError(), "respond launch");
+  HIP_TRY(hipEventRecord(sl.done, st), "slot done");
   if (s->profile) {
     HIP_TRY(hipEventSynchronize(s->ev[5]), "profile sync");
     for (int i = 0; i < 5; ++i) {
@@ -452,6 +500,12 @@ int run_pipeline(inspire_server_t* s, const uint8_t* d_query, int nq, uint8_t* d
   return INSPIRE_OK;
 }
 
+Slot& take_slot(inspire_server_t* s) {
+  Slot& sl = s->slots[s->next_slot % s->slots.size()];
+  ++s->next_slot;
+  return sl;
+}
+
 }  // namespace
 
 extern "C" int inspire_respond_device(inspire_server_t* s, const uint8_t* d_query, size_t len, int nq,
@@ -462,7 +516,7 @@ extern "C" int inspire_respond_device(inspire_server_t* s, const uint8_t* d_quer
   if (len < query_bytes(s->shape.db_rows, nq)) return set_error(INSPIRE_ERR_INVALID, "keyword query truncated");
   if (nq == 0) return INSPIRE_OK;
   hipStream_t st = stream ? (hipStream_t)stream : s->stream;
-  return run_pipeline(s, d_query, nq, d_out, st);
+  return run_pipeline(s, take_slot(s), d_query, nq, d_out, st);
 }
 
 extern "C" int inspire_respond_serialized(inspire_server_t* s, const uint8_t* query, size_t len, uint8_t* resp,
@@ -479,10 +533,12 @@ extern "C" int inspire_respond_serialized(inspire_server_t* s, const uint8_t* qu
   if (!resp || cap < out_bytes) return set_error(INSPIRE_ERR_CAPACITY, "response buffer too small");
   const size_t blob = (size_t)s->c * kRespBytes;
   if (nq > 0) {
-    HIP_TRY(hipMemcpyAsync(s->qbytes, query, need, hipMemcpyHostToDevice, s->stream), "upload query");
-    int rc = run_pipeline(s, s->qbytes, nq, s->out, s->stream);
+    Slot& sl = take_slot(s);
+    HIP_TRY(hipStreamWaitEvent(s->stream, sl.done, 0), "slot wait");
+    HIP_TRY(hipMemcpyAsync(sl.qbytes, query, need, hipMemcpyHostToDevice, s->stream), "upload query");
+    int rc = run_pipeline(s, sl, sl.qbytes, nq, sl.out, s->stream);
     if (rc) return rc;
-    HIP_TRY(hipMemcpyAsync(s->host_out.data(), s->out, (size_t)nq * blob, hipMemcpyDeviceToHost, s->stream),
+    HIP_TRY(hipMemcpyAsync(s->host_out.data(), sl.out, (size_t)nq * blob, hipMemcpyDeviceToHost, s->stream),
             "download responses");
     HIP_TRY(hipStreamSynchronize(s->stream), "respond");
   }

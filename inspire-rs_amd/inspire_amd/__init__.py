@@ -80,6 +80,7 @@ def lib():
         L.inspire_gemv_mfma_scratch_bytes.argtypes = [i64]
         L.inspire_gemv_mfma_scratch_bytes.restype = sz
         L.inspire_server_set_gemv_kernel.argtypes = [vp, i32]
+        L.inspire_server_set_slots.argtypes = [vp, i32]
         L.inspire_packing_device.argtypes = [vp, vp, vp, vp, vp, vp, i64, vp]
         L.inspire_post_device.argtypes = [vp, vp, i64, vp]
         L.inspire_fold_device.argtypes = [vp, vp, vp, vp, vp, i64, i32, i32, vp]
@@ -167,6 +168,9 @@ class Server:
         check(lib().inspire_respond_serialized(self.h, ctypes.cast(qb, ctypes.c_void_p), len(query),
                                                ctypes.cast(out, ctypes.c_void_p), cap, ctypes.byref(n)))
         return out.raw[:n.value]
+
+    def set_slots(self, count):
+        check(lib().inspire_server_set_slots(self.h, count))
 
     def set_gemv_kernel(self, kind):
         check(lib().inspire_server_set_gemv_kernel(self.h, kind))
