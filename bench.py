@@ -109,9 +109,11 @@ def profiled_traffic(kernel_prefix):
     path = os.path.join(ROOT, "profiles", "respond_1m_single_query_trace_and_hbm.md")
     if not os.path.exists(path):
         return None
+    in_hbm = False
     for line in open(path):
+        in_hbm = in_hbm or line.startswith("## HBM counters")
         cells = [c.strip() for c in line.split("|")]
-        if len(cells) > 5 and cells[1].startswith(kernel_prefix):
+        if in_hbm and len(cells) > 5 and cells[1].startswith(kernel_prefix):
             try:
                 return float(cells[4]) + float(cells[5])
             except ValueError:
