@@ -130,6 +130,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--queries", type=int, default=8, help="distinct resident queries cycled through")
     ap.add_argument("--slots", type=int, default=2, help="requests in flight (request slots / streams)")
+    ap.add_argument("--batches", default="1,8,32,64,128", help="batch sizes for the batched respond ('' to skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
@@ -232,6 +233,45 @@ def main():
                      "traffic": profiled_traffic("inspire::k_packing"), "algorithmic_bytes": pack_bytes,
                      "gemv": {"achieved": gemv_gbs, "frac": gemv_gbs / HBM_PEAK_GBS, "algorithmic_bytes": gemv_bytes}},
     }
+    # Batched respond: B keyword queries per pass over the DB and bold_t.
+    batched = {}
+    if args.batches:
+        sizes = [int(x) for x in args.batches.split(",") if x]
+        t0 = time.time()
+        srv.set_max_batch(max(sizes))
+        log("batch buffers + fragment-ordered bold_t built in %.1fs" % (time.time() - t0))
+        single_ms = total_ms / args.steps
+        for B in sizes:
+            bq, bqlen, bstride = make_queries(IA, torch, db_rows, min(B, args.queries), 17)
+            bout = torch.empty(B * 2 * plan["c"] * IA.RESP_BYTES, dtype=torch.uint8, device="cuda")
+            blob = 2 * plan["c"] * IA.RESP_BYTES
+            # cycle the distinct queries over the B positions of the batch
+            if B > args.queries:
+                reps = (B + args.queries - 1) // args.queries
+                bq = bq.repeat(reps)[:B * bstride].contiguous()
+            st = streams[0]
+
+            def run_batch():
+                srv.respond_batch_device(bq, bstride, bqlen, 2, B, bout, blob, st.cuda_stream)
+
+            run_batch()
+            torch.cuda.synchronize()
+            reps = max(2, min(20, int(4000 / (B * single_ms + 1))))
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+            ev[0].record(st)
+            for r in range(reps):
+                run_batch()
+                ev[r + 1].record(st)
+            torch.cuda.synchronize()
+            per = sorted(ev[r].elapsed_time(ev[r + 1]) for r in range(reps))
+            tot = ev[0].elapsed_time(ev[reps])
+            qps_b = B * reps / (tot / 1e3)
+            batched[str(B)] = {"qps": qps_b, "ms_per_batch": tot / reps, "p50_batch_ms": per[len(per) // 2],
+                               "p99_batch_ms": per[-1], "speedup_vs_single_query": qps_b / qps}
+            log("batch %d: %.2f ms/batch, %.1f q/s (%.1fx single)" % (B, tot / reps, qps_b, qps_b / qps))
+            del bq, bout
+    result["batched"] = batched
+
     srv.close()
     del outs, dq
     torch.cuda.empty_cache()

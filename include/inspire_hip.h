@@ -174,6 +174,27 @@ int inspire_respond_device(inspire_server_t* srv, const uint8_t* d_query, size_t
  * the device; a slot is reused only after its previous request completed. */
 int inspire_server_set_slots(inspire_server_t* srv, int count);
 
+/* Packing inner-product kernel of the single-query path: INSPIRE_PACKING_VALU
+ * (default; reads bold_t in the reference's condensed layout) or
+ * INSPIRE_PACKING_MFMA (int8-limb MFMA over a fragment-ordered copy of bold_t
+ * that this call builds on the device, 8 bytes per coefficient). */
+#define INSPIRE_PACKING_VALU 0
+#define INSPIRE_PACKING_MFMA 1
+int inspire_server_set_packing_kernel(inspire_server_t* srv, int kind);
+
+/* Batched respond.  set_max_batch allocates buffers for up to max_batch
+ * keyword queries per call (and builds the fragment-ordered bold_t).
+ * respond_batch_device answers `count` keyword queries in one pass over the
+ * database and the key-switch material: query i is serialized at
+ * d_queries + i*q_stride (len bytes each, nq hash queries each), its
+ * responses go to d_out + i*out_stride (nq*(instances/d)*12288 bytes), each
+ * identical to what inspire_respond_device returns for that query alone.
+ * Asynchronous on `stream` (NULL = the server's stream); a second batch waits
+ * for the first one's buffers. */
+int inspire_server_set_max_batch(inspire_server_t* srv, int max_batch);
+int inspire_respond_batch_device(inspire_server_t* srv, const uint8_t* d_queries, size_t q_stride, size_t len,
+                                 int nq, int count, uint8_t* d_out, size_t out_stride, void* stream);
+
 /* First-pass kernel used by the server: INSPIRE_GEMV_MFMA (default, exact
  * int8-limb MFMA) or INSPIRE_GEMV_VALU (64-bit multiply-accumulate). */
 #define INSPIRE_GEMV_VALU 0

@@ -34,8 +34,9 @@ constexpr int kColsPerBlock = 4 * kTilesPerWave * 16;        // 128
 __host__ __device__ inline int frag_row(int s, int g, int t) { return 64 * s + 8 * g + t + (t >= 8 ? 24 : 0); }
 
 // afrag: [db_rows/64][64 lanes][16 B]; rsum: [16] int32 (row i = 8h+4p+l).
-__global__ void k_gemv_prep(const uint64_t* __restrict__ rows, int nq, int db_rows, int8_t* __restrict__ afrag,
-                            int32_t* __restrict__ rsum) {
+// Called by every thread of a 1-D grid (blockIdx.x, threadIdx.x).
+__device__ void k_gemv_prep_body(const uint64_t* __restrict__ rows, int nq, int db_rows, int8_t* __restrict__ afrag,
+                                 int32_t* __restrict__ rsum) {
   __shared__ int32_t part[16];
   if (threadIdx.x < 16) part[threadIdx.x] = 0;
   __syncthreads();
@@ -65,6 +66,11 @@ __global__ void k_gemv_prep(const uint64_t* __restrict__ rows, int nq, int db_ro
     if (local[i]) atomicAdd(&part[i], local[i]);
   __syncthreads();
   if (threadIdx.x < 16) atomicAdd(&rsum[threadIdx.x], part[threadIdx.x]);
+}
+
+__global__ void k_gemv_prep(const uint64_t* __restrict__ rows, int nq, int db_rows, int8_t* __restrict__ afrag,
+                            int32_t* __restrict__ rsum) {
+  k_gemv_prep_body(rows, nq, db_rows, afrag, rsum);
 }
 
 __device__ inline uint32_t plane_lo(uint32_t a, uint32_t b) {
@@ -138,6 +144,127 @@ __global__ __launch_bounds__(256) void k_gemv_mfma(uint64_t* __restrict__ c, con
     const int64_t col = col_base + tt * 16 + j;
     if (p == 0 && h < nq && col < db_cols) c[(int64_t)h * db_cols + col] = crt_compose((uint32_t)red, (uint32_t)other);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Batched form: MQ keyword queries (2 hash queries each, one 16-row M-tile per
+// keyword query) share every DB fragment.  Block = 4 waves x 16 columns; the
+// query fragments of a 256-row K chunk are staged in LDS for all MQ queries.
+// afrag: [client][db_rows/64][64][16]; rsum: [client][16].
+// ---------------------------------------------------------------------------
+
+constexpr int kBatchChunkSteps = 4;  // 256 rows per LDS chunk
+
+template <int MQ>
+__global__ __launch_bounds__(256) void k_gemv_mfma_batch(uint64_t* __restrict__ c, int64_t c_client_stride,
+                                                         const int8_t* __restrict__ afrag,
+                                                         const int32_t* __restrict__ rsum,
+                                                         const uint16_t* __restrict__ db, int nq, int db_rows,
+                                                         int64_t db_cols, int client0, int count) {
+  __shared__ v4i a_lds[MQ][kBatchChunkSteps][64];  // MQ x 4 KiB
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int j = lane & 15, g = lane >> 4;
+  const int64_t col = (int64_t)blockIdx.x * 64 + wave * 16 + j;
+  const bool valid = col < db_cols;
+  const uint16_t* colp = db + (valid ? col : 0) * (int64_t)db_rows + 8 * g;
+  const int steps = db_rows / 64;
+  const int nclients = min(MQ, count - client0);
+
+  v4i acc_lo[MQ], acc_hi[MQ];
+#pragma unroll
+  for (int q = 0; q < MQ; ++q) {
+    acc_lo[q] = v4i{0, 0, 0, 0};
+    acc_hi[q] = v4i{0, 0, 0, 0};
+  }
+  for (int s0 = 0; s0 < steps; s0 += kBatchChunkSteps) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < MQ * kBatchChunkSteps * 64; e += 256) {
+      const int q = e / (kBatchChunkSteps * 64), r = e % (kBatchChunkSteps * 64);
+      v4i v = {0, 0, 0, 0};
+      if (q < nclients)
+        v = reinterpret_cast<const v4i*>(afrag)[((int64_t)(client0 + q) * steps + s0) * 64 + r];
+      a_lds[q][r / 64][r % 64] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ss = 0; ss < kBatchChunkSteps; ++ss) {
+      const uint16_t* p = colp + 64 * (s0 + ss);
+      u4 d0 = {0, 0, 0, 0}, d1 = {0, 0, 0, 0};
+      if (valid) {
+        d0 = __builtin_nontemporal_load(reinterpret_cast<const u4*>(p));
+        d1 = __builtin_nontemporal_load(reinterpret_cast<const u4*>(p + 32));
+      }
+      const v4i lo = {(int)plane_lo(d0.x, d0.y), (int)plane_lo(d0.z, d0.w), (int)plane_lo(d1.x, d1.y),
+                      (int)plane_lo(d1.z, d1.w)};
+      const v4i hi = {(int)plane_hi(d0.x, d0.y), (int)plane_hi(d0.z, d0.w), (int)plane_hi(d1.x, d1.y),
+                      (int)plane_hi(d1.z, d1.w)};
+#pragma unroll
+      for (int q = 0; q < MQ; ++q) {
+        const v4i a = a_lds[q][ss][lane];
+        acc_lo[q] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, lo, acc_lo[q], 0, 0, 0);
+        acc_hi[q] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, hi, acc_hi[q], 0, 0, 0);
+      }
+    }
+  }
+  const int h = g >> 1, pr = g & 1;
+#pragma unroll
+  for (int q = 0; q < MQ; ++q) {
+    const int client = client0 + q;
+    int64_t v = 0;
+    if (client < count) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        v += ((int64_t)256 * acc_hi[q][r] + (int64_t)acc_lo[q][r] + (int64_t)rsum[client * 16 + 4 * g + r] * 32896)
+             << (7 * r);
+    }
+    const uint64_t red = pr ? barrett64((uint64_t)v, kCr1Q1, kQ1) : barrett64((uint64_t)v, kCr1Q0, kQ0);
+    const uint64_t other = __shfl_down(red, 16, 64);
+    if (pr == 0 && h < nq && valid && client < count)
+      c[(int64_t)client * c_client_stride + (int64_t)h * db_cols + col] = crt_compose((uint32_t)red, (uint32_t)other);
+  }
+}
+
+// Fragments of `count` clients, each with nq <= 2 rows of db_rows (stride
+// rows_client_stride elements between clients).
+__global__ void k_gemv_prep_batch(const uint64_t* __restrict__ rows, int64_t rows_client_stride, int nq, int db_rows,
+                                  int count, int8_t* __restrict__ afrag, int32_t* __restrict__ rsum) {
+  const int client = blockIdx.y;
+  if (client >= count) return;
+  k_gemv_prep_body(rows + (int64_t)client * rows_client_stride, nq, db_rows,
+                   afrag + (int64_t)client * (db_rows / 64) * 64 * 16, rsum + client * 16);
+}
+
+size_t gemv_mfma_batch_scratch_bytes(int db_rows, int count) {
+  return (size_t)count * ((size_t)(db_rows / 64) * 64 * 16 + 16 * sizeof(int32_t));
+}
+
+int launch_gemv_mfma_batch(uint64_t* c, int64_t c_client_stride, const uint64_t* rows, int64_t rows_client_stride,
+                           int nq, const uint16_t* db, int db_rows, int64_t db_cols, int count, void* scratch,
+                           hipStream_t s) {
+  if (nq < 1 || nq > 2 || count < 1 || db_rows <= 0 || db_rows % 256 != 0 || db_rows > (1 << 17) || db_cols <= 0)
+    return -1;
+  int8_t* afrag = static_cast<int8_t*>(scratch);
+  int32_t* rsum = reinterpret_cast<int32_t*>(afrag + (size_t)count * (db_rows / 64) * 64 * 16);
+  hipLaunchKernelGGL(k_zero_u32, dim3((count * 16 + 255) / 256), dim3(256), 0, s, reinterpret_cast<uint32_t*>(rsum),
+                     count * 16);
+  const int prep_threads = (db_rows / 64) * 64;
+  hipLaunchKernelGGL(k_gemv_prep_batch, dim3((prep_threads + 255) / 256, count), dim3(256), 0, s, rows,
+                     rows_client_stride, nq, db_rows, count, afrag, rsum);
+  const unsigned blocks = (unsigned)((db_cols + 63) / 64);
+  for (int c0 = 0; c0 < count; c0 += 16) {
+    const int left = count - c0;
+    if (left > 8)
+      hipLaunchKernelGGL(k_gemv_mfma_batch<16>, dim3(blocks), dim3(256), 0, s, c, c_client_stride, afrag, rsum, db, nq,
+                         db_rows, db_cols, c0, count);
+    else if (left > 4)
+      hipLaunchKernelGGL(k_gemv_mfma_batch<8>, dim3(blocks), dim3(256), 0, s, c, c_client_stride, afrag, rsum, db, nq,
+                         db_rows, db_cols, c0, count);
+    else
+      hipLaunchKernelGGL(k_gemv_mfma_batch<4>, dim3(blocks), dim3(256), 0, s, c, c_client_stride, afrag, rsum, db, nq,
+                         db_rows, db_cols, c0, count);
+  }
+  return 0;
 }
 
 size_t gemv_mfma_scratch_bytes(int db_rows) { return (size_t)(db_rows / 64) * 64 * 16 + 16 * sizeof(int32_t); }

@@ -106,6 +106,57 @@ __global__ void k_unpack_query(const uint8_t* __restrict__ q, int nq, int db_row
   }
 }
 
+__global__ void k_unpack_query_batch(const uint8_t* __restrict__ q, int64_t q_stride, int nq, int db_rows,
+                                     uint64_t* __restrict__ y_body, uint64_t* __restrict__ z_body,
+                                     uint64_t* __restrict__ rows, uint32_t* __restrict__ bodies) {
+  const int b = blockIdx.y;
+  const int64_t keys = (int64_t)kTExp * kN;
+  const uint8_t* qb = q + (int64_t)b * q_stride;
+  const int64_t per = (int64_t)nq;
+  // Same field walk as k_unpack_query, client b's buffers.
+  const int64_t kf = (int64_t)kTExp * kN * 2;
+  const int64_t per_q = (int64_t)db_rows + 6 * kN;
+  const int64_t total = kf + per * per_q;
+  const uint64_t m28 = (1ull << 28) - 1;
+  uint64_t* yb = y_body + b * keys;
+  uint64_t* zb = z_body + b * keys;
+  uint64_t* rb = rows + b * per * db_rows;
+  uint32_t* bb = bodies + b * per * 12 * kN;
+  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < total; f += (int64_t)gridDim.x * blockDim.x) {
+    const uint8_t* p = qb + 1 + 7 * f;
+    uint64_t v = 0;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) v |= (uint64_t)p[i] << (8 * i);
+    const uint64_t lo = v & m28, hi = (v >> 28) & m28;
+    if (f < kf) {
+      if (f & 1)
+        zb[f >> 1] = lo | (hi << 32);
+      else
+        yb[f >> 1] = lo | (hi << 32);
+    } else {
+      const int64_t g = f - kf;
+      const int64_t qi = g / per_q, r = g % per_q;
+      if (r < db_rows) {
+        rb[qi * db_rows + r] = lo | (hi << 32);
+      } else {
+        const int64_t e = r - db_rows;
+        const int64_t i = e / kN, z = e % kN;
+        bb[((qi * 6 + i) * 2 + 0) * kN + z] = (uint32_t)lo;
+        bb[((qi * 6 + i) * 2 + 1) * kN + z] = (uint32_t)hi;
+      }
+    }
+  }
+}
+
+void launch_unpack_query_batch(const uint8_t* q, int64_t q_stride, int nq, int db_rows, int count, uint64_t* y_body,
+                               uint64_t* z_body, uint64_t* rows, uint32_t* bodies, hipStream_t s) {
+  const int64_t total = (int64_t)kTExp * kN * 2 + (int64_t)nq * (db_rows + 6 * kN);
+  int blocks = (int)((total + 255) / 256);
+  if (blocks > 1024) blocks = 1024;
+  hipLaunchKernelGGL(k_unpack_query_batch, dim3(blocks, count), dim3(256), 0, s, q, q_stride, nq, db_rows, y_body,
+                     z_body, rows, bodies);
+}
+
 void launch_unpack_query(const uint8_t* q, int nq, int db_rows, uint64_t* y_body, uint64_t* z_body,
                          uint64_t* rows, uint32_t* bodies, hipStream_t s) {
   const int64_t total = (int64_t)kTExp * kN * 2 + (int64_t)nq * (db_rows + 6 * kN);
@@ -415,16 +466,18 @@ void launch_post(uint64_t* P, const uint32_t* S, const uint32_t* tw, int count, 
 constexpr int kFoldThreads = 1024;
 
 struct FoldArgs {
-  uint8_t* out;              // [nq][c][12288]
+  uint8_t* out;              // [clients] x out_stride bytes, each [nq][c][12288]
   const uint64_t* a_hat;     // [instances][N]
-  const uint64_t* cvec;      // [nq][instances*N]
-  const uint64_t* P;         // [instances][N]
-  const uint32_t* bodies;    // [nq][6][2][N]
+  const uint64_t* cvec;      // [clients][nq][instances*N]
+  const uint64_t* P;         // [clients][instances][N]
+  const uint32_t* bodies;    // [clients][nq][6][2][N]
   const uint32_t* neg_a;     // [2][N]
   const uint32_t* tw;        // NTT tables
   int64_t instances;
+  int64_t out_stride;        // bytes between clients' outputs
   int d;
   int c;
+  int nq;
 };
 
 __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
@@ -432,12 +485,15 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
   uint32_t* dig = lds;                       // [2][6][N]
   uint32_t* prod = lds + 2 * 6 * kN;         // [2 (prime)][2 (row)][N]
   const NttTables tw{args.tw};
-  const int h = blockIdx.x / args.c;
+  const int per_client = args.nq * args.c;
+  const int b = blockIdx.x / per_client;
+  const int h = (blockIdx.x % per_client) / args.c;
   const int w = blockIdx.x % args.c;
   const int tid = threadIdx.x;
   const int d = args.d;
-  const uint64_t* cv = args.cvec + (size_t)h * args.instances * kN;
-  const uint32_t* body = args.bodies + (size_t)h * 6 * 2 * kN;
+  const uint64_t* cv = args.cvec + ((size_t)b * args.nq + h) * args.instances * kN;
+  const uint64_t* Pc = args.P + (size_t)b * args.instances * kN;
+  const uint32_t* body = args.bodies + ((size_t)b * args.nq + h) * 6 * 2 * kN;
 
   // sum[r][zz], zz in {tid, tid+1024}
   uint64_t sum[2][2];
@@ -447,7 +503,7 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
     for (int s = 0; s < 2; ++s) {
       const int z = tid + s * 1024;
       sum[0][s] = args.a_hat[j * kN + z];
-      sum[1][s] = reduce_small_q(cv[j * kN + z] + args.P[j * kN + z]);
+      sum[1][s] = reduce_small_q(cv[j * kN + z] + Pc[j * kN + z]);
     }
   }
 
@@ -512,7 +568,7 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
       const uint64_t x0 = crt_compose(prod[0 * kN + z], prod[2 * kN + z]);
       const uint64_t x1 = crt_compose(prod[1 * kN + z], prod[3 * kN + z]);
       sum[0][s] = reduce_small_q(x0 + args.a_hat[j * kN + z]);
-      sum[1][s] = reduce_small_q(x1 + cv[j * kN + z] + args.P[j * kN + z]);
+      sum[1][s] = reduce_small_q(x1 + cv[j * kN + z] + Pc[j * kN + z]);
     }
     // The next iteration writes dig only; prod is rewritten after two barriers.
   }
@@ -528,7 +584,8 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
     r1[z] = (uint32_t)rescale_from_q(sum[1][s], kQPrime1);
   }
   __syncthreads();
-  uint32_t* out32 = reinterpret_cast<uint32_t*>(args.out + ((size_t)h * args.c + w) * kRespBytes);
+  uint32_t* out32 =
+      reinterpret_cast<uint32_t*>(args.out + (size_t)b * args.out_stride + ((size_t)h * args.c + w) * kRespBytes);
   constexpr int kRow0Bits = 28 * kN;
   constexpr int kTotalBits = 28 * kN + 20 * kN;
   for (int wd = tid; wd < kRespBytes / 4; wd += kFoldThreads) {
@@ -561,8 +618,14 @@ size_t fold_lds_bytes() { return (size_t)(2 * 6 * kN + 4 * kN) * sizeof(uint32_t
 int launch_fold(uint8_t* out, const uint64_t* a_hat, const uint64_t* cvec, const uint64_t* P,
                 const uint32_t* bodies, const uint32_t* neg_a, const uint32_t* tw, int64_t instances, int d,
                 int nq, hipStream_t s) {
-  if (d <= 0 || instances % d != 0) return -1;
-  FoldArgs a{out, a_hat, cvec, P, bodies, neg_a, tw, instances, d, (int)(instances / d)};
+  return launch_fold_batch(out, 0, a_hat, cvec, P, bodies, neg_a, tw, instances, d, nq, 1, s);
+}
+
+int launch_fold_batch(uint8_t* out, int64_t out_stride, const uint64_t* a_hat, const uint64_t* cvec,
+                      const uint64_t* P, const uint32_t* bodies, const uint32_t* neg_a, const uint32_t* tw,
+                      int64_t instances, int d, int nq, int clients, hipStream_t s) {
+  if (d <= 0 || instances % d != 0 || nq <= 0 || clients <= 0) return -1;
+  FoldArgs a{out, a_hat, cvec, P, bodies, neg_a, tw, instances, out_stride, d, (int)(instances / d), nq};
   static bool attr_set = false;
   if (!attr_set) {
     // Raises the dynamic-LDS ceiling for this kernel (128 KB of the 160 KB per CU).
@@ -570,7 +633,7 @@ int launch_fold(uint8_t* out, const uint64_t* a_hat, const uint64_t* cvec, const
                               (int)fold_lds_bytes());
     attr_set = true;
   }
-  hipLaunchKernelGGL(k_fold, dim3(nq * a.c), dim3(kFoldThreads), fold_lds_bytes(), s, a);
+  hipLaunchKernelGGL(k_fold, dim3(clients * nq * a.c), dim3(kFoldThreads), fold_lds_bytes(), s, a);
   return 0;
 }
 

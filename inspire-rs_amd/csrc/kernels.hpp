@@ -17,11 +17,38 @@ void launch_unpack_query(const uint8_t* q, int nq, int db_rows, uint64_t* y_body
 
 __global__ void k_zero_u32(uint32_t* p, int n);
 
+// Batched unpack: client b's query at q + b*q_stride; outputs per client
+// y/z [3][N], rows [nq][db_rows], bodies [nq][6][2][N].
+void launch_unpack_query_batch(const uint8_t* q, int64_t q_stride, int nq, int db_rows, int count, uint64_t* y_body,
+                               uint64_t* z_body, uint64_t* rows, uint32_t* bodies, hipStream_t s);
+// Fold for `clients` keyword queries; client b's responses at out + b*out_stride.
+int launch_fold_batch(uint8_t* out, int64_t out_stride, const uint64_t* a_hat, const uint64_t* cvec,
+                      const uint64_t* P, const uint32_t* bodies, const uint32_t* neg_a, const uint32_t* tw,
+                      int64_t instances, int d, int nq, int clients, hipStream_t s);
+
 // Exact int8-limb MFMA GEMV for nq <= 2 hash queries; scratch holds the query
 // fragments (gemv_mfma_scratch_bytes(db_rows) bytes, 16-byte aligned).
 size_t gemv_mfma_scratch_bytes(int db_rows);
 int launch_gemv_mfma(uint64_t* c, const uint64_t* rows, int nq, const uint16_t* db, int db_rows, int64_t db_cols,
                      void* scratch, hipStream_t s);
+
+// Batched: `count` keyword queries of nq <= 2 rows each; c[client][h][db_cols].
+size_t gemv_mfma_batch_scratch_bytes(int db_rows, int count);
+int launch_gemv_mfma_batch(uint64_t* c, int64_t c_client_stride, const uint64_t* rows, int64_t rows_client_stride,
+                           int nq, const uint16_t* db, int db_rows, int64_t db_cols, int count, void* scratch,
+                           hipStream_t s);
+
+// Tiled int8-limb key-switch material for the MFMA packing (packing_mfma.hip).
+size_t tiled_bold_t_bytes(int64_t instances);
+void launch_tile_bold_t(uint8_t* out, const uint64_t* bt, const uint64_t* btb, const uint64_t* bth, int64_t instances,
+                        hipStream_t s);
+void launch_tile_bold_t_synth(uint8_t* out, uint64_t seed, int64_t instances, hipStream_t s);
+int batch_key_pad(int count);
+size_t batch_keys_bytes(int bpad);
+// S[client][instances][2][N]; y_body/z_body: [count][3][N] (key_client_stride elements apart).
+int launch_packing_mfma(uint32_t* S, int64_t s_client_stride, const uint8_t* tiled, uint32_t* ykeys, uint32_t* zkeys,
+                        const uint64_t* y_body, const uint64_t* z_body, int64_t key_client_stride, int count,
+                        int64_t instances, hipStream_t s);
 
 // c: [nq][db_cols] canonical mod q; rows: [nq][db_rows] condensed; db: [db_cols][db_rows].
 int launch_gemv(uint64_t* c, const uint64_t* rows, int nq, const uint16_t* db, int db_rows, int64_t db_cols,

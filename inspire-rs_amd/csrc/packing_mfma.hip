@@ -1,0 +1,410 @@
+// Packing inner product for a batch of clients on int8 MFMA (exact).
+//
+// For every CRT prime p and NTT slot z the packing of packing_online_kernel.cu:18-92
+// is a small GEMM over the 6141 key-switch terms n:
+//     S_b[j][p][z] = sum_n Y_b[p][z][n] * T[p][z][j][n]   (mod q_p)
+// with Y_b the client's rotated packing keys (rotation_kernel.cu:21-59) and T the
+// device-resident key-switch material (bold_t, bold_t_bar, bold_t_hat).
+//
+// Both operands are split into four 7-bit limbs (residues < 2^28), so every limb
+// product is in [0, 127^2] and an int32 accumulator over 6144 terms stays below
+// 2^27.  Rows of a 16x16 tile are (client, limb l) = 4 clients x 4 limbs, columns
+// (output j, limb m) = 4 outputs x 4 limbs; the 16 partial sums of one
+// (client, output) pair are recombined as sum_{l,m} 2^(7(l+m)) C_lm mod q_p.
+//
+// Term order n (K): n = 6i + 2k + w for rotation i < 1023, key row k < 3 and
+// w = 0 (y with bold_t) or 1 (y-bar with bold_t_bar); n = 6138 + k for z_body with
+// bold_t_hat; n = 6141..6143 are zero padding.  K step s covers n = 64s .. 64s+63
+// and lane (row/col r, group g) holds n = 64s + 16g + t in byte t.
+//
+// T is stored fragment-ordered ("tiled"): [p][z][ntile][step][lane][16 B], one
+// 1 KiB fragment per (16-column tile, K step), so a wave fetches a fragment with
+// one fully contiguous 1 KiB load.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "modarith.hpp"
+
+namespace inspire {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+
+constexpr int kKPad = 6144;            // 6141 terms padded to 96 steps of 64
+constexpr int kKSteps = kKPad / 64;    // 96
+constexpr int kTermsRot = 6 * kNumRot; // 6138
+
+__host__ __device__ inline int64_t tiled_ntiles(int64_t instances) { return (instances + 3) / 4; }
+
+// Bytes of the tiled key-switch material for `instances` outputs.
+size_t tiled_bold_t_bytes(int64_t instances) {
+  return (size_t)2 * kN * tiled_ntiles(instances) * kKSteps * 1024;
+}
+
+// Source of term n of output j at slot z: condensed u64 from the reference layout.
+__device__ inline uint64_t term_value(const uint64_t* __restrict__ bt, const uint64_t* __restrict__ btb,
+                                      const uint64_t* __restrict__ bth, int64_t j, int n, int z) {
+  if (n < kTermsRot) {
+    const int i = n / 6, r = n % 6, k = r >> 1;
+    const uint64_t* src = (r & 1) ? btb : bt;
+    return src[((size_t)j * kNInner + 3 * i + k) * kN + z];
+  }
+  if (n < kTermsRot + kTExp) return bth[((size_t)j * kTExp + (n - kTermsRot)) * kN + z];
+  return 0;
+}
+
+// One thread per 16-byte fragment row: out[p][z][nt][s][lane][16].
+__global__ void k_tile_bold_t(uint8_t* __restrict__ out, const uint64_t* __restrict__ bt,
+                              const uint64_t* __restrict__ btb, const uint64_t* __restrict__ bth,
+                              int64_t instances) {
+  const int64_t ntiles = tiled_ntiles(instances);
+  const int64_t total = 2 * (int64_t)kN * ntiles * kKSteps * 64;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int lane = (int)(e % 64);
+    int64_t r = e / 64;
+    const int s = (int)(r % kKSteps);
+    r /= kKSteps;
+    const int64_t nt = r % ntiles;
+    r /= ntiles;
+    const int z = (int)(r % kN);
+    const int p = (int)(r / kN);
+    const int col = lane & 15, g = lane >> 4;
+    const int64_t j = 4 * nt + (col >> 2);
+    const int m = col & 3;
+    uint32_t w[4] = {0, 0, 0, 0};
+    if (j < instances) {
+      for (int t = 0; t < 16; ++t) {
+        const uint64_t v = term_value(bt, btb, bth, j, 64 * s + 16 * g + t, z);
+        const uint32_t res = (uint32_t)(p ? (v >> 32) : v);
+        w[t >> 2] |= ((res >> (7 * m)) & 127u) << (8 * (t & 3));
+      }
+    }
+    reinterpret_cast<u4*>(out)[e] = u4{w[0], w[1], w[2], w[3]};
+  }
+}
+
+// Same layout filled straight from the synthetic generator (streams of
+// bold_t = 2, bold_t_bar = 3, bold_t_hat = 4 in condensed kind).
+__device__ inline uint64_t synth_condensed(uint64_t seed, uint32_t stream, uint64_t idx) {
+  const uint64_t h = synth_word(seed, stream, idx);
+  const uint64_t lo = ((h & 0xffffffffull) * kQ0) >> 32;
+  const uint64_t hi = ((h >> 32) * kQ1) >> 32;
+  return lo | (hi << 32);
+}
+
+__global__ void k_tile_bold_t_synth(uint8_t* __restrict__ out, uint64_t seed, int64_t instances) {
+  const int64_t ntiles = tiled_ntiles(instances);
+  const int64_t total = 2 * (int64_t)kN * ntiles * kKSteps * 64;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int lane = (int)(e % 64);
+    int64_t r = e / 64;
+    const int s = (int)(r % kKSteps);
+    r /= kKSteps;
+    const int64_t nt = r % ntiles;
+    r /= ntiles;
+    const int z = (int)(r % kN);
+    const int p = (int)(r / kN);
+    const int col = lane & 15, g = lane >> 4;
+    const int64_t j = 4 * nt + (col >> 2);
+    const int m = col & 3;
+    uint32_t w[4] = {0, 0, 0, 0};
+    if (j < instances) {
+      for (int t = 0; t < 16; ++t) {
+        const int n = 64 * s + 16 * g + t;
+        uint64_t v = 0;
+        if (n < kTermsRot) {
+          const int i = n / 6, rr = n % 6, k = rr >> 1;
+          v = synth_condensed(seed, (rr & 1) ? 3 : 2, ((uint64_t)j * kNInner + 3 * i + k) * kN + z);
+        } else if (n < kTermsRot + kTExp) {
+          v = synth_condensed(seed, 4, ((uint64_t)j * kTExp + (n - kTermsRot)) * kN + z);
+        }
+        const uint32_t res = (uint32_t)(p ? (v >> 32) : v);
+        w[t >> 2] |= ((res >> (7 * m)) & 127u) << (8 * (t & 3));
+      }
+    }
+    reinterpret_cast<u4*>(out)[e] = u4{w[0], w[1], w[2], w[3]};
+  }
+}
+
+// Client keys for a batch, per prime and client-minor so the gathers of one
+// term for all clients are contiguous:  ykeys[p][k][slot][bpad], zkeys[p][k][z][bpad].
+// y_body, z_body: [count][3][N] condensed (k_unpack_query output of each client).
+__global__ void k_batch_keys(uint32_t* __restrict__ ykeys, uint32_t* __restrict__ zkeys,
+                             const uint64_t* __restrict__ y_body, const uint64_t* __restrict__ z_body,
+                             int64_t client_stride, int count, int bpad) {
+  const int64_t total = (int64_t)kTExp * kN * bpad;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e % bpad);
+    const int64_t ks = e / bpad;  // k*N + slot
+    uint64_t y = 0, zz = 0;
+    if (b < count) {
+      y = y_body[(int64_t)b * client_stride + ks];
+      zz = z_body[(int64_t)b * client_stride + ks];
+    }
+    ykeys[e] = (uint32_t)y;
+    ykeys[total + e] = (uint32_t)(y >> 32);
+    zkeys[e] = (uint32_t)zz;
+    zkeys[total + e] = (uint32_t)(zz >> 32);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The batched GEMM.  Block = WM x WN waves; the block owns MT M-tiles
+// (4*MT clients) and NT N-tiles (4*NT outputs) of one (p, z).  Wave (wm, wn)
+// owns M-tiles [wm*MT/WM, ...) x N-tiles [wn*NT/WN, ...).
+// ---------------------------------------------------------------------------
+
+struct PackMfmaArgs {
+  uint32_t* S;               // [bpad][instances][2][N] (clients b < count written)
+  const uint8_t* tiled;      // [2][N][ntiles][96][64][16]
+  const uint32_t* ykeys;     // [2][3][N][bpad]
+  const uint32_t* zkeys;     // [2][3][N][bpad]
+  int64_t instances;
+  int64_t ntiles;
+  int bpad;                  // clients padded to a multiple of 4*MT
+  int count;                 // real clients
+  int ngroups;               // N-tile groups per (p, z)
+  int mgroups;               // M-tile groups per (p, z)
+  int64_t s_client_stride;   // elements between clients in S
+};
+
+template <int MT, int NT, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN) void k_packing_mfma(PackMfmaArgs a) {
+  constexpr int kThreads = 64 * WM * WN;
+  constexpr int MTW = MT / WM, NTW = NT / WN;
+  // With one wave row (WM == 1) every B fragment feeds exactly one wave, which
+  // loads it straight from HBM; otherwise B fragments are staged in LDS once
+  // and read by the WM waves that share them.
+  constexpr bool kDirect = WM == 1;
+  constexpr int kBStage = kDirect ? 1 : NT;
+  constexpr int kBLoads = kDirect ? 0 : (NT * 64 + kThreads - 1) / kThreads;
+  static_assert(MT % WM == 0 && NT % WN == 0, "tiles must split evenly over waves");
+  __shared__ u4 bfrag[2][kBStage][64];
+  __shared__ u4 afrag[2][MT][64];
+  __shared__ int src_slot[2][64];   // per term of the step: key slot (or -1 for zero)
+  __shared__ int src_row[2][64];    // key row k, +3 for z_body
+
+  // XCD-aware block order: consecutive z stay on one XCD (block id mod 8).
+  const int64_t bid = blockIdx.x;
+  const int xcd = (int)(bid % 8);
+  const int64_t rest = bid / 8;
+  const int zl = (int)(rest % (kN / 8));
+  const int64_t r2 = rest / (kN / 8);
+  const int z = xcd * (kN / 8) + zl;
+  const int p = (int)(r2 % 2);
+  const int64_t r3 = r2 / 2;
+  const int ng = (int)(r3 % a.ngroups);
+  const int mg = (int)(r3 / a.ngroups);
+  const int nt0 = ng * NT, mt0 = mg * MT;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave / WN, wn = wave % WN;
+  const uint32_t ez = 2u * (__brev((uint32_t)z) >> 21) + 1u;
+  const size_t key_plane = (size_t)kTExp * kN * a.bpad;
+  const uint32_t* yk = a.ykeys + (size_t)p * key_plane;
+  const uint32_t* zk = a.zkeys + (size_t)p * key_plane;
+  const uint8_t* tbase = a.tiled + (((size_t)p * kN + z) * a.ntiles) * (size_t)kKSteps * 1024;
+
+  // Term sources of step s: NTT slot of the key value (or -1) and key row.
+  auto stage_terms = [&](int s, int buf) {
+    if (tid < 64) {
+      const int n = 64 * s + tid;
+      int slot = -1, row = 0;
+      if (n < kTermsRot) {
+        const int i = n / 6, rr = n % 6;
+        uint32_t g = 1, b5 = 5, e = (uint32_t)i;  // g = 5^i mod 4096 (packing.rs:1639-1652)
+        while (e) {
+          if (e & 1) g = (g * b5) & (2 * kN - 1);
+          b5 = (b5 * b5) & (2 * kN - 1);
+          e >>= 1;
+        }
+        const uint32_t e1 = (ez * g) & (2 * kN - 1);
+        const int s1 = (int)(__brev(e1 >> 1) >> 21);
+        slot = (rr & 1) ? (kN - 1 - s1) : s1;
+        row = rr >> 1;
+      } else if (n < kTermsRot + kTExp) {
+        slot = z;
+        row = 3 + (n - kTermsRot);
+      }
+      src_slot[buf][tid] = slot;
+      src_row[buf][tid] = row;
+    }
+  };
+  auto frag_ptr = [&](int nt, int s) {
+    return reinterpret_cast<const u4*>(tbase + ((size_t)nt * kKSteps + s) * 1024) + lane;
+  };
+  auto load_b_stage = [&](int s, u4 (&regs)[kBLoads > 0 ? kBLoads : 1]) {
+#pragma unroll
+    for (int r = 0; r < kBLoads; ++r) {
+      const int e = tid + r * kThreads;
+      u4 v = {0, 0, 0, 0};
+      if (e < NT * 64) {
+        const int nt = nt0 + e / 64;
+        if (nt < a.ntiles)
+          v = __builtin_nontemporal_load(reinterpret_cast<const u4*>(tbase + ((size_t)nt * kKSteps + s) * 1024) +
+                                         (e % 64));
+      }
+      regs[r] = v;
+    }
+  };
+  auto store_b_stage = [&](int buf, const u4 (&regs)[kBLoads > 0 ? kBLoads : 1]) {
+#pragma unroll
+    for (int r = 0; r < kBLoads; ++r) {
+      const int e = tid + r * kThreads;
+      if (e < NT * 64) bfrag[buf][e / 64][e % 64] = regs[r];
+    }
+  };
+  auto load_b_direct = [&](int s, u4 (&regs)[NTW]) {
+#pragma unroll
+    for (int y = 0; y < NTW; ++y) {
+      const int nt = nt0 + wn * NTW + y;
+      regs[y] = nt < a.ntiles ? __builtin_nontemporal_load(frag_ptr(nt, s)) : u4{0, 0, 0, 0};
+    }
+  };
+  // A fragment rows: lane = 4*bl + l + 16*g, dword tq holds terms n = 16g + 4tq .. +3.
+  auto stage_a = [&](int buf) {
+    for (int e = tid; e < MT * 64; e += kThreads) {
+      const int tq = e & 3, g = (e >> 2) & 3, bl = (e >> 4) & 3, mt = e >> 6;
+      const int b = 4 * (mt0 + mt) + bl;
+      uint32_t vals[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int tn = 16 * g + 4 * tq + u;
+        const int slot = src_slot[buf][tn], row = src_row[buf][tn];
+        uint32_t v = 0;
+        if (slot >= 0) {
+          const uint32_t* base = row < 3 ? yk : zk;
+          v = base[((size_t)(row % 3) * kN + slot) * a.bpad + b];
+        }
+        vals[u] = v;
+      }
+#pragma unroll
+      for (int l = 0; l < 4; ++l) {
+        uint32_t w = 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) w |= ((vals[u] >> (7 * l)) & 127u) << (8 * u);
+        reinterpret_cast<uint32_t*>(&afrag[buf][mt][4 * bl + l + 16 * g])[tq] = w;
+      }
+    }
+  };
+
+  v4i acc[MTW][NTW];
+#pragma unroll
+  for (int x = 0; x < MTW; ++x)
+#pragma unroll
+    for (int y = 0; y < NTW; ++y) acc[x][y] = v4i{0, 0, 0, 0};
+
+  u4 bregs[kBLoads > 0 ? kBLoads : 1];
+  u4 bcur[NTW], bnext[NTW];
+  stage_terms(0, 0);
+  if (kDirect) {
+    load_b_direct(0, bcur);
+  } else {
+    load_b_stage(0, bregs);
+    store_b_stage(0, bregs);
+  }
+  __syncthreads();
+  stage_a(0);
+  __syncthreads();
+
+  for (int s = 0; s < kKSteps; ++s) {
+    const int cur = s & 1, nxt = cur ^ 1;
+    const bool more = s + 1 < kKSteps;
+    if (more) {
+      stage_terms(s + 1, nxt);
+      if (kDirect)
+        load_b_direct(s + 1, bnext);
+      else
+        load_b_stage(s + 1, bregs);
+    }
+    v4i af[MTW];
+#pragma unroll
+    for (int x = 0; x < MTW; ++x) af[x] = __builtin_bit_cast(v4i, afrag[cur][wm * MTW + x][lane]);
+#pragma unroll
+    for (int y = 0; y < NTW; ++y) {
+      const v4i bf = kDirect ? __builtin_bit_cast(v4i, bcur[y]) : __builtin_bit_cast(v4i, bfrag[cur][wn * NTW + y][lane]);
+#pragma unroll
+      for (int x = 0; x < MTW; ++x) acc[x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[x], bf, acc[x][y], 0, 0, 0);
+    }
+    if (!kDirect && more) store_b_stage(nxt, bregs);
+    __syncthreads();  // terms of step s+1 visible; buffers of step s free
+    if (more) stage_a(nxt);
+    __syncthreads();
+    if (kDirect) {
+#pragma unroll
+      for (int y = 0; y < NTW; ++y) bcur[y] = bnext[y];
+    }
+  }
+
+  // Epilogue.  Lane (col c = lane%16, G = lane/16) holds C[4G + l][c], l = 0..3:
+  // client bl = G, output jl = c/4, output limb m = c%4.
+  const int c = lane & 15, G = lane >> 4;
+  const int m = c & 3, jl = c >> 2;
+  const uint32_t q = prime(p);
+  const uint32_t pw = (uint32_t)(((uint64_t)1 << (7 * m)) % q);  // 2^(7m) mod q
+#pragma unroll
+  for (int x = 0; x < MTW; ++x) {
+#pragma unroll
+    for (int y = 0; y < NTW; ++y) {
+      uint64_t X = 0;
+#pragma unroll
+      for (int l = 0; l < 4; ++l) X += (uint64_t)(uint32_t)acc[x][y][l] << (7 * l);
+      uint32_t v = (uint32_t)barrett64(X, cr1(p), q);
+      v = p ? mulmod<1>(v, pw) : mulmod<0>(v, pw);
+      uint32_t tot = v + __shfl_xor(v, 1, 64);
+      tot = (tot >= q) ? tot - q : tot;
+      uint32_t tot2 = tot + __shfl_xor(tot, 2, 64);
+      tot2 = (tot2 >= q) ? tot2 - q : tot2;
+      const int mt = mt0 + wm * MTW + x, nt = nt0 + wn * NTW + y;
+      const int b = 4 * mt + G;
+      const int64_t j = 4 * (int64_t)nt + jl;
+      if (m == 0 && b < a.count && j < a.instances)
+        a.S[(size_t)b * a.s_client_stride + ((size_t)j * 2 + p) * kN + z] = tot2;
+    }
+  }
+}
+
+void launch_tile_bold_t(uint8_t* out, const uint64_t* bt, const uint64_t* btb, const uint64_t* bth, int64_t instances,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_tile_bold_t, dim3(65536), dim3(256), 0, s, out, bt, btb, bth, instances);
+}
+
+void launch_tile_bold_t_synth(uint8_t* out, uint64_t seed, int64_t instances, hipStream_t s) {
+  hipLaunchKernelGGL(k_tile_bold_t_synth, dim3(65536), dim3(256), 0, s, out, seed, instances);
+}
+
+int batch_key_pad(int count) {
+  const int mt = count <= 4 ? 1 : (count <= 16 ? 4 : 16);
+  const int unit = 4 * mt;
+  return (count + unit - 1) / unit * unit;
+}
+
+size_t batch_keys_bytes(int bpad) { return (size_t)2 * kTExp * kN * bpad * sizeof(uint32_t); }
+
+template <int MT, int NT, int WM, int WN>
+static void launch_cfg(PackMfmaArgs a, hipStream_t s) {
+  a.ngroups = (int)((a.ntiles + NT - 1) / NT);
+  a.mgroups = a.bpad / (4 * MT);
+  const int64_t blocks = (int64_t)kN * 2 * a.ngroups * a.mgroups;
+  hipLaunchKernelGGL((k_packing_mfma<MT, NT, WM, WN>), dim3((unsigned)blocks), dim3(64 * WM * WN), 0, s, a);
+}
+
+int launch_packing_mfma(uint32_t* S, int64_t s_client_stride, const uint8_t* tiled, uint32_t* ykeys, uint32_t* zkeys,
+                        const uint64_t* y_body, const uint64_t* z_body, int64_t key_client_stride, int count,
+                        int64_t instances, hipStream_t s) {
+  if (count < 1) return -1;
+  const int bpad = batch_key_pad(count);
+  const int64_t total = (int64_t)kTExp * kN * bpad;
+  hipLaunchKernelGGL(k_batch_keys, dim3((unsigned)((total + 255) / 256 > 4096 ? 4096 : (total + 255) / 256)),
+                     dim3(256), 0, s, ykeys, zkeys, y_body, z_body, key_client_stride, count, bpad);
+  PackMfmaArgs a{S, tiled, ykeys, zkeys, instances, tiled_ntiles(instances), bpad, count, 0, 0, s_client_stride};
+  if (bpad <= 4)
+    launch_cfg<1, 44, 1, 4>(a, s);      // 1 M-tile: 4 waves x 11 N-tiles, B direct from HBM
+  else if (bpad <= 16)
+    launch_cfg<4, 44, 1, 4>(a, s);      // 4 M-tiles: 4 waves x (4 x 11) tiles, B direct
+  else
+    launch_cfg<16, 22, 4, 2>(a, s);     // 16 M-tiles: waves 4 x 2 of 4 x 11 tiles, B via LDS
+  return 0;
+}
+
+}  // namespace inspire

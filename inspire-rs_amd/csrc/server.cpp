@@ -216,6 +216,16 @@ struct Slot {
   uint32_t *bodies = nullptr, *S = nullptr;
   uint8_t* out = nullptr;
   void* gemv_scratch = nullptr;  // MFMA query fragments
+  uint32_t* keys = nullptr;      // MFMA packing: client keys per prime (y then z)
+  hipEvent_t done = nullptr;
+};
+
+// Buffers for batched responds (inspire_respond_batch_device).
+struct Batch {
+  int max = 0;
+  uint64_t *y_body = nullptr, *z_body = nullptr, *rows = nullptr, *P = nullptr, *cvec = nullptr;
+  uint32_t *bodies = nullptr, *S = nullptr, *keys = nullptr;
+  void* gemv_scratch = nullptr;
   hipEvent_t done = nullptr;
 };
 
@@ -237,6 +247,10 @@ struct inspire_server {
   std::vector<Slot> slots;
   size_t next_slot = 0;
   int gemv_kind = INSPIRE_GEMV_MFMA;
+  int packing_kind = INSPIRE_PACKING_VALU;
+  uint8_t* tiled = nullptr;      // fragment-ordered int8 limbs of bold_t (MFMA packing)
+  bool tiled_valid = false;
+  Batch batch;
   std::vector<uint8_t> host_out;
   // optional per-stage timing
   bool profile = false;
@@ -265,11 +279,57 @@ int dev_alloc(T** p, size_t count, const char* what) {
 
 constexpr int kDefaultSlots = 2;
 
+void free_batch(inspire_server_t* s) {
+  Batch& b = s->batch;
+  if (b.done) (void)hipEventSynchronize(b.done);
+  void* ptrs[] = {b.y_body, b.z_body, b.rows, b.P, b.cvec, b.bodies, b.S, b.keys, b.gemv_scratch};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (b.done) (void)hipEventDestroy(b.done);
+  b = Batch{};
+}
+
+int alloc_batch(inspire_server_t* s, int max) {
+  Batch& b = s->batch;
+  const int64_t inst = s->shape.instances, rows = s->shape.db_rows, mq = s->shape.max_queries;
+  int rc = INSPIRE_OK;
+  b.max = max;
+  if (!rc) rc = dev_alloc(&b.y_body, (size_t)max * kTExp * kN, "batch y_body");
+  if (!rc) rc = dev_alloc(&b.z_body, (size_t)max * kTExp * kN, "batch z_body");
+  if (!rc) rc = dev_alloc(&b.rows, (size_t)max * mq * rows, "batch rows");
+  if (!rc) rc = dev_alloc(&b.bodies, (size_t)max * mq * 6 * 2 * kN, "batch bodies");
+  if (!rc) rc = dev_alloc(&b.S, (size_t)max * inst * 2 * kN, "batch packing output");
+  if (!rc) rc = dev_alloc(&b.P, (size_t)max * inst * kN, "batch post output");
+  if (!rc) rc = dev_alloc(&b.cvec, (size_t)max * mq * s->db_cols, "batch gemv output");
+  if (!rc) rc = dev_alloc((uint8_t**)&b.keys, 2 * batch_keys_bytes(batch_key_pad(max)), "batch keys");
+  if (!rc)
+    rc = dev_alloc((uint8_t**)&b.gemv_scratch, gemv_mfma_batch_scratch_bytes((int)rows, max), "batch gemv scratch");
+  if (!rc && hipEventCreateWithFlags(&b.done, hipEventDisableTiming) != hipSuccess)
+    rc = set_error(INSPIRE_ERR_HIP, "hipEventCreate failed");
+  return rc;
+}
+
+// Builds the fragment-ordered copy of bold_t used by the MFMA packing.
+int ensure_tiled(inspire_server_t* s) {
+  if (!s->tiled) {
+    int rc = dev_alloc(&s->tiled, tiled_bold_t_bytes(s->shape.instances), "tiled bold_t");
+    if (rc) return rc;
+    s->tiled_valid = false;
+  }
+  if (!s->tiled_valid && s->have_precomp) {
+    launch_tile_bold_t(s->tiled, s->bt, s->btb, s->bth, s->shape.instances, s->stream);
+    HIP_TRY(hipGetLastError(), "tile bold_t launch");
+    HIP_TRY(hipStreamSynchronize(s->stream), "tile bold_t");
+    s->tiled_valid = true;
+  }
+  return INSPIRE_OK;
+}
+
 void free_slots(inspire_server_t* s) {
   for (Slot& sl : s->slots) {
     if (sl.done) (void)hipEventSynchronize(sl.done);
-    void* ptrs[] = {sl.qbytes, sl.y_body, sl.z_body, sl.rows, sl.bodies, sl.S, sl.P, sl.cvec, sl.out,
-                    sl.gemv_scratch};
+    void* ptrs[] = {sl.qbytes, sl.y_body, sl.z_body, sl.rows,         sl.bodies,
+                    sl.S,      sl.P,      sl.cvec,   sl.out, sl.gemv_scratch, sl.keys};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     if (sl.done) (void)hipEventDestroy(sl.done);
@@ -293,6 +353,7 @@ int alloc_slots(inspire_server_t* s, int count) {
     if (!rc) rc = dev_alloc(&sl.cvec, (size_t)mq * s->db_cols, "gemv output");
     if (!rc) rc = dev_alloc(&sl.out, (size_t)mq * s->c * kRespBytes, "responses");
     if (!rc) rc = dev_alloc((uint8_t**)&sl.gemv_scratch, gemv_mfma_scratch_bytes((int)rows), "gemv scratch");
+    if (!rc) rc = dev_alloc((uint8_t**)&sl.keys, 2 * batch_keys_bytes(batch_key_pad(1)), "packing keys");
     if (!rc && hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess)
       rc = set_error(INSPIRE_ERR_HIP, "hipEventCreate failed");
     if (rc) return rc;
@@ -347,7 +408,8 @@ extern "C" void inspire_server_destroy(inspire_server_t* s) {
   if (!s) return;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   free_slots(s);
-  void* ptrs[] = {s->db, s->bt, s->btb, s->bth, s->a_hat};
+  free_batch(s);
+  void* ptrs[] = {s->db, s->bt, s->btb, s->bth, s->a_hat, s->tiled};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -364,6 +426,66 @@ extern "C" int inspire_server_set_slots(inspire_server_t* s, int count) {
 }
 
 extern "C" void* inspire_server_stream(inspire_server_t* s) { return s ? (void*)s->stream : nullptr; }
+
+extern "C" int inspire_server_set_packing_kernel(inspire_server_t* s, int kind) {
+  if (!s || (kind != INSPIRE_PACKING_VALU && kind != INSPIRE_PACKING_MFMA))
+    return set_error(INSPIRE_ERR_INVALID, "unknown packing kernel");
+  if (kind == INSPIRE_PACKING_MFMA) {
+    int rc = ensure_tiled(s);
+    if (rc) return rc;
+  }
+  s->packing_kind = kind;
+  return INSPIRE_OK;
+}
+
+extern "C" int inspire_server_set_max_batch(inspire_server_t* s, int max_batch) {
+  if (!s || max_batch < 1 || max_batch > 1024) return set_error(INSPIRE_ERR_INVALID, "max_batch must be in [1, 1024]");
+  HIP_TRY(hipDeviceSynchronize(), "drain before resizing the batch");
+  free_batch(s);
+  int rc = alloc_batch(s, max_batch);
+  if (rc) {
+    std::string msg = g_last_error;
+    free_batch(s);
+    return set_error(rc, msg);
+  }
+  return ensure_tiled(s);
+}
+
+extern "C" int inspire_respond_batch_device(inspire_server_t* s, const uint8_t* d_queries, size_t q_stride,
+                                            size_t len, int nq, int count, uint8_t* d_out, size_t out_stride,
+                                            void* stream) {
+  if (!s || !d_queries || !d_out) return set_error(INSPIRE_ERR_INVALID, "null argument");
+  if (!s->have_db || !s->have_precomp) return set_error(INSPIRE_ERR_STATE, "database or precomputation not loaded");
+  if (count < 1 || count > s->batch.max) return set_error(INSPIRE_ERR_INVALID, "batch larger than max_batch");
+  if (nq < 1 || nq > s->shape.max_queries || nq > 2) return set_error(INSPIRE_ERR_INVALID, "hash queries per query");
+  if (len < query_bytes(s->shape.db_rows, nq) || (count > 1 && q_stride < len))
+    return set_error(INSPIRE_ERR_INVALID, "keyword query truncated");
+  const size_t blob = (size_t)nq * s->c * kRespBytes;
+  if (count > 1 && out_stride < blob) return set_error(INSPIRE_ERR_INVALID, "output stride too small");
+  if (!s->tiled_valid) {
+    int rc = ensure_tiled(s);
+    if (rc) return rc;
+  }
+  hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+  Batch& b = s->batch;
+  const int64_t rows = s->shape.db_rows, inst = s->shape.instances;
+  HIP_TRY(hipStreamWaitEvent(st, b.done, 0), "batch wait");
+  launch_unpack_query_batch(d_queries, (int64_t)q_stride, nq, (int)rows, count, b.y_body, b.z_body, b.rows, b.bodies,
+                            st);
+  const int bpad = batch_key_pad(count);
+  launch_packing_mfma(b.S, inst * 2 * kN, s->tiled, b.keys, b.keys + batch_keys_bytes(bpad) / 4, b.y_body, b.z_body,
+                      (int64_t)kTExp * kN, count, inst, st);
+  launch_post(b.P, b.S, s->consts->tw, (int)(count * inst), st);
+  if (launch_gemv_mfma_batch(b.cvec, (int64_t)nq * s->db_cols, b.rows, (int64_t)nq * rows, nq, s->db, (int)rows,
+                             s->db_cols, count, b.gemv_scratch, st))
+    return set_error(INSPIRE_ERR_INVALID, "unsupported GEMV shape");
+  if (launch_fold_batch(d_out, (int64_t)out_stride, s->a_hat, b.cvec, b.P, b.bodies, s->consts->neg_a, s->consts->tw,
+                        inst, s->shape.interpolate_degree, nq, count, st))
+    return set_error(INSPIRE_ERR_INVALID, "unsupported fold shape");
+  HIP_TRY(hipGetLastError(), "batch launch");
+  HIP_TRY(hipEventRecord(b.done, st), "batch done");
+  return INSPIRE_OK;
+}
 
 extern "C" int inspire_server_set_gemv_kernel(inspire_server_t* s, int kind) {
   if (!s || (kind != INSPIRE_GEMV_VALU && kind != INSPIRE_GEMV_MFMA))
@@ -413,7 +535,8 @@ extern "C" int inspire_server_set_precomp(inspire_server_t* s, const uint64_t* b
   HIP_TRY(hipMemcpyAsync(s->a_hat, a_hat, inst * kN * 8, k, s->stream), "copy a_hat");
   HIP_TRY(hipStreamSynchronize(s->stream), "copy precomp");
   s->have_precomp = true;
-  return INSPIRE_OK;
+  s->tiled_valid = false;
+  return s->tiled ? ensure_tiled(s) : INSPIRE_OK;
 }
 
 extern "C" int inspire_server_set_stash(inspire_server_t* s, const uint8_t* entries, uint32_t entry_size,
@@ -439,7 +562,8 @@ extern "C" int inspire_server_fill_synthetic(inspire_server_t* s, uint64_t seed)
   HIP_TRY(hipGetLastError(), "synthetic fill launch");
   HIP_TRY(hipStreamSynchronize(s->stream), "synthetic fill");
   s->have_db = s->have_precomp = true;
-  return INSPIRE_OK;
+  s->tiled_valid = false;
+  return s->tiled ? ensure_tiled(s) : INSPIRE_OK;
 }
 
 extern "C" size_t inspire_server_response_bytes(const inspire_server_t* s, int nq) {
@@ -466,7 +590,13 @@ int run_pipeline(inspire_server_t* s, Slot& sl, const uint8_t* d_query, int nq, 
   mark(0);
   launch_unpack_query(d_query, nq, (int)rows, sl.y_body, sl.z_body, sl.rows, sl.bodies, st);
   mark(1);
-  launch_packing(sl.S, s->bt, s->btb, s->bth, sl.y_body, sl.z_body, (int)inst, st);
+  if (s->packing_kind == INSPIRE_PACKING_MFMA) {
+    if (!s->tiled_valid) return set_error(INSPIRE_ERR_STATE, "tiled key-switch material not built");
+    launch_packing_mfma(sl.S, inst * 2 * kN, s->tiled, sl.keys, sl.keys + batch_keys_bytes(batch_key_pad(1)) / 4,
+                        sl.y_body, sl.z_body, (int64_t)kTExp * kN, 1, inst, st);
+  } else {
+    launch_packing(sl.S, s->bt, s->btb, s->bth, sl.y_body, sl.z_body, (int)inst, st);
+  }
   mark(2);
   launch_post(sl.P, sl.S, s->consts->tw, (int)inst, st);
   mark(3);

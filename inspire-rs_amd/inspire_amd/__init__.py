@@ -81,6 +81,9 @@ def lib():
         L.inspire_gemv_mfma_scratch_bytes.restype = sz
         L.inspire_server_set_gemv_kernel.argtypes = [vp, i32]
         L.inspire_server_set_slots.argtypes = [vp, i32]
+        L.inspire_server_set_packing_kernel.argtypes = [vp, i32]
+        L.inspire_server_set_max_batch.argtypes = [vp, i32]
+        L.inspire_respond_batch_device.argtypes = [vp, vp, sz, sz, i32, i32, vp, sz, vp]
         L.inspire_packing_device.argtypes = [vp, vp, vp, vp, vp, vp, i64, vp]
         L.inspire_post_device.argtypes = [vp, vp, i64, vp]
         L.inspire_fold_device.argtypes = [vp, vp, vp, vp, vp, i64, i32, i32, vp]
@@ -172,6 +175,16 @@ class Server:
     def set_slots(self, count):
         check(lib().inspire_server_set_slots(self.h, count))
 
+    def set_packing_kernel(self, kind):
+        check(lib().inspire_server_set_packing_kernel(self.h, kind))
+
+    def set_max_batch(self, max_batch):
+        check(lib().inspire_server_set_max_batch(self.h, max_batch))
+
+    def respond_batch_device(self, d_queries, q_stride, length, nq, count, d_out, out_stride, stream=None):
+        check(lib().inspire_respond_batch_device(self.h, _ptr(d_queries), q_stride, length, nq, count, _ptr(d_out),
+                                                 out_stride, stream))
+
     def set_gemv_kernel(self, kind):
         check(lib().inspire_server_set_gemv_kernel(self.h, kind))
 
@@ -203,6 +216,7 @@ def gemv_device(d_c, d_rows, nq, d_db, db_rows, db_cols, stream=None):
 
 
 GEMV_VALU, GEMV_MFMA = 0, 1
+PACKING_VALU, PACKING_MFMA = 0, 1
 
 
 def gemv_mfma_scratch_bytes(db_rows):

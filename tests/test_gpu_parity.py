@@ -297,3 +297,75 @@ def test_legacy_rotation_and_packing(torch):
     L.gpu_packing_online_free()
     L.gpu_rotation_free()
     assert (out.reshape(inst, 2, N) == O.packing(bt, btb, bth, y, z, inst)).all()
+
+
+# ---------------------------------------------------------------------------
+# MFMA packing and batched respond
+# ---------------------------------------------------------------------------
+
+def _queries(db_rows, seeds, nq=2, small=False):
+    out = []
+    for s in seeds:
+        y, z, rows, bodies = O.synth_query(s, db_rows, nq=nq, small=small)
+        out.append((O.serialize_query(y, z, rows, bodies), (y, z, rows, bodies)))
+    return out
+
+
+@pytest.mark.parametrize("db_rows,inst,d", [(2048, 8, 8), (4096, 12, 4), (2048, 2, 1)])
+def test_packing_mfma_respond_matches_valu(torch, db_rows, inst, d):
+    say("packing mfma vs valu respond", db_rows, inst, d)
+    srv = IA.Server(db_rows, inst, d)
+    srv.fill_synthetic(21)
+    for q, _ in _queries(db_rows, [22, 23]):
+        srv.set_packing_kernel(IA.PACKING_VALU)
+        a = srv.respond_serialized(q)
+        srv.set_packing_kernel(IA.PACKING_MFMA)
+        b = srv.respond_serialized(q)
+        assert a == b
+    srv.close()
+
+
+def test_packing_mfma_matches_live_oracle(torch):
+    say("packing mfma vs oracle")
+    db_rows, inst, d = 2048, 4, 2
+    st = O.synth_state(31, db_rows, inst)
+    srv = IA.Server(db_rows, inst, d)
+    srv.set_db(st[0])
+    srv.set_precomp(st[1], st[2], st[3], st[4])
+    srv.set_packing_kernel(IA.PACKING_MFMA)
+    q, (y, z, rows, bodies) = _queries(db_rows, [32], small=True)[0]
+    exp = O.respond_raw(st[0], db_rows, inst, d, st[1], st[2], st[3], st[4], y, z, rows, bodies)
+    assert srv.respond_serialized(q) == O.serialize_response(exp, [])
+    srv.close()
+
+
+@pytest.mark.parametrize("count", [1, 3, 5, 17, 20])
+def test_batch_results_equal_solo_results(torch, count):
+    # every query's response is the same alone and in the mix
+    say("batch vs solo", count)
+    db_rows, inst, d = 2048, 8, 4
+    srv = IA.Server(db_rows, inst, d)
+    srv.fill_synthetic(41)
+    srv.set_max_batch(32)
+    qs = _queries(db_rows, [100 + i for i in range(count)], small=False)
+    qlen = len(qs[0][0])
+    stride = (qlen + 16 + 255) // 256 * 256
+    host = np.zeros(stride * count, dtype=np.uint8)
+    for i, (q, _) in enumerate(qs):
+        host[i * stride:i * stride + qlen] = np.frombuffer(q, dtype=np.uint8)
+    dq = dev_from(torch, host)
+    blob = 2 * (inst // d) * O.RESP_BYTES
+    dout = empty(torch, count * blob, np.uint8)
+    srv.respond_batch_device(dq, stride, qlen, 2, count, dout, blob)
+    torch.cuda.synchronize()
+    out = dout.cpu().numpy().tobytes()
+    for i, (q, _) in enumerate(qs):
+        solo = srv.respond_serialized(q)
+        parts = solo[8:8 + blob // 2] + solo[8 + blob // 2 + 4:8 + blob // 2 + 4 + blob // 2]
+        assert out[i * blob:(i + 1) * blob] == parts, i
+    # one query also against the oracle
+    st = O.synth_state(41, db_rows, inst)
+    y, z, rows, bodies = qs[0][1]
+    exp = O.respond_raw(st[0], db_rows, inst, d, st[1], st[2], st[3], st[4], y, z, rows, bodies)
+    assert out[:blob] == b"".join(exp)
+    srv.close()
