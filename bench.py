@@ -266,8 +266,15 @@ def main():
             per = sorted(ev[r].elapsed_time(ev[r + 1]) for r in range(reps))
             tot = ev[0].elapsed_time(ev[reps])
             qps_b = B * reps / (tot / 1e3)
+            srv.profile(True)
+            run_batch()
+            torch.cuda.synchronize()
+            bst, _ = srv.profile_read()
+            srv.profile(False)
             batched[str(B)] = {"qps": qps_b, "ms_per_batch": tot / reps, "p50_batch_ms": per[len(per) // 2],
-                               "p99_batch_ms": per[-1], "speedup_vs_single_query": qps_b / qps}
+                               "p99_batch_ms": per[-1], "speedup_vs_single_query": qps_b / qps,
+                               "stage_ms": {n: bst[i] for i, n in enumerate(names)}}
+            log("  stages:", {n: round(bst[i], 3) for i, n in enumerate(names)})
             log("batch %d: %.2f ms/batch, %.1f q/s (%.1fx single)" % (B, tot / reps, qps_b, qps_b / qps))
             del bq, bout
     result["batched"] = batched

@@ -262,27 +262,38 @@ __global__ __launch_bounds__(64 * WM * WN) void k_packing_mfma(PackMfmaArgs a) {
     }
   };
   // A fragment rows: lane = 4*bl + l + 16*g, dword tq holds terms n = 16g + 4tq .. +3.
-  auto stage_a = [&](int buf) {
-    for (int e = tid; e < MT * 64; e += kThreads) {
+  // Item e (< MT*64) of a step: tq = e%4, g = (e/4)%4, bl = (e/16)%4, mt = e/64.
+  constexpr int kAItems = (MT * 64 + kThreads - 1) / kThreads;
+  auto gather_a = [&](int buf, uint32_t (&vals)[kAItems][4]) {
+#pragma unroll
+    for (int r = 0; r < kAItems; ++r) {
+      const int e = tid + r * kThreads;
       const int tq = e & 3, g = (e >> 2) & 3, bl = (e >> 4) & 3, mt = e >> 6;
       const int b = 4 * (mt0 + mt) + bl;
-      uint32_t vals[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int tn = 16 * g + 4 * tq + u;
         const int slot = src_slot[buf][tn], row = src_row[buf][tn];
         uint32_t v = 0;
-        if (slot >= 0) {
+        if (e < MT * 64 && slot >= 0) {
           const uint32_t* base = row < 3 ? yk : zk;
           v = base[((size_t)(row % 3) * kN + slot) * a.bpad + b];
         }
-        vals[u] = v;
+        vals[r][u] = v;
       }
+    }
+  };
+  auto store_a = [&](int buf, const uint32_t (&vals)[kAItems][4]) {
+#pragma unroll
+    for (int r = 0; r < kAItems; ++r) {
+      const int e = tid + r * kThreads;
+      if (e >= MT * 64) continue;
+      const int tq = e & 3, g = (e >> 2) & 3, bl = (e >> 4) & 3, mt = e >> 6;
 #pragma unroll
       for (int l = 0; l < 4; ++l) {
         uint32_t w = 0;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) w |= ((vals[u] >> (7 * l)) & 127u) << (8 * u);
+        for (int u = 0; u < 4; ++u) w |= ((vals[r][u] >> (7 * l)) & 127u) << (8 * u);
         reinterpret_cast<uint32_t*>(&afrag[buf][mt][4 * bl + l + 16 * g])[tq] = w;
       }
     }
@@ -294,9 +305,14 @@ __global__ __launch_bounds__(64 * WM * WN) void k_packing_mfma(PackMfmaArgs a) {
 #pragma unroll
     for (int y = 0; y < NTW; ++y) acc[x][y] = v4i{0, 0, 0, 0};
 
+  // Pipeline: while step s runs on the matrix cores, the key gathers and the
+  // B fragments of step s+1 are in flight and the term sources of step s+2
+  // are computed; one barrier per step.
   u4 bregs[kBLoads > 0 ? kBLoads : 1];
   u4 bcur[NTW], bnext[NTW];
+  uint32_t avals[kAItems][4];
   stage_terms(0, 0);
+  stage_terms(1, 1);
   if (kDirect) {
     load_b_direct(0, bcur);
   } else {
@@ -304,19 +320,21 @@ __global__ __launch_bounds__(64 * WM * WN) void k_packing_mfma(PackMfmaArgs a) {
     store_b_stage(0, bregs);
   }
   __syncthreads();
-  stage_a(0);
+  gather_a(0, avals);
+  store_a(0, avals);
   __syncthreads();
 
   for (int s = 0; s < kKSteps; ++s) {
     const int cur = s & 1, nxt = cur ^ 1;
     const bool more = s + 1 < kKSteps;
     if (more) {
-      stage_terms(s + 1, nxt);
+      gather_a(nxt, avals);
       if (kDirect)
         load_b_direct(s + 1, bnext);
       else
         load_b_stage(s + 1, bregs);
     }
+    if (s + 2 < kKSteps) stage_terms(s + 2, cur);
     v4i af[MTW];
 #pragma unroll
     for (int x = 0; x < MTW; ++x) af[x] = __builtin_bit_cast(v4i, afrag[cur][wm * MTW + x][lane]);
@@ -326,9 +344,10 @@ __global__ __launch_bounds__(64 * WM * WN) void k_packing_mfma(PackMfmaArgs a) {
 #pragma unroll
       for (int x = 0; x < MTW; ++x) acc[x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[x], bf, acc[x][y], 0, 0, 0);
     }
-    if (!kDirect && more) store_b_stage(nxt, bregs);
-    __syncthreads();  // terms of step s+1 visible; buffers of step s free
-    if (more) stage_a(nxt);
+    if (more) {
+      store_a(nxt, avals);
+      if (!kDirect) store_b_stage(nxt, bregs);
+    }
     __syncthreads();
     if (kDirect) {
 #pragma unroll
@@ -374,7 +393,7 @@ void launch_tile_bold_t_synth(uint8_t* out, uint64_t seed, int64_t instances, hi
 }
 
 int batch_key_pad(int count) {
-  const int mt = count <= 4 ? 1 : (count <= 16 ? 4 : 16);
+  const int mt = count <= 4 ? 1 : (count <= 16 ? 4 : 8);
   const int unit = 4 * mt;
   return (count + unit - 1) / unit * unit;
 }
@@ -403,7 +422,7 @@ int launch_packing_mfma(uint32_t* S, int64_t s_client_stride, const uint8_t* til
   else if (bpad <= 16)
     launch_cfg<4, 44, 1, 4>(a, s);      // 4 M-tiles: 4 waves x (4 x 11) tiles, B direct
   else
-    launch_cfg<16, 22, 4, 2>(a, s);     // 16 M-tiles: waves 4 x 2 of 4 x 11 tiles, B via LDS
+    launch_cfg<8, 22, 2, 2>(a, s);      // 8 M-tiles: waves 2 x 2 of 4 x 11 tiles, B via LDS
   return 0;
 }
 

@@ -469,21 +469,39 @@ extern "C" int inspire_respond_batch_device(inspire_server_t* s, const uint8_t* 
   hipStream_t st = stream ? (hipStream_t)stream : s->stream;
   Batch& b = s->batch;
   const int64_t rows = s->shape.db_rows, inst = s->shape.instances;
+  auto mark = [&](int i) {
+    if (s->profile) (void)hipEventRecord(s->ev[i], st);
+  };
   HIP_TRY(hipStreamWaitEvent(st, b.done, 0), "batch wait");
+  mark(0);
   launch_unpack_query_batch(d_queries, (int64_t)q_stride, nq, (int)rows, count, b.y_body, b.z_body, b.rows, b.bodies,
                             st);
+  mark(1);
   const int bpad = batch_key_pad(count);
   launch_packing_mfma(b.S, inst * 2 * kN, s->tiled, b.keys, b.keys + batch_keys_bytes(bpad) / 4, b.y_body, b.z_body,
                       (int64_t)kTExp * kN, count, inst, st);
+  mark(2);
   launch_post(b.P, b.S, s->consts->tw, (int)(count * inst), st);
+  mark(3);
   if (launch_gemv_mfma_batch(b.cvec, (int64_t)nq * s->db_cols, b.rows, (int64_t)nq * rows, nq, s->db, (int)rows,
                              s->db_cols, count, b.gemv_scratch, st))
     return set_error(INSPIRE_ERR_INVALID, "unsupported GEMV shape");
+  mark(4);
   if (launch_fold_batch(d_out, (int64_t)out_stride, s->a_hat, b.cvec, b.P, b.bodies, s->consts->neg_a, s->consts->tw,
                         inst, s->shape.interpolate_degree, nq, count, st))
     return set_error(INSPIRE_ERR_INVALID, "unsupported fold shape");
+  mark(5);
   HIP_TRY(hipGetLastError(), "batch launch");
   HIP_TRY(hipEventRecord(b.done, st), "batch done");
+  if (s->profile) {
+    HIP_TRY(hipEventSynchronize(s->ev[5]), "profile sync");
+    for (int i = 0; i < 5; ++i) {
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, s->ev[i], s->ev[i + 1]), "profile elapsed");
+      s->stage_ms[i] += ms;
+    }
+    ++s->profiled;
+  }
   return INSPIRE_OK;
 }
 
