@@ -445,7 +445,7 @@ __global__ __launch_bounds__(1024) void k_post(uint64_t* __restrict__ P, const u
   __syncthreads();
   uint32_t* const polys[2] = {buf[0], buf[1]};
   const int pr[2] = {0, 1};
-  ntt_inverse_multi<1024, 2>(polys, pr, tw);
+  ntt_inverse_r4<1024, 2>(polys, pr, tw);
   for (int z = threadIdx.x; z < kN; z += 1024) P[(size_t)j * kN + z] = crt_compose(buf[0][z], buf[1][z]);
 }
 
@@ -464,6 +464,7 @@ void launch_post(uint64_t* P, const uint32_t* S, const uint32_t* tw, int count, 
 // ---------------------------------------------------------------------------
 
 constexpr int kFoldThreads = 1024;
+constexpr int kFoldSlots = kN / kFoldThreads;  // NTT slots owned per thread
 
 struct FoldArgs {
   uint8_t* out;              // [clients] x out_stride bytes, each [nq][c][12288]
@@ -481,9 +482,11 @@ struct FoldArgs {
 };
 
 __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
+  // LDS: the 6 gadget-digit polynomials for both primes (96 KiB) and the two
+  // product rows of both primes (32 KiB).
   extern __shared__ uint32_t lds[];
-  uint32_t* dig = lds;                       // [2][6][N]
-  uint32_t* prod = lds + 2 * 6 * kN;         // [2 (prime)][2 (row)][N]
+  uint32_t* dig = lds;                 // [2 (prime)][6][N]
+  uint32_t* prod = lds + 12 * kN;      // [2 (prime)][2 (row)][N]
   const NttTables tw{args.tw};
   const int per_client = args.nq * args.c;
   const int b = blockIdx.x / per_client;
@@ -495,82 +498,76 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
   const uint64_t* Pc = args.P + (size_t)b * args.instances * kN;
   const uint32_t* body = args.bodies + ((size_t)b * args.nq + h) * 6 * 2 * kN;
 
-  // sum[r][zz], zz in {tid, tid+1024}
-  uint64_t sum[2][2];
+  // sum[r][s] for slots z = tid + kFoldThreads*s
+  uint64_t sum[2][kFoldSlots];
   {
     const size_t j = (size_t)w * d + (d - 1);
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      const int z = tid + s * 1024;
+    for (int s = 0; s < kFoldSlots; ++s) {
+      const int z = tid + s * kFoldThreads;
       sum[0][s] = args.a_hat[j * kN + z];
       sum[1][s] = reduce_small_q(cv[j * kN + z] + Pc[j * kN + z]);
     }
   }
 
-  uint32_t* const fpolys[12] = {dig + 0 * kN, dig + 1 * kN, dig + 2 * kN, dig + 3 * kN, dig + 4 * kN, dig + 5 * kN,
-                                dig + 6 * kN, dig + 7 * kN, dig + 8 * kN, dig + 9 * kN, dig + 10 * kN,
-                                dig + 11 * kN};
-  const int fprime[12] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1};
-  uint32_t* const ipolys[4] = {prod + 0 * kN, prod + 1 * kN, prod + 2 * kN, prod + 3 * kN};
+  uint32_t* const dpolys[12] = {dig,          dig + kN,     dig + 2 * kN, dig + 3 * kN,
+                                dig + 4 * kN, dig + 5 * kN, dig + 6 * kN, dig + 7 * kN,
+                                dig + 8 * kN, dig + 9 * kN, dig + 10 * kN, dig + 11 * kN};
+  const int dprime[12] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1};
+  uint32_t* const ipolys[4] = {prod, prod + kN, prod + 2 * kN, prod + 3 * kN};
   const int iprime[4] = {0, 0, 1, 1};
   const uint64_t mask = (1ull << kGadgetBits) - 1;
 
   for (int i = d - 2; i >= 0; --i) {
-    // G^-1(sum): digit k of row r goes to gadget row r + 2k (gadget.rs:35-61).
+    // G^-1(sum): digit k of row r is gadget row r + 2k (gadget.rs:35-61); the
+    // digits (< 2^19) are canonical for both primes.
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      const int z = tid + s * 1024;
+    for (int s = 0; s < kFoldSlots; ++s) {
+      const int z = tid + s * kFoldThreads;
 #pragma unroll
       for (int r = 0; r < 2; ++r)
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
           const uint32_t dv = (uint32_t)((sum[r][s] >> (kGadgetBits * k)) & mask);
-          dig[(0 * 6 + r + 2 * k) * kN + z] = dv;
-          dig[(1 * 6 + r + 2 * k) * kN + z] = dv;
+          dig[(r + 2 * k) * kN + z] = dv;
+          dig[(6 + r + 2 * k) * kN + z] = dv;
         }
     }
     __syncthreads();
-    ntt_forward_multi<kFoldThreads, 12>(fpolys, fprime, tw);
+    ntt_forward_r4m<kFoldThreads, 12>(dpolys, dprime, tw);
     // ct_gsw * G: row 0 = NTT(-a) * sum_k G_k; row 1 = sum_k body_k * G_k.
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      const int z = tid + s * 1024;
+    for (int s = 0; s < kFoldSlots; ++s) {
+      const int z = tid + s * kFoldThreads;
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
+        const uint32_t q = prime(c);
         uint32_t gsum = 0;
         uint64_t acc = 0;
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
-          const uint32_t gk = dig[(c * 6 + k) * kN + z];
+          const uint32_t gk = dig[(6 * c + k) * kN + z];
           gsum += gk;
           acc += (uint64_t)body[(k * 2 + c) * kN + z] * gk;
         }
-        const uint32_t q = prime(c);
-        uint32_t gs = gsum % q;
+        const uint32_t gs = gsum % q;
         const uint32_t na = args.neg_a[c * kN + z];
-        uint32_t p0, p1;
-        if (c == 0) {
-          p0 = mulmod<0>(na, gs);
-        } else {
-          p0 = mulmod<1>(na, gs);
-        }
-        p1 = (uint32_t)barrett64(acc, cr1(c), q);
-        prod[(c * 2 + 0) * kN + z] = p0;
-        prod[(c * 2 + 1) * kN + z] = p1;
+        prod[(c * 2 + 0) * kN + z] = c == 0 ? mulmod<0>(na, gs) : mulmod<1>(na, gs);
+        prod[(c * 2 + 1) * kN + z] = (uint32_t)barrett64(acc, cr1(c), q);
       }
     }
     __syncthreads();
-    ntt_inverse_multi<kFoldThreads, 4>(ipolys, iprime, tw);
+    ntt_inverse_r4<kFoldThreads, 4>(ipolys, iprime, tw);
     const size_t j = (size_t)w * d + i;
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      const int z = tid + s * 1024;
+    for (int s = 0; s < kFoldSlots; ++s) {
+      const int z = tid + s * kFoldThreads;
       const uint64_t x0 = crt_compose(prod[0 * kN + z], prod[2 * kN + z]);
       const uint64_t x1 = crt_compose(prod[1 * kN + z], prod[3 * kN + z]);
       sum[0][s] = reduce_small_q(x0 + args.a_hat[j * kN + z]);
       sum[1][s] = reduce_small_q(x1 + cv[j * kN + z] + Pc[j * kN + z]);
     }
-    // The next iteration writes dig only; prod is rewritten after two barriers.
+    // dig is rewritten next; prod only after two more barriers.
   }
 
   // switch_and_keep(2^20, 268369921, 2048): row 0 -> 28 bits, row 1 -> 20 bits.
@@ -578,8 +575,8 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
   uint32_t* r0 = dig;
   uint32_t* r1 = dig + kN;
 #pragma unroll
-  for (int s = 0; s < 2; ++s) {
-    const int z = tid + s * 1024;
+  for (int s = 0; s < kFoldSlots; ++s) {
+    const int z = tid + s * kFoldThreads;
     r0[z] = (uint32_t)rescale_from_q(sum[0][s], kQPrime2);
     r1[z] = (uint32_t)rescale_from_q(sum[1][s], kQPrime1);
   }
@@ -613,7 +610,7 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
   }
 }
 
-size_t fold_lds_bytes() { return (size_t)(2 * 6 * kN + 4 * kN) * sizeof(uint32_t); }
+size_t fold_lds_bytes() { return (size_t)(12 * kN + 4 * kN) * sizeof(uint32_t); }
 
 int launch_fold(uint8_t* out, const uint64_t* a_hat, const uint64_t* cvec, const uint64_t* P,
                 const uint32_t* bodies, const uint32_t* neg_a, const uint32_t* tw, int64_t instances, int d,

@@ -262,40 +262,46 @@ __global__ __launch_bounds__(64 * WM * WN) void k_packing_mfma(PackMfmaArgs a) {
     }
   };
   // A fragment rows: lane = 4*bl + l + 16*g, dword tq holds terms n = 16g + 4tq .. +3.
-  // Item e (< MT*64) of a step: tq = e%4, g = (e/4)%4, bl = (e/16)%4, mt = e/64.
-  constexpr int kAItems = (MT * 64 + kThreads - 1) / kThreads;
-  auto gather_a = [&](int buf, uint32_t (&vals)[kAItems][4]) {
+  // Item e (< 16*MT) of a step: client quad cq = e % MT (= M-tile), term quad
+  // tq4 = e / MT (g = tq4/4, tq = tq4%4).  Consecutive lanes read consecutive
+  // client quads of one term, i.e. contiguous 16-byte key loads.
+  constexpr int kAItems = (16 * MT + kThreads - 1) / kThreads;
+  auto gather_a = [&](int buf, u4 (&vals)[kAItems][4]) {
 #pragma unroll
     for (int r = 0; r < kAItems; ++r) {
       const int e = tid + r * kThreads;
-      const int tq = e & 3, g = (e >> 2) & 3, bl = (e >> 4) & 3, mt = e >> 6;
-      const int b = 4 * (mt0 + mt) + bl;
+      const int cq = e % MT, tq4 = e / MT;
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int tn = 16 * g + 4 * tq + u;
-        const int slot = src_slot[buf][tn], row = src_row[buf][tn];
-        uint32_t v = 0;
-        if (e < MT * 64 && slot >= 0) {
-          const uint32_t* base = row < 3 ? yk : zk;
-          v = base[((size_t)(row % 3) * kN + slot) * a.bpad + b];
+        u4 v = {0, 0, 0, 0};
+        if (e < 16 * MT) {
+          const int tn = 4 * tq4 + u;
+          const int slot = src_slot[buf][tn], row = src_row[buf][tn];
+          if (slot >= 0) {
+            const uint32_t* base = row < 3 ? yk : zk;
+            v = *reinterpret_cast<const u4*>(base + ((size_t)(row % 3) * kN + slot) * a.bpad + 4 * (mt0 + cq));
+          }
         }
         vals[r][u] = v;
       }
     }
   };
-  auto store_a = [&](int buf, const uint32_t (&vals)[kAItems][4]) {
+  auto store_a = [&](int buf, const u4 (&vals)[kAItems][4]) {
 #pragma unroll
     for (int r = 0; r < kAItems; ++r) {
       const int e = tid + r * kThreads;
-      if (e >= MT * 64) continue;
-      const int tq = e & 3, g = (e >> 2) & 3, bl = (e >> 4) & 3, mt = e >> 6;
+      if (e >= 16 * MT) continue;
+      const int cq = e % MT, tq4 = e / MT;
+      const int g = tq4 >> 2, tq = tq4 & 3;
 #pragma unroll
-      for (int l = 0; l < 4; ++l) {
-        uint32_t w = 0;
+      for (int bl = 0; bl < 4; ++bl)
 #pragma unroll
-        for (int u = 0; u < 4; ++u) w |= ((vals[r][u] >> (7 * l)) & 127u) << (8 * u);
-        reinterpret_cast<uint32_t*>(&afrag[buf][mt][4 * bl + l + 16 * g])[tq] = w;
-      }
+        for (int l = 0; l < 4; ++l) {
+          uint32_t w = 0;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) w |= ((vals[r][u][bl] >> (7 * l)) & 127u) << (8 * u);
+          reinterpret_cast<uint32_t*>(&afrag[buf][cq][4 * bl + l + 16 * g])[tq] = w;
+        }
     }
   };
 
@@ -310,7 +316,7 @@ __global__ __launch_bounds__(64 * WM * WN) void k_packing_mfma(PackMfmaArgs a) {
   // are computed; one barrier per step.
   u4 bregs[kBLoads > 0 ? kBLoads : 1];
   u4 bcur[NTW], bnext[NTW];
-  uint32_t avals[kAItems][4];
+  u4 avals[kAItems][4];
   stage_terms(0, 0);
   stage_terms(1, 1);
   if (kDirect) {
