@@ -155,7 +155,11 @@ __global__ __launch_bounds__(256) void k_gemv_mfma(uint64_t* __restrict__ c, con
 
 constexpr int kBatchChunkSteps = 4;  // 256 rows per LDS chunk
 
-template <int MQ>
+// Block = 4 waves; wave w owns TW column tiles (16 columns each), so every
+// query fragment read from LDS feeds 2*TW MFMAs and every DB fragment feeds
+// 2*MQ MFMAs.  The DB fragments of a chunk are requested before the chunk's
+// query fragments are staged, so the two overlap.
+template <int MQ, int TW>
 __global__ __launch_bounds__(256) void k_gemv_mfma_batch(uint64_t* __restrict__ c, int64_t c_client_stride,
                                                          const int8_t* __restrict__ afrag,
                                                          const int32_t* __restrict__ rsum,
@@ -165,45 +169,63 @@ __global__ __launch_bounds__(256) void k_gemv_mfma_batch(uint64_t* __restrict__ 
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int j = lane & 15, g = lane >> 4;
-  const int64_t col = (int64_t)blockIdx.x * 64 + wave * 16 + j;
-  const bool valid = col < db_cols;
-  const uint16_t* colp = db + (valid ? col : 0) * (int64_t)db_rows + 8 * g;
+  const int64_t col0 = (int64_t)blockIdx.x * (64 * TW) + wave * (16 * TW);
+  const uint16_t* colp[TW];
+  bool valid[TW];
+#pragma unroll
+  for (int tt = 0; tt < TW; ++tt) {
+    const int64_t col = col0 + 16 * tt + j;
+    valid[tt] = col < db_cols;
+    colp[tt] = db + (valid[tt] ? col : 0) * (int64_t)db_rows + 8 * g;
+  }
   const int steps = db_rows / 64;
   const int nclients = min(MQ, count - client0);
 
-  v4i acc_lo[MQ], acc_hi[MQ];
+  v4i acc_lo[MQ][TW], acc_hi[MQ][TW];
 #pragma unroll
-  for (int q = 0; q < MQ; ++q) {
-    acc_lo[q] = v4i{0, 0, 0, 0};
-    acc_hi[q] = v4i{0, 0, 0, 0};
-  }
+  for (int q = 0; q < MQ; ++q)
+#pragma unroll
+    for (int tt = 0; tt < TW; ++tt) {
+      acc_lo[q][tt] = v4i{0, 0, 0, 0};
+      acc_hi[q][tt] = v4i{0, 0, 0, 0};
+    }
   for (int s0 = 0; s0 < steps; s0 += kBatchChunkSteps) {
+    u4 dd[kBatchChunkSteps][TW][2];
+#pragma unroll
+    for (int ss = 0; ss < kBatchChunkSteps; ++ss)
+#pragma unroll
+      for (int tt = 0; tt < TW; ++tt) {
+        const uint16_t* p = colp[tt] + 64 * (s0 + ss);
+        dd[ss][tt][0] = valid[tt] ? __builtin_nontemporal_load(reinterpret_cast<const u4*>(p)) : u4{0, 0, 0, 0};
+        dd[ss][tt][1] = valid[tt] ? __builtin_nontemporal_load(reinterpret_cast<const u4*>(p + 32)) : u4{0, 0, 0, 0};
+      }
     __syncthreads();
     for (int e = threadIdx.x; e < MQ * kBatchChunkSteps * 64; e += 256) {
       const int q = e / (kBatchChunkSteps * 64), r = e % (kBatchChunkSteps * 64);
       v4i v = {0, 0, 0, 0};
-      if (q < nclients)
-        v = reinterpret_cast<const v4i*>(afrag)[((int64_t)(client0 + q) * steps + s0) * 64 + r];
+      if (q < nclients) v = reinterpret_cast<const v4i*>(afrag)[((int64_t)(client0 + q) * steps + s0) * 64 + r];
       a_lds[q][r / 64][r % 64] = v;
     }
     __syncthreads();
 #pragma unroll
     for (int ss = 0; ss < kBatchChunkSteps; ++ss) {
-      const uint16_t* p = colp + 64 * (s0 + ss);
-      u4 d0 = {0, 0, 0, 0}, d1 = {0, 0, 0, 0};
-      if (valid) {
-        d0 = __builtin_nontemporal_load(reinterpret_cast<const u4*>(p));
-        d1 = __builtin_nontemporal_load(reinterpret_cast<const u4*>(p + 32));
+      v4i lo[TW], hi[TW];
+#pragma unroll
+      for (int tt = 0; tt < TW; ++tt) {
+        const u4 d0 = dd[ss][tt][0], d1 = dd[ss][tt][1];
+        lo[tt] = v4i{(int)plane_lo(d0.x, d0.y), (int)plane_lo(d0.z, d0.w), (int)plane_lo(d1.x, d1.y),
+                     (int)plane_lo(d1.z, d1.w)};
+        hi[tt] = v4i{(int)plane_hi(d0.x, d0.y), (int)plane_hi(d0.z, d0.w), (int)plane_hi(d1.x, d1.y),
+                     (int)plane_hi(d1.z, d1.w)};
       }
-      const v4i lo = {(int)plane_lo(d0.x, d0.y), (int)plane_lo(d0.z, d0.w), (int)plane_lo(d1.x, d1.y),
-                      (int)plane_lo(d1.z, d1.w)};
-      const v4i hi = {(int)plane_hi(d0.x, d0.y), (int)plane_hi(d0.z, d0.w), (int)plane_hi(d1.x, d1.y),
-                      (int)plane_hi(d1.z, d1.w)};
 #pragma unroll
       for (int q = 0; q < MQ; ++q) {
         const v4i a = a_lds[q][ss][lane];
-        acc_lo[q] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, lo, acc_lo[q], 0, 0, 0);
-        acc_hi[q] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, hi, acc_hi[q], 0, 0, 0);
+#pragma unroll
+        for (int tt = 0; tt < TW; ++tt) {
+          acc_lo[q][tt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, lo[tt], acc_lo[q][tt], 0, 0, 0);
+          acc_hi[q][tt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, hi[tt], acc_hi[q][tt], 0, 0, 0);
+        }
       }
     }
   }
@@ -211,17 +233,23 @@ __global__ __launch_bounds__(256) void k_gemv_mfma_batch(uint64_t* __restrict__ 
 #pragma unroll
   for (int q = 0; q < MQ; ++q) {
     const int client = client0 + q;
-    int64_t v = 0;
-    if (client < count) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-        v += ((int64_t)256 * acc_hi[q][r] + (int64_t)acc_lo[q][r] + (int64_t)rsum[client * 16 + 4 * g + r] * 32896)
-             << (7 * r);
+    for (int tt = 0; tt < TW; ++tt) {
+      int64_t v = 0;
+      if (client < count) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          v += ((int64_t)256 * acc_hi[q][tt][r] + (int64_t)acc_lo[q][tt][r] +
+                (int64_t)rsum[client * 16 + 4 * g + r] * 32896)
+               << (7 * r);
+      }
+      const uint64_t red = pr ? barrett64((uint64_t)v, kCr1Q1, kQ1) : barrett64((uint64_t)v, kCr1Q0, kQ0);
+      const uint64_t other = __shfl_down(red, 16, 64);
+      const int64_t col = col0 + 16 * tt + j;
+      if (pr == 0 && h < nq && valid[tt] && client < count)
+        c[(int64_t)client * c_client_stride + (int64_t)h * db_cols + col] =
+            crt_compose((uint32_t)red, (uint32_t)other);
     }
-    const uint64_t red = pr ? barrett64((uint64_t)v, kCr1Q1, kQ1) : barrett64((uint64_t)v, kCr1Q0, kQ0);
-    const uint64_t other = __shfl_down(red, 16, 64);
-    if (pr == 0 && h < nq && valid && client < count)
-      c[(int64_t)client * c_client_stride + (int64_t)h * db_cols + col] = crt_compose((uint32_t)red, (uint32_t)other);
   }
 }
 
@@ -251,18 +279,14 @@ int launch_gemv_mfma_batch(uint64_t* c, int64_t c_client_stride, const uint64_t*
   const int prep_threads = (db_rows / 64) * 64;
   hipLaunchKernelGGL(k_gemv_prep_batch, dim3((prep_threads + 255) / 256, count), dim3(256), 0, s, rows,
                      rows_client_stride, nq, db_rows, count, afrag, rsum);
-  const unsigned blocks = (unsigned)((db_cols + 63) / 64);
-  for (int c0 = 0; c0 < count; c0 += 16) {
+  for (int c0 = 0; c0 < count; c0 += 8) {
     const int left = count - c0;
-    if (left > 8)
-      hipLaunchKernelGGL(k_gemv_mfma_batch<16>, dim3(blocks), dim3(256), 0, s, c, c_client_stride, afrag, rsum, db, nq,
-                         db_rows, db_cols, c0, count);
-    else if (left > 4)
-      hipLaunchKernelGGL(k_gemv_mfma_batch<8>, dim3(blocks), dim3(256), 0, s, c, c_client_stride, afrag, rsum, db, nq,
-                         db_rows, db_cols, c0, count);
+    if (left > 4)
+      hipLaunchKernelGGL((k_gemv_mfma_batch<8, 2>), dim3((unsigned)((db_cols + 127) / 128)), dim3(256), 0, s, c,
+                         c_client_stride, afrag, rsum, db, nq, db_rows, db_cols, c0, count);
     else
-      hipLaunchKernelGGL(k_gemv_mfma_batch<4>, dim3(blocks), dim3(256), 0, s, c, c_client_stride, afrag, rsum, db, nq,
-                         db_rows, db_cols, c0, count);
+      hipLaunchKernelGGL((k_gemv_mfma_batch<4, 2>), dim3((unsigned)((db_cols + 127) / 128)), dim3(256), 0, s, c,
+                         c_client_stride, afrag, rsum, db, nq, db_rows, db_cols, c0, count);
   }
   return 0;
 }

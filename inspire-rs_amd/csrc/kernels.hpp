@@ -45,10 +45,13 @@ void launch_tile_bold_t(uint8_t* out, const uint64_t* bt, const uint64_t* btb, c
 void launch_tile_bold_t_synth(uint8_t* out, uint64_t seed, int64_t instances, hipStream_t s);
 int batch_key_pad(int count);
 size_t batch_keys_bytes(int bpad);
-// S[client][instances][2][N]; y_body/z_body: [count][3][N] (key_client_stride elements apart).
-int launch_packing_mfma(uint32_t* S, int64_t s_client_stride, const uint8_t* tiled, uint32_t* ykeys, uint32_t* zkeys,
-                        const uint64_t* y_body, const uint64_t* z_body, int64_t key_client_stride, int count,
-                        int64_t instances, hipStream_t s);
+// Device bytes of the client-key buffer for up to `count` clients.
+size_t packing_keys_bytes(int count);
+// S[client][instances][2][N]; y_body/z_body: [count][3][N] (key_client_stride
+// elements apart); term_src: the per-device [N][6144] term-source table.
+int launch_packing_mfma(uint32_t* S, int64_t s_client_stride, const uint8_t* tiled, const uint16_t* term_src,
+                        uint32_t* keys, const uint64_t* y_body, const uint64_t* z_body, int64_t key_client_stride,
+                        int count, int64_t instances, hipStream_t s);
 
 // c: [nq][db_cols] canonical mod q; rows: [nq][db_rows] condensed; db: [db_cols][db_rows].
 int launch_gemv(uint64_t* c, const uint64_t* rows, int nq, const uint16_t* db, int db_rows, int64_t db_cols,

@@ -389,6 +389,205 @@ __global__ __launch_bounds__(64 * WM * WN) void k_packing_mfma(PackMfmaArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Wide batched GEMM: one block holds all 4*MT clients of its group, so each
+// B fragment leaves HBM once per group, and the block's four waves split the NT
+// N-tiles (NTW each, the last wave possibly fewer).  B fragments go straight
+// from HBM into the wave's registers; the A fragments (built from the clients'
+// keys) are shared through LDS.  Both run D steps ahead of the matrix cores
+// through register rings, and every load is unconditional (clamped addresses),
+// so the only waits in the loop are the ones on the data a step consumes.
+//
+// Keys are stored as limb bytes, keys[p][row][bpad] with row = plane*N + slot
+// (planes 0-2: y rows, 3-5: z_body rows) and a final zero row; byte l of a key
+// is bits 7l..7l+6 of its residue.  term_src[z][n] names the row term n reads.
+// ---------------------------------------------------------------------------
+
+constexpr int64_t kKeyRows = 6 * kN + 1;
+
+__device__ inline uint32_t limb_bytes(uint32_t r) {
+  return (r & 127u) | ((r << 1) & (127u << 8)) | ((r << 2) & (127u << 16)) | ((r << 3) & (127u << 24));
+}
+
+__global__ void k_batch_keys_limbs(uint32_t* __restrict__ keys, const uint64_t* __restrict__ y_body,
+                                   const uint64_t* __restrict__ z_body, int64_t client_stride, int count, int bpad) {
+  const int64_t total = kKeyRows * bpad;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e % bpad);
+    const int64_t row = e / bpad;
+    uint64_t v = 0;
+    if (b < count && row < 6 * kN) {
+      const int plane = (int)(row / kN);
+      const int64_t off = (int64_t)b * client_stride + (plane % 3) * kN + row % kN;
+      v = plane < 3 ? y_body[off] : z_body[off];
+    }
+    keys[e] = limb_bytes((uint32_t)v);
+    keys[total + e] = limb_bytes((uint32_t)(v >> 32));
+  }
+}
+
+struct PackWideArgs {
+  uint32_t* S;                // [count][instances][2][N]
+  const uint8_t* tiled;       // [2][N][ntiles][96][64][16]
+  const uint32_t* keys;       // [2][kKeyRows][bpad] limb bytes
+  const uint16_t* term_src;   // [N][kKPad]
+  int64_t instances;
+  int64_t ntiles;
+  int bpad;
+  int count;
+  int ngroups;                // N-tile groups per (p, z)
+  int mgroups;                // client groups (4*MT clients each)
+  int64_t s_client_stride;
+};
+
+template <int MT, int NTW, int NT, int D>
+__global__ __launch_bounds__(256) void k_packing_wide(PackWideArgs a) {
+  static_assert(16 * MT <= 256 && (16 * MT) % 64 == 0, "A items must fill whole waves");
+  static_assert(NTW * 4 >= NT && NTW * 3 < NT, "four waves cover the N-tiles");
+  static_assert(kKSteps % D == 0 && D % 2 == 0, "ring depth must divide the steps and be even");
+  constexpr int kAWaves = 16 * MT / 64;
+  constexpr int kTileStride = 65;  // u4 per A tile in LDS: one pad row keeps the limb writes conflict-free
+  __shared__ uint16_t tsrc[kKPad];
+  __shared__ u4 afrag[2][MT * kTileStride];
+
+  // Blocks of one (p, z) are consecutive on one XCD (block id mod 8), client
+  // groups innermost so the groups' reads of a B fragment meet in L2.
+  const int64_t bid = blockIdx.x;
+  const int xcd = (int)(bid % 8);
+  int64_t rest = bid / 8;
+  const int mg = (int)(rest % a.mgroups);
+  rest /= a.mgroups;
+  const int ng = (int)(rest % a.ngroups);
+  rest /= a.ngroups;
+  const int z = xcd * (kN / 8) + (int)(rest % (kN / 8));
+  const int p = (int)(rest / (kN / 8));
+
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const bool a_wave = wave < kAWaves;
+
+  {
+    const u4* src = reinterpret_cast<const u4*>(a.term_src + (size_t)z * kKPad);
+    for (int e = tid; e < kKPad / 8; e += 256) reinterpret_cast<u4*>(tsrc)[e] = src[e];
+  }
+
+  // A item of this thread: client quad cq, terms 4*tq4 .. 4*tq4+3 of a step.
+  const int cq = tid % MT, tq4 = (tid / MT) & 15;
+  const int g = tq4 >> 2, tq = tq4 & 3;
+  const uint32_t* keys_p = a.keys + (size_t)p * kKeyRows * a.bpad + 4 * (mg * MT + cq);
+  const uint32_t row_bytes = (uint32_t)a.bpad * 4u;
+
+  // B tiles of this wave (clamped: tiles past the group or the matrix repeat
+  // the last real one and their sums are dropped).
+  const int nt0 = ng * NT;
+  const uint8_t* tb = a.tiled + ((size_t)p * kN + z) * a.ntiles * kKSteps * 1024;
+  const uint8_t* bbase[NTW];
+#pragma unroll
+  for (int y = 0; y < NTW; ++y) {
+    const int ntl = min(wave * NTW + y, NT - 1);
+    const int64_t nt = min((int64_t)(nt0 + ntl), a.ntiles - 1);
+    bbase[y] = tb + (size_t)nt * kKSteps * 1024 + lane * 16;
+  }
+
+  auto gather = [&](int s, u4 (&v)[4]) {
+    const uint2 ts = *reinterpret_cast<const uint2*>(&tsrc[64 * s + 4 * tq4]);
+    const uint32_t r[4] = {ts.x & 0xffffu, ts.x >> 16, ts.y & 0xffffu, ts.y >> 16};
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      v[u] = *reinterpret_cast<const u4*>(reinterpret_cast<const uint8_t*>(keys_p) + r[u] * row_bytes);
+  };
+  // 4x4 byte transposes: dword (client bl, limb l) gets limb l of the four terms.
+  auto transform = [&](int buf, const u4 (&v)[4]) {
+    uint32_t* base = reinterpret_cast<uint32_t*>(&afrag[buf][cq * kTileStride + 16 * g]) + tq;
+#pragma unroll
+    for (int bl = 0; bl < 4; ++bl) {
+      const uint32_t x0 = v[0][bl], x1 = v[1][bl], x2 = v[2][bl], x3 = v[3][bl];
+      const uint32_t t0 = __builtin_amdgcn_perm(x1, x0, 0x05010400u);
+      const uint32_t t1 = __builtin_amdgcn_perm(x1, x0, 0x07030602u);
+      const uint32_t t2 = __builtin_amdgcn_perm(x3, x2, 0x05010400u);
+      const uint32_t t3 = __builtin_amdgcn_perm(x3, x2, 0x07030602u);
+      base[(4 * bl + 0) * 4] = __builtin_amdgcn_perm(t2, t0, 0x05040100u);
+      base[(4 * bl + 1) * 4] = __builtin_amdgcn_perm(t2, t0, 0x07060302u);
+      base[(4 * bl + 2) * 4] = __builtin_amdgcn_perm(t3, t1, 0x05040100u);
+      base[(4 * bl + 3) * 4] = __builtin_amdgcn_perm(t3, t1, 0x07060302u);
+    }
+  };
+  auto load_b = [&](int s, u4 (&regs)[NTW]) {
+#pragma unroll
+    for (int y = 0; y < NTW; ++y)
+      regs[y] = __builtin_nontemporal_load(reinterpret_cast<const u4*>(bbase[y] + (size_t)s * 1024));
+  };
+
+  v4i acc[MT][NTW];
+#pragma unroll
+  for (int x = 0; x < MT; ++x)
+#pragma unroll
+    for (int y = 0; y < NTW; ++y) acc[x][y] = v4i{0, 0, 0, 0};
+
+  // Rings: A keys of step t in araw[t % D], B fragments of step t in breg[t % D].
+  u4 araw[D][4];
+  u4 breg[D][NTW];
+  __syncthreads();
+  if (a_wave) {
+    gather(0, araw[0]);
+    transform(0, araw[0]);
+  }
+#pragma unroll
+  for (int t = 0; t < D; ++t) {
+    if (a_wave) gather(t + 1, araw[(t + 1) % D]);
+    load_b(t, breg[t]);
+  }
+  __syncthreads();
+
+  for (int s0 = 0; s0 < kKSteps; s0 += D) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int s = s0 + d;
+      if (a_wave) {
+        transform((d + 1) & 1, araw[(d + 1) % D]);             // A of step s+1 (unused past the end)
+        gather(min(s + 1 + D, kKSteps - 1), araw[(d + 1) % D]);
+      }
+      const u4* at = afrag[d & 1];
+#pragma unroll
+      for (int x = 0; x < MT; ++x) {
+        const v4i af = __builtin_bit_cast(v4i, at[x * kTileStride + lane]);
+#pragma unroll
+        for (int y = 0; y < NTW; ++y)
+          acc[x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, __builtin_bit_cast(v4i, breg[d][y]), acc[x][y], 0, 0, 0);
+      }
+      load_b(min(s + D, kKSteps - 1), breg[d]);
+      __syncthreads();
+      __builtin_amdgcn_sched_barrier(0);  // keep each step's loads in its own step
+    }
+  }
+
+  // Epilogue as in k_packing_mfma: lane (col c, G) holds C[4G + l][c].
+  const int c = lane & 15, G = lane >> 4;
+  const int m = c & 3, jl = c >> 2;
+  const uint32_t q = prime(p);
+  const uint32_t pw = (uint32_t)(((uint64_t)1 << (7 * m)) % q);
+#pragma unroll
+  for (int x = 0; x < MT; ++x) {
+#pragma unroll
+    for (int y = 0; y < NTW; ++y) {
+      uint64_t X = 0;
+#pragma unroll
+      for (int l = 0; l < 4; ++l) X += (uint64_t)(uint32_t)acc[x][y][l] << (7 * l);
+      uint32_t v = (uint32_t)barrett64(X, cr1(p), q);
+      v = p ? mulmod<1>(v, pw) : mulmod<0>(v, pw);
+      uint32_t tot = v + __shfl_xor(v, 1, 64);
+      tot = (tot >= q) ? tot - q : tot;
+      uint32_t tot2 = tot + __shfl_xor(tot, 2, 64);
+      tot2 = (tot2 >= q) ? tot2 - q : tot2;
+      const int ntl = wave * NTW + y;
+      const int64_t j = 4 * (int64_t)(nt0 + ntl) + jl;
+      const int b = 4 * (mg * MT + x) + G;
+      if (m == 0 && ntl < NT && b < a.count && j < a.instances)
+        a.S[(size_t)b * a.s_client_stride + ((size_t)j * 2 + p) * kN + z] = tot2;
+    }
+  }
+}
+
 void launch_tile_bold_t(uint8_t* out, const uint64_t* bt, const uint64_t* btb, const uint64_t* bth, int64_t instances,
                         hipStream_t s) {
   hipLaunchKernelGGL(k_tile_bold_t, dim3(65536), dim3(256), 0, s, out, bt, btb, bth, instances);
@@ -398,13 +597,25 @@ void launch_tile_bold_t_synth(uint8_t* out, uint64_t seed, int64_t instances, hi
   hipLaunchKernelGGL(k_tile_bold_t_synth, dim3(65536), dim3(256), 0, s, out, seed, instances);
 }
 
+// Client padding: 4 and 16 clients for the k_packing_mfma configurations,
+// then groups of 32 and 64 for k_packing_wide.
 int batch_key_pad(int count) {
-  const int mt = count <= 4 ? 1 : (count <= 16 ? 4 : 8);
-  const int unit = 4 * mt;
+  const int unit = count <= 4 ? 4 : (count <= 16 ? 16 : (count <= 32 ? 32 : 64));
   return (count + unit - 1) / unit * unit;
 }
 
 size_t batch_keys_bytes(int bpad) { return (size_t)2 * kTExp * kN * bpad * sizeof(uint32_t); }
+
+// Key buffer for up to `count` clients, large enough for either key layout.
+size_t packing_keys_bytes(int count) { return (size_t)2 * kKeyRows * batch_key_pad(count) * sizeof(uint32_t); }
+
+template <int MT, int NTW, int NT, int D>
+static void launch_wide(PackWideArgs a, hipStream_t s) {
+  a.ngroups = (int)((a.ntiles + NT - 1) / NT);
+  a.mgroups = a.bpad / (4 * MT);
+  const int64_t blocks = (int64_t)kN * 2 * a.ngroups * a.mgroups;
+  hipLaunchKernelGGL((k_packing_wide<MT, NTW, NT, D>), dim3((unsigned)blocks), dim3(256), 0, s, a);
+}
 
 template <int MT, int NT, int WM, int WN>
 static void launch_cfg(PackMfmaArgs a, hipStream_t s) {
@@ -414,21 +625,32 @@ static void launch_cfg(PackMfmaArgs a, hipStream_t s) {
   hipLaunchKernelGGL((k_packing_mfma<MT, NT, WM, WN>), dim3((unsigned)blocks), dim3(64 * WM * WN), 0, s, a);
 }
 
-int launch_packing_mfma(uint32_t* S, int64_t s_client_stride, const uint8_t* tiled, uint32_t* ykeys, uint32_t* zkeys,
-                        const uint64_t* y_body, const uint64_t* z_body, int64_t key_client_stride, int count,
-                        int64_t instances, hipStream_t s) {
+int launch_packing_mfma(uint32_t* S, int64_t s_client_stride, const uint8_t* tiled, const uint16_t* term_src,
+                        uint32_t* keys, const uint64_t* y_body, const uint64_t* z_body, int64_t key_client_stride,
+                        int count, int64_t instances, hipStream_t s) {
   if (count < 1) return -1;
   const int bpad = batch_key_pad(count);
+  if (bpad > 16) {
+    const int64_t total = kKeyRows * bpad;
+    hipLaunchKernelGGL(k_batch_keys_limbs, dim3((unsigned)((total + 255) / 256 > 4096 ? 4096 : (total + 255) / 256)),
+                       dim3(256), 0, s, keys, y_body, z_body, key_client_stride, count, bpad);
+    PackWideArgs a{S, tiled, keys, term_src, instances, tiled_ntiles(instances), bpad, count, 0, 0, s_client_stride};
+    if (bpad == 32)
+      launch_wide<8, 6, 22, 4>(a, s);   // 32 clients: waves of 8 x 6 tiles over 22 N-tiles
+    else
+      launch_wide<16, 3, 11, 4>(a, s);  // 64-client groups: waves of 16 x 3 tiles over 11 N-tiles
+    return 0;
+  }
+  uint32_t* ykeys = keys;
+  uint32_t* zkeys = keys + batch_keys_bytes(bpad) / sizeof(uint32_t);
   const int64_t total = (int64_t)kTExp * kN * bpad;
   hipLaunchKernelGGL(k_batch_keys, dim3((unsigned)((total + 255) / 256 > 4096 ? 4096 : (total + 255) / 256)),
                      dim3(256), 0, s, ykeys, zkeys, y_body, z_body, key_client_stride, count, bpad);
   PackMfmaArgs a{S, tiled, ykeys, zkeys, instances, tiled_ntiles(instances), bpad, count, 0, 0, s_client_stride};
   if (bpad <= 4)
     launch_cfg<1, 44, 1, 4>(a, s);      // 1 M-tile: 4 waves x 11 N-tiles, B direct from HBM
-  else if (bpad <= 16)
-    launch_cfg<4, 44, 1, 4>(a, s);      // 4 M-tiles: 4 waves x (4 x 11) tiles, B direct
   else
-    launch_cfg<8, 22, 2, 2>(a, s);      // 8 M-tiles: waves 2 x 2 of 4 x 11 tiles, B via LDS
+    launch_cfg<4, 44, 1, 4>(a, s);      // 4 M-tiles: 4 waves x (4 x 11) tiles, B direct
   return 0;
 }
 

@@ -57,20 +57,29 @@ const DeviceConstants* device_constants() {
   DeviceConstants& dc = g_consts[dev];
   if (dc.tw) return &dc;
   std::vector<uint32_t> tw(2 * 4 * kN), na(2 * kN);
+  std::vector<uint16_t> ts((size_t)kN * kPackTerms);
   build_ntt_tables(tw.data());
   build_rgsw_neg_a(na.data(), tw.data());
+  build_packing_term_sources(ts.data());
   uint32_t *dtw = nullptr, *dna = nullptr;
-  if (hipMalloc(&dtw, tw.size() * 4) != hipSuccess) return nullptr;
-  if (hipMalloc(&dna, na.size() * 4) != hipSuccess) {
-    (void)hipFree(dtw);
+  uint16_t* dts = nullptr;
+  auto release = [&]() {
+    if (dtw) (void)hipFree(dtw);
+    if (dna) (void)hipFree(dna);
+    if (dts) (void)hipFree(dts);
+  };
+  if (hipMalloc(&dtw, tw.size() * 4) != hipSuccess || hipMalloc(&dna, na.size() * 4) != hipSuccess ||
+      hipMalloc(&dts, ts.size() * 2) != hipSuccess) {
+    release();
     return nullptr;
   }
   if (hipMemcpy(dtw, tw.data(), tw.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(dna, na.data(), na.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
-    (void)hipFree(dtw);
-    (void)hipFree(dna);
+      hipMemcpy(dna, na.data(), na.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dts, ts.data(), ts.size() * 2, hipMemcpyHostToDevice) != hipSuccess) {
+    release();
     return nullptr;
   }
+  dc.term_src = dts;
   dc.neg_a = dna;
   dc.tw = dtw;
   return &dc;
@@ -301,7 +310,7 @@ int alloc_batch(inspire_server_t* s, int max) {
   if (!rc) rc = dev_alloc(&b.S, (size_t)max * inst * 2 * kN, "batch packing output");
   if (!rc) rc = dev_alloc(&b.P, (size_t)max * inst * kN, "batch post output");
   if (!rc) rc = dev_alloc(&b.cvec, (size_t)max * mq * s->db_cols, "batch gemv output");
-  if (!rc) rc = dev_alloc((uint8_t**)&b.keys, 2 * batch_keys_bytes(batch_key_pad(max)), "batch keys");
+  if (!rc) rc = dev_alloc((uint8_t**)&b.keys, packing_keys_bytes(max), "batch keys");
   if (!rc)
     rc = dev_alloc((uint8_t**)&b.gemv_scratch, gemv_mfma_batch_scratch_bytes((int)rows, max), "batch gemv scratch");
   if (!rc && hipEventCreateWithFlags(&b.done, hipEventDisableTiming) != hipSuccess)
@@ -353,7 +362,7 @@ int alloc_slots(inspire_server_t* s, int count) {
     if (!rc) rc = dev_alloc(&sl.cvec, (size_t)mq * s->db_cols, "gemv output");
     if (!rc) rc = dev_alloc(&sl.out, (size_t)mq * s->c * kRespBytes, "responses");
     if (!rc) rc = dev_alloc((uint8_t**)&sl.gemv_scratch, gemv_mfma_scratch_bytes((int)rows), "gemv scratch");
-    if (!rc) rc = dev_alloc((uint8_t**)&sl.keys, 2 * batch_keys_bytes(batch_key_pad(1)), "packing keys");
+    if (!rc) rc = dev_alloc((uint8_t**)&sl.keys, packing_keys_bytes(1), "packing keys");
     if (!rc && hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess)
       rc = set_error(INSPIRE_ERR_HIP, "hipEventCreate failed");
     if (rc) return rc;
@@ -477,8 +486,7 @@ extern "C" int inspire_respond_batch_device(inspire_server_t* s, const uint8_t* 
   launch_unpack_query_batch(d_queries, (int64_t)q_stride, nq, (int)rows, count, b.y_body, b.z_body, b.rows, b.bodies,
                             st);
   mark(1);
-  const int bpad = batch_key_pad(count);
-  launch_packing_mfma(b.S, inst * 2 * kN, s->tiled, b.keys, b.keys + batch_keys_bytes(bpad) / 4, b.y_body, b.z_body,
+  launch_packing_mfma(b.S, inst * 2 * kN, s->tiled, s->consts->term_src, b.keys, b.y_body, b.z_body,
                       (int64_t)kTExp * kN, count, inst, st);
   mark(2);
   launch_post(b.P, b.S, s->consts->tw, (int)(count * inst), st);
@@ -610,8 +618,8 @@ int run_pipeline(inspire_server_t* s, Slot& sl, const uint8_t* d_query, int nq, 
   mark(1);
   if (s->packing_kind == INSPIRE_PACKING_MFMA) {
     if (!s->tiled_valid) return set_error(INSPIRE_ERR_STATE, "tiled key-switch material not built");
-    launch_packing_mfma(sl.S, inst * 2 * kN, s->tiled, sl.keys, sl.keys + batch_keys_bytes(batch_key_pad(1)) / 4,
-                        sl.y_body, sl.z_body, (int64_t)kTExp * kN, 1, inst, st);
+    launch_packing_mfma(sl.S, inst * 2 * kN, s->tiled, s->consts->term_src, sl.keys, sl.y_body, sl.z_body,
+                        (int64_t)kTExp * kN, 1, inst, st);
   } else {
     launch_packing(sl.S, s->bt, s->btb, s->bth, sl.y_body, sl.z_body, (int)inst, st);
   }

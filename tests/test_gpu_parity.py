@@ -339,14 +339,18 @@ def test_packing_mfma_matches_live_oracle(torch):
     srv.close()
 
 
-@pytest.mark.parametrize("count", [1, 3, 5, 17, 20])
-def test_batch_results_equal_solo_results(torch, count):
+# Client counts cover every packing configuration: 4- and 16-client tiles,
+# 32- and 64-client wide blocks, and two 64-client groups (70); 96 instances
+# give several N-tile groups with a partial last one.
+@pytest.mark.parametrize("count,inst,d", [(1, 8, 4), (3, 8, 4), (5, 8, 4), (17, 8, 4), (20, 8, 4), (40, 8, 4),
+                                          (70, 8, 4), (24, 96, 8), (40, 96, 8)])
+def test_batch_results_equal_solo_results(torch, count, inst, d):
     # every query's response is the same alone and in the mix
-    say("batch vs solo", count)
-    db_rows, inst, d = 2048, 8, 4
+    say("batch vs solo", count, inst, d)
+    db_rows = 2048
     srv = IA.Server(db_rows, inst, d)
     srv.fill_synthetic(41)
-    srv.set_max_batch(32)
+    srv.set_max_batch(max(32, count))
     qs = _queries(db_rows, [100 + i for i in range(count)], small=False)
     qlen = len(qs[0][0])
     stride = (qlen + 16 + 255) // 256 * 256
