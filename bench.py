@@ -31,7 +31,7 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def make_queries(IA, torch, db_rows, count, seed, nq=2):
+def make_queries(IA, torch, db_rows, count, seed, nq=2, return_host=False):
     """Serialized synthetic keyword queries (uniform residues, as from a client
     with a real ternary key), staged back to back in one device buffer."""
     import numpy as np
@@ -57,7 +57,53 @@ def make_queries(IA, torch, db_rows, count, seed, nq=2):
     for i, bl in enumerate(blobs):
         host[i * stride:i * stride + qlen] = np.frombuffer(bl, dtype=np.uint8)
     dev = torch.from_numpy(host).cuda()
+    if return_host:
+        return dev, qlen, stride, blobs
     return dev, qlen, stride
+
+
+def serve_poisson(IA, eng, blobs, rate, n, seed):
+    """Open-loop Poisson arrivals at `rate` q/s through the continuous-batching
+    engine: a submitter thread sleeps to each arrival time and submits; a waiter
+    collects each query's submit-to-ready latency (measured in the library)."""
+    import queue
+    import threading
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    tickets = queue.Queue()
+    lat = []
+    b0 = eng.stats()
+
+    def waiter():
+        while True:
+            t = tickets.get()
+            if t is None:
+                return
+            lat.append(eng.wait(t)[1])
+
+    w = threading.Thread(target=waiter)
+    w.start()
+    dropped = 0
+    t0 = time.perf_counter()
+    for i in range(n):
+        dt = t0 + arrivals[i] - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        try:
+            tickets.put(eng.submit(blobs[i % len(blobs)]))
+        except IA.InspireError:
+            dropped += 1
+    tickets.put(None)
+    w.join()
+    elapsed = time.perf_counter() - t0
+    b1 = eng.stats()
+    lat.sort()
+    pct = lambda q: lat[min(len(lat) - 1, int(round(q * (len(lat) - 1))))]
+    nb = b1["batches"] - b0["batches"]
+    return {"offered_qps": rate, "achieved_qps": len(lat) / elapsed, "queries": len(lat), "dropped": dropped,
+            "p50_ms": pct(0.5), "p99_ms": pct(0.99), "mean_ms": sum(lat) / len(lat),
+            "mean_batch": (b1["queries"] - b0["queries"]) / max(nb, 1)}
 
 
 def cpu_baseline(plan, out_samples=32, fold_chains=2):
@@ -131,6 +177,9 @@ def main():
     ap.add_argument("--queries", type=int, default=8, help="distinct resident queries cycled through")
     ap.add_argument("--slots", type=int, default=2, help="requests in flight (request slots / streams)")
     ap.add_argument("--batches", default="1,8,32,64,128", help="batch sizes for the batched respond ('' to skip)")
+    ap.add_argument("--rates", default="100,500,1000,2000,3000",
+                    help="Poisson arrival rates (q/s) for the serving run ('' to skip)")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrivals per rate span about this long")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
@@ -278,6 +327,22 @@ def main():
             log("batch %d: %.2f ms/batch, %.1f q/s (%.1fx single)" % (B, tot / reps, qps_b, qps_b / qps))
             del bq, bout
     result["batched"] = batched
+
+    # Serving: continuous batching under open-loop Poisson arrivals.
+    serving = []
+    if args.rates:
+        _, _, _, sblobs = make_queries(IA, torch, db_rows, args.queries, 19, return_host=True)
+        eng = IA.Engine(srv, nq=2, max_batch=64, window_ms=0.0, capacity=2048)
+        log("engine max_batch %d" % eng.stats()["max_batch"])
+        for rate in [float(x) for x in args.rates.split(",") if x]:
+            n = int(min(3000, max(100, rate * args.serve_seconds)))
+            r = serve_poisson(IA, eng, sblobs, rate, n, 23)
+            serving.append(r)
+            log("serve %.0f q/s offered: %.1f q/s done, p50 %.2f ms, p99 %.2f ms, mean batch %.1f, dropped %d" %
+                (rate, r["achieved_qps"], r["p50_ms"], r["p99_ms"], r["mean_batch"], r["dropped"]))
+        eng.close()
+    result["serving"] = {"engine": "continuous batching, max_batch 64, no arrival window, second batch in flight when a full one waits",
+                         "runs": serving}
 
     srv.close()
     del outs, dq

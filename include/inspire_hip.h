@@ -243,6 +243,31 @@ int inspire_post_device(uint64_t* d_P, const uint32_t* d_S, int64_t count, void*
 int inspire_fold_device(uint8_t* d_out, const uint64_t* d_a_hat, const uint64_t* d_c, const uint64_t* d_P,
                         const uint32_t* d_bodies, int64_t instances, int d, int nq, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Continuous-batching engine (no counterpart in the reference, whose server
+ * loop answers one query per round trip, server.rs:116-158).  Callers submit
+ * serialized keyword queries (nq hash queries each) from any thread; a
+ * dispatcher packs the pending ones, up to max_batch, into one batched respond,
+ * waiting up to window_ms for more arrivals when fewer than max_batch are
+ * pending (0: dispatch at once).  A second batch goes in flight (its upload
+ * overlapping the running one) only when a full batch is pending.
+ * max_batch is capped at creation so the batch buffers fit in the free device
+ * memory (less 1 GiB); inspire_engine_stats reports the cap.  capacity bounds
+ * the submitted-but-not-waited queries (submit returns INSPIRE_ERR_CAPACITY
+ * past it).  Every ticket is waited exactly once; wait writes the serialized
+ * response (stash included) and the submit-to-ready latency.  The server must
+ * not be used by other callers while an engine runs on it.
+ * ---------------------------------------------------------------------- */
+typedef struct inspire_engine inspire_engine_t;
+int inspire_engine_create(inspire_server_t* srv, int nq, int max_batch, double window_ms, int capacity,
+                          inspire_engine_t** out);
+int inspire_engine_submit(inspire_engine_t* eng, const uint8_t* query, size_t len, uint64_t* ticket);
+int inspire_engine_wait(inspire_engine_t* eng, uint64_t ticket, uint8_t* resp, size_t cap, size_t* resp_len,
+                        double* latency_ms);
+int inspire_engine_stats(inspire_engine_t* eng, int64_t* batches, int64_t* queries, int* max_batch);
+/* Finishes the queued and running batches, then frees the engine. */
+void inspire_engine_destroy(inspire_engine_t* eng);
+
 #ifdef __cplusplus
 }
 #endif

@@ -147,108 +147,120 @@ __global__ __launch_bounds__(256) void k_gemv_mfma(uint64_t* __restrict__ c, con
 }
 
 // ---------------------------------------------------------------------------
-// Batched form: MQ keyword queries (2 hash queries each, one 16-row M-tile per
-// keyword query) share every DB fragment.  Block = 4 waves x 16 columns; the
-// query fragments of a 256-row K chunk are staged in LDS for all MQ queries.
-// afrag: [client][db_rows/64][64][16]; rsum: [client][16].
+// Batched form: a block covers 16*TW DB columns for 4*CPW keyword queries
+// (2 hash queries each, one 16-row M-tile per keyword query); wave w owns CPW
+// of the queries and all TW column tiles.  The block stages each step's DB fragments once in LDS as the two
+// signed byte planes (lo, hi) that every wave reads; each wave loads its own
+// query fragments straight from L2.  Both streams run D steps ahead through
+// register rings with clamped (unconditional) loads.  Query groups of one
+// column block are consecutive on one XCD (block id mod 8), so their DB reads
+// meet in L2 and the DB leaves HBM about once per batch.
 // ---------------------------------------------------------------------------
 
-constexpr int kBatchChunkSteps = 4;  // 256 rows per LDS chunk
+template <int CPW, int TW, int D>
+__global__ __launch_bounds__(256) void k_gemv_wide(uint64_t* __restrict__ c, int64_t c_client_stride,
+                                                   const int8_t* __restrict__ afrag, const int32_t* __restrict__ rsum,
+                                                   const uint16_t* __restrict__ db, int nq, int db_rows,
+                                                   int64_t db_cols, int count, int ngroups, int64_t col_blocks) {
+  static_assert(TW * 64 == 256, "one DB staging item per thread");
+  __shared__ v4i planes[2][TW][2][64];
+  const int64_t bid = blockIdx.x;
+  const int xcd = (int)(bid % 8);
+  const int64_t rest = bid / 8;
+  const int grp = (int)(rest % ngroups);
+  const int64_t cb = (rest / ngroups) * 8 + xcd;
+  if (cb >= col_blocks) return;  // whole block: the grid rounds column blocks up to a multiple of 8
+  const int64_t col0 = cb * (16 * TW);
 
-// Block = 4 waves; wave w owns TW column tiles (16 columns each), so every
-// query fragment read from LDS feeds 2*TW MFMAs and every DB fragment feeds
-// 2*MQ MFMAs.  The DB fragments of a chunk are requested before the chunk's
-// query fragments are staged, so the two overlap.
-template <int MQ, int TW>
-__global__ __launch_bounds__(256) void k_gemv_mfma_batch(uint64_t* __restrict__ c, int64_t c_client_stride,
-                                                         const int8_t* __restrict__ afrag,
-                                                         const int32_t* __restrict__ rsum,
-                                                         const uint16_t* __restrict__ db, int nq, int db_rows,
-                                                         int64_t db_cols, int client0, int count) {
-  __shared__ v4i a_lds[MQ][kBatchChunkSteps][64];  // MQ x 4 KiB
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int j = lane & 15, g = lane >> 4;
-  const int64_t col0 = (int64_t)blockIdx.x * (64 * TW) + wave * (16 * TW);
-  const uint16_t* colp[TW];
-  bool valid[TW];
-#pragma unroll
-  for (int tt = 0; tt < TW; ++tt) {
-    const int64_t col = col0 + 16 * tt + j;
-    valid[tt] = col < db_cols;
-    colp[tt] = db + (valid[tt] ? col : 0) * (int64_t)db_rows + 8 * g;
-  }
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int steps = db_rows / 64;
-  const int nclients = min(MQ, count - client0);
+  // DB staging item: tile st = tid / 64, lane slot (column j, row group g) = tid % 64.
+  const int st = tid >> 6, sj = lane & 15, sg = lane >> 4;
+  const int64_t scol = min(col0 + 16 * st + sj, db_cols - 1);
+  const uint16_t* dbp = db + scol * (int64_t)db_rows + 8 * sg;
+  // Query fragments of this wave's clients (clamped to the last real one).
+  const int client0 = grp * 4 * CPW + wave * CPW;
+  const u4* ap[CPW];
+#pragma unroll
+  for (int q = 0; q < CPW; ++q)
+    ap[q] = reinterpret_cast<const u4*>(afrag) + (int64_t)min(client0 + q, count - 1) * steps * 64 + lane;
 
-  v4i acc_lo[MQ][TW], acc_hi[MQ][TW];
+  auto load_db = [&](int s, u4 (&d)[2]) {
+    const uint16_t* p = dbp + 64 * s;
+    d[0] = __builtin_nontemporal_load(reinterpret_cast<const u4*>(p));
+    d[1] = __builtin_nontemporal_load(reinterpret_cast<const u4*>(p + 32));
+  };
+  auto stage_db = [&](int buf, const u4 (&d)[2]) {
+    planes[buf][st][0][lane] = v4i{(int)plane_lo(d[0].x, d[0].y), (int)plane_lo(d[0].z, d[0].w),
+                                   (int)plane_lo(d[1].x, d[1].y), (int)plane_lo(d[1].z, d[1].w)};
+    planes[buf][st][1][lane] = v4i{(int)plane_hi(d[0].x, d[0].y), (int)plane_hi(d[0].z, d[0].w),
+                                   (int)plane_hi(d[1].x, d[1].y), (int)plane_hi(d[1].z, d[1].w)};
+  };
+  auto load_a = [&](int s, u4 (&a)[CPW]) {
 #pragma unroll
-  for (int q = 0; q < MQ; ++q)
+    for (int q = 0; q < CPW; ++q) a[q] = ap[q][(int64_t)s * 64];
+  };
+
+  v4i acc[CPW][TW][2];
 #pragma unroll
-    for (int tt = 0; tt < TW; ++tt) {
-      acc_lo[q][tt] = v4i{0, 0, 0, 0};
-      acc_hi[q][tt] = v4i{0, 0, 0, 0};
-    }
-  for (int s0 = 0; s0 < steps; s0 += kBatchChunkSteps) {
-    u4 dd[kBatchChunkSteps][TW][2];
+  for (int q = 0; q < CPW; ++q)
 #pragma unroll
-    for (int ss = 0; ss < kBatchChunkSteps; ++ss)
+    for (int t = 0; t < TW; ++t) acc[q][t][0] = acc[q][t][1] = v4i{0, 0, 0, 0};
+
+  // Rings: DB words of step t in dreg[t % D], query fragments in areg[t % D].
+  u4 dreg[D][2];
+  u4 areg[D][CPW];
+  load_db(0, dreg[0]);
+  stage_db(0, dreg[0]);
 #pragma unroll
-      for (int tt = 0; tt < TW; ++tt) {
-        const uint16_t* p = colp[tt] + 64 * (s0 + ss);
-        dd[ss][tt][0] = valid[tt] ? __builtin_nontemporal_load(reinterpret_cast<const u4*>(p)) : u4{0, 0, 0, 0};
-        dd[ss][tt][1] = valid[tt] ? __builtin_nontemporal_load(reinterpret_cast<const u4*>(p + 32)) : u4{0, 0, 0, 0};
-      }
-    __syncthreads();
-    for (int e = threadIdx.x; e < MQ * kBatchChunkSteps * 64; e += 256) {
-      const int q = e / (kBatchChunkSteps * 64), r = e % (kBatchChunkSteps * 64);
-      v4i v = {0, 0, 0, 0};
-      if (q < nclients) v = reinterpret_cast<const v4i*>(afrag)[((int64_t)(client0 + q) * steps + s0) * 64 + r];
-      a_lds[q][r / 64][r % 64] = v;
-    }
-    __syncthreads();
+  for (int t = 0; t < D; ++t) {
+    load_db(min(t + 1, steps - 1), dreg[(t + 1) % D]);
+    load_a(min(t, steps - 1), areg[t]);
+  }
+  __syncthreads();
+
+  for (int s0 = 0; s0 < steps; s0 += D) {
 #pragma unroll
-    for (int ss = 0; ss < kBatchChunkSteps; ++ss) {
-      v4i lo[TW], hi[TW];
+    for (int d = 0; d < D; ++d) {
+      const int s = s0 + d;
+      stage_db((d + 1) & 1, dreg[(d + 1) % D]);  // DB of step s+1 (unused past the end)
+      load_db(min(s + 1 + D, steps - 1), dreg[(d + 1) % D]);
 #pragma unroll
-      for (int tt = 0; tt < TW; ++tt) {
-        const u4 d0 = dd[ss][tt][0], d1 = dd[ss][tt][1];
-        lo[tt] = v4i{(int)plane_lo(d0.x, d0.y), (int)plane_lo(d0.z, d0.w), (int)plane_lo(d1.x, d1.y),
-                     (int)plane_lo(d1.z, d1.w)};
-        hi[tt] = v4i{(int)plane_hi(d0.x, d0.y), (int)plane_hi(d0.z, d0.w), (int)plane_hi(d1.x, d1.y),
-                     (int)plane_hi(d1.z, d1.w)};
-      }
+      for (int t = 0; t < TW; ++t) {
+        const v4i lo = planes[d & 1][t][0][lane], hi = planes[d & 1][t][1][lane];
 #pragma unroll
-      for (int q = 0; q < MQ; ++q) {
-        const v4i a = a_lds[q][ss][lane];
-#pragma unroll
-        for (int tt = 0; tt < TW; ++tt) {
-          acc_lo[q][tt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, lo[tt], acc_lo[q][tt], 0, 0, 0);
-          acc_hi[q][tt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, hi[tt], acc_hi[q][tt], 0, 0, 0);
+        for (int q = 0; q < CPW; ++q) {
+          const v4i a = __builtin_bit_cast(v4i, areg[d][q]);
+          acc[q][t][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, lo, acc[q][t][0], 0, 0, 0);
+          acc[q][t][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, hi, acc[q][t][1], 0, 0, 0);
         }
       }
+      load_a(min(s + D, steps - 1), areg[d]);
+      __syncthreads();
+      __builtin_amdgcn_sched_barrier(0);  // keep each step's loads in its own step
     }
   }
+
+  // Lane (j, G) holds rows 4G..4G+3 = limbs 0..3 of (h, p) = (G >> 1, G & 1).
+  const int j = lane & 15, g = lane >> 4;
   const int h = g >> 1, pr = g & 1;
 #pragma unroll
-  for (int q = 0; q < MQ; ++q) {
+  for (int q = 0; q < CPW; ++q) {
     const int client = client0 + q;
+    const int rc = min(client, count - 1);
+    int64_t rs[4];
 #pragma unroll
-    for (int tt = 0; tt < TW; ++tt) {
+    for (int r = 0; r < 4; ++r) rs[r] = (int64_t)rsum[rc * 16 + 4 * g + r] * 32896;
+#pragma unroll
+    for (int t = 0; t < TW; ++t) {
       int64_t v = 0;
-      if (client < count) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-          v += ((int64_t)256 * acc_hi[q][tt][r] + (int64_t)acc_lo[q][tt][r] +
-                (int64_t)rsum[client * 16 + 4 * g + r] * 32896)
-               << (7 * r);
-      }
+      for (int r = 0; r < 4; ++r) v += ((int64_t)256 * acc[q][t][1][r] + (int64_t)acc[q][t][0][r] + rs[r]) << (7 * r);
       const uint64_t red = pr ? barrett64((uint64_t)v, kCr1Q1, kQ1) : barrett64((uint64_t)v, kCr1Q0, kQ0);
       const uint64_t other = __shfl_down(red, 16, 64);
-      const int64_t col = col0 + 16 * tt + j;
-      if (pr == 0 && h < nq && valid[tt] && client < count)
-        c[(int64_t)client * c_client_stride + (int64_t)h * db_cols + col] =
-            crt_compose((uint32_t)red, (uint32_t)other);
+      const int64_t col = col0 + 16 * t + j;
+      if (pr == 0 && h < nq && col < db_cols && client < count)
+        c[(int64_t)client * c_client_stride + (int64_t)h * db_cols + col] = crt_compose((uint32_t)red, (uint32_t)other);
     }
   }
 }
@@ -279,15 +291,14 @@ int launch_gemv_mfma_batch(uint64_t* c, int64_t c_client_stride, const uint64_t*
   const int prep_threads = (db_rows / 64) * 64;
   hipLaunchKernelGGL(k_gemv_prep_batch, dim3((prep_threads + 255) / 256, count), dim3(256), 0, s, rows,
                      rows_client_stride, nq, db_rows, count, afrag, rsum);
-  for (int c0 = 0; c0 < count; c0 += 8) {
-    const int left = count - c0;
-    if (left > 4)
-      hipLaunchKernelGGL((k_gemv_mfma_batch<8, 2>), dim3((unsigned)((db_cols + 127) / 128)), dim3(256), 0, s, c,
-                         c_client_stride, afrag, rsum, db, nq, db_rows, db_cols, c0, count);
-    else
-      hipLaunchKernelGGL((k_gemv_mfma_batch<4, 2>), dim3((unsigned)((db_cols + 127) / 128)), dim3(256), 0, s, c,
-                         c_client_stride, afrag, rsum, db, nq, db_rows, db_cols, c0, count);
-  }
+  // 16 queries per column-block group: 4 waves x 4 queries x 4 column tiles,
+  // a 2-step ring (two blocks per CU).
+  constexpr int kCpw = 4, kTw = 4;
+  const int ngroups = (count + 4 * kCpw - 1) / (4 * kCpw);
+  const int64_t col_blocks = (db_cols + 16 * kTw - 1) / (16 * kTw);
+  const int64_t blocks = (col_blocks + 7) / 8 * 8 * ngroups;
+  hipLaunchKernelGGL((k_gemv_wide<kCpw, kTw, 2>), dim3((unsigned)blocks), dim3(256), 0, s, c, c_client_stride, afrag,
+                     rsum, db, nq, db_rows, db_cols, count, ngroups, col_blocks);
   return 0;
 }
 

@@ -391,12 +391,15 @@ __global__ __launch_bounds__(64 * WM * WN) void k_packing_mfma(PackMfmaArgs a) {
 
 // ---------------------------------------------------------------------------
 // Wide batched GEMM: one block holds all 4*MT clients of its group, so each
-// B fragment leaves HBM once per group, and the block's four waves split the NT
-// N-tiles (NTW each, the last wave possibly fewer).  B fragments go straight
-// from HBM into the wave's registers; the A fragments (built from the clients'
-// keys) are shared through LDS.  Both run D steps ahead of the matrix cores
-// through register rings, and every load is unconditional (clamped addresses),
-// so the only waits in the loop are the ones on the data a step consumes.
+// B fragment leaves HBM once per group.  The block's waves form WM rows x 4
+// columns: row wm owns M-tiles [wm*MT/WM, (wm+1)*MT/WM), column wn owns N-tiles
+// [wn*NTW, (wn+1)*NTW) of the group's NT (the last column possibly fewer).
+// B fragments go straight from HBM into the wave's registers; the A fragments
+// (built from the clients' keys by every thread of the block) are shared
+// through LDS.  Both run D steps ahead of the matrix cores through register
+// rings, and every load is unconditional (clamped addresses), so the only
+// waits in the loop are the ones on the data a step consumes.  With WM = 2 two
+// waves share each SIMD and cover each other's LDS and barrier waits.
 //
 // Keys are stored as limb bytes, keys[p][row][bpad] with row = plane*N + slot
 // (planes 0-2: y rows, 3-5: z_body rows) and a final zero row; byte l of a key
@@ -440,13 +443,38 @@ struct PackWideArgs {
   int64_t s_client_stride;
 };
 
-template <int MT, int NTW, int NT, int D>
-__global__ __launch_bounds__(256) void k_packing_wide(PackWideArgs a) {
-  static_assert(16 * MT <= 256 && (16 * MT) % 64 == 0, "A items must fill whole waves");
-  static_assert(NTW * 4 >= NT && NTW * 3 < NT, "four waves cover the N-tiles");
+// Key vector of CPT consecutive clients.
+template <int CPT>
+struct KeyVec;
+template <>
+struct KeyVec<4> {
+  typedef unsigned int type __attribute__((ext_vector_type(4)));
+  __device__ static uint32_t get(type v, int c) { return v[c]; }
+};
+template <>
+struct KeyVec<2> {
+  typedef unsigned int type __attribute__((ext_vector_type(2)));
+  __device__ static uint32_t get(type v, int c) { return v[c]; }
+};
+template <>
+struct KeyVec<1> {
+  typedef unsigned int type;
+  __device__ static uint32_t get(type v, int) { return v; }
+};
+
+template <int MT, int NTW, int NT, int D, int WM>
+__global__ __launch_bounds__(256 * WM) void k_packing_wide(PackWideArgs a) {
+  constexpr int kThreads = 256 * WM;
+  constexpr int kParts = kThreads / (16 * MT);  // threads sharing one (client quad, term quad) item
+  constexpr int kCpt = 4 / kParts;              // clients per thread
+  constexpr int MTW = MT / WM;
+  static_assert(kParts * 16 * MT == kThreads && (kParts == 1 || kParts == 2 || kParts == 4),
+                "the block's threads split the A items evenly");
+  static_assert(MT % WM == 0, "M-tiles split evenly over wave rows");
+  static_assert(NTW * 4 >= NT && NTW * 3 < NT, "four wave columns cover the N-tiles");
   static_assert(kKSteps % D == 0 && D % 2 == 0, "ring depth must divide the steps and be even");
-  constexpr int kAWaves = 16 * MT / 64;
-  constexpr int kTileStride = 65;  // u4 per A tile in LDS: one pad row keeps the limb writes conflict-free
+  typedef typename KeyVec<kCpt>::type KV;
+  constexpr int kTileStride = 65;  // u4 per A tile in LDS: one pad row spreads the limb writes over the banks
   __shared__ uint16_t tsrc[kKPad];
   __shared__ u4 afrag[2][MT * kTileStride];
 
@@ -464,17 +492,19 @@ __global__ __launch_bounds__(256) void k_packing_wide(PackWideArgs a) {
 
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const bool a_wave = wave < kAWaves;
+  const int wm = wave / 4, wn = wave % 4;
 
   {
     const u4* src = reinterpret_cast<const u4*>(a.term_src + (size_t)z * kKPad);
-    for (int e = tid; e < kKPad / 8; e += 256) reinterpret_cast<u4*>(tsrc)[e] = src[e];
+    for (int e = tid; e < kKPad / 8; e += kThreads) reinterpret_cast<u4*>(tsrc)[e] = src[e];
   }
 
-  // A item of this thread: client quad cq, terms 4*tq4 .. 4*tq4+3 of a step.
-  const int cq = tid % MT, tq4 = (tid / MT) & 15;
+  // A item part of this thread: clients part*kCpt.. of client quad cq, terms
+  // 4*tq4 .. 4*tq4+3 of a step.  Consecutive threads read consecutive clients.
+  const int part = tid % kParts, cq = (tid / kParts) % MT, tq4 = tid / (kParts * MT);
   const int g = tq4 >> 2, tq = tq4 & 3;
-  const uint32_t* keys_p = a.keys + (size_t)p * kKeyRows * a.bpad + 4 * (mg * MT + cq);
+  const uint8_t* keys_p = reinterpret_cast<const uint8_t*>(a.keys + (size_t)p * kKeyRows * a.bpad +
+                                                           4 * (mg * MT + cq) + part * kCpt);
   const uint32_t row_bytes = (uint32_t)a.bpad * 4u;
 
   // B tiles of this wave (clamped: tiles past the group or the matrix repeat
@@ -484,24 +514,25 @@ __global__ __launch_bounds__(256) void k_packing_wide(PackWideArgs a) {
   const uint8_t* bbase[NTW];
 #pragma unroll
   for (int y = 0; y < NTW; ++y) {
-    const int ntl = min(wave * NTW + y, NT - 1);
+    const int ntl = min(wn * NTW + y, NT - 1);
     const int64_t nt = min((int64_t)(nt0 + ntl), a.ntiles - 1);
     bbase[y] = tb + (size_t)nt * kKSteps * 1024 + lane * 16;
   }
 
-  auto gather = [&](int s, u4 (&v)[4]) {
+  auto gather = [&](int s, KV (&v)[4]) {
     const uint2 ts = *reinterpret_cast<const uint2*>(&tsrc[64 * s + 4 * tq4]);
     const uint32_t r[4] = {ts.x & 0xffffu, ts.x >> 16, ts.y & 0xffffu, ts.y >> 16};
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
-      v[u] = *reinterpret_cast<const u4*>(reinterpret_cast<const uint8_t*>(keys_p) + r[u] * row_bytes);
+    for (int u = 0; u < 4; ++u) v[u] = *reinterpret_cast<const KV*>(keys_p + r[u] * row_bytes);
   };
   // 4x4 byte transposes: dword (client bl, limb l) gets limb l of the four terms.
-  auto transform = [&](int buf, const u4 (&v)[4]) {
+  auto transform = [&](int buf, const KV (&v)[4]) {
     uint32_t* base = reinterpret_cast<uint32_t*>(&afrag[buf][cq * kTileStride + 16 * g]) + tq;
 #pragma unroll
-    for (int bl = 0; bl < 4; ++bl) {
-      const uint32_t x0 = v[0][bl], x1 = v[1][bl], x2 = v[2][bl], x3 = v[3][bl];
+    for (int c = 0; c < kCpt; ++c) {
+      const int bl = part * kCpt + c;
+      const uint32_t x0 = KeyVec<kCpt>::get(v[0], c), x1 = KeyVec<kCpt>::get(v[1], c);
+      const uint32_t x2 = KeyVec<kCpt>::get(v[2], c), x3 = KeyVec<kCpt>::get(v[3], c);
       const uint32_t t0 = __builtin_amdgcn_perm(x1, x0, 0x05010400u);
       const uint32_t t1 = __builtin_amdgcn_perm(x1, x0, 0x07030602u);
       const uint32_t t2 = __builtin_amdgcn_perm(x3, x2, 0x05010400u);
@@ -518,23 +549,21 @@ __global__ __launch_bounds__(256) void k_packing_wide(PackWideArgs a) {
       regs[y] = __builtin_nontemporal_load(reinterpret_cast<const u4*>(bbase[y] + (size_t)s * 1024));
   };
 
-  v4i acc[MT][NTW];
+  v4i acc[MTW][NTW];
 #pragma unroll
-  for (int x = 0; x < MT; ++x)
+  for (int x = 0; x < MTW; ++x)
 #pragma unroll
     for (int y = 0; y < NTW; ++y) acc[x][y] = v4i{0, 0, 0, 0};
 
   // Rings: A keys of step t in araw[t % D], B fragments of step t in breg[t % D].
-  u4 araw[D][4];
+  KV araw[D][4];
   u4 breg[D][NTW];
   __syncthreads();
-  if (a_wave) {
-    gather(0, araw[0]);
-    transform(0, araw[0]);
-  }
+  gather(0, araw[0]);
+  transform(0, araw[0]);
 #pragma unroll
   for (int t = 0; t < D; ++t) {
-    if (a_wave) gather(t + 1, araw[(t + 1) % D]);
+    gather(t + 1, araw[(t + 1) % D]);
     load_b(t, breg[t]);
   }
   __syncthreads();
@@ -543,13 +572,11 @@ __global__ __launch_bounds__(256) void k_packing_wide(PackWideArgs a) {
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       const int s = s0 + d;
-      if (a_wave) {
-        transform((d + 1) & 1, araw[(d + 1) % D]);             // A of step s+1 (unused past the end)
-        gather(min(s + 1 + D, kKSteps - 1), araw[(d + 1) % D]);
-      }
-      const u4* at = afrag[d & 1];
+      transform((d + 1) & 1, araw[(d + 1) % D]);  // A of step s+1 (unused past the end)
+      gather(min(s + 1 + D, kKSteps - 1), araw[(d + 1) % D]);
+      const u4* at = afrag[d & 1] + wm * MTW * kTileStride;
 #pragma unroll
-      for (int x = 0; x < MT; ++x) {
+      for (int x = 0; x < MTW; ++x) {
         const v4i af = __builtin_bit_cast(v4i, at[x * kTileStride + lane]);
 #pragma unroll
         for (int y = 0; y < NTW; ++y)
@@ -567,7 +594,7 @@ __global__ __launch_bounds__(256) void k_packing_wide(PackWideArgs a) {
   const uint32_t q = prime(p);
   const uint32_t pw = (uint32_t)(((uint64_t)1 << (7 * m)) % q);
 #pragma unroll
-  for (int x = 0; x < MT; ++x) {
+  for (int x = 0; x < MTW; ++x) {
 #pragma unroll
     for (int y = 0; y < NTW; ++y) {
       uint64_t X = 0;
@@ -579,9 +606,9 @@ __global__ __launch_bounds__(256) void k_packing_wide(PackWideArgs a) {
       tot = (tot >= q) ? tot - q : tot;
       uint32_t tot2 = tot + __shfl_xor(tot, 2, 64);
       tot2 = (tot2 >= q) ? tot2 - q : tot2;
-      const int ntl = wave * NTW + y;
+      const int ntl = wn * NTW + y;
       const int64_t j = 4 * (int64_t)(nt0 + ntl) + jl;
-      const int b = 4 * (mg * MT + x) + G;
+      const int b = 4 * (mg * MT + wm * MTW + x) + G;
       if (m == 0 && ntl < NT && b < a.count && j < a.instances)
         a.S[(size_t)b * a.s_client_stride + ((size_t)j * 2 + p) * kN + z] = tot2;
     }
@@ -609,12 +636,12 @@ size_t batch_keys_bytes(int bpad) { return (size_t)2 * kTExp * kN * bpad * sizeo
 // Key buffer for up to `count` clients, large enough for either key layout.
 size_t packing_keys_bytes(int count) { return (size_t)2 * kKeyRows * batch_key_pad(count) * sizeof(uint32_t); }
 
-template <int MT, int NTW, int NT, int D>
+template <int MT, int NTW, int NT, int D, int WM>
 static void launch_wide(PackWideArgs a, hipStream_t s) {
   a.ngroups = (int)((a.ntiles + NT - 1) / NT);
   a.mgroups = a.bpad / (4 * MT);
   const int64_t blocks = (int64_t)kN * 2 * a.ngroups * a.mgroups;
-  hipLaunchKernelGGL((k_packing_wide<MT, NTW, NT, D>), dim3((unsigned)blocks), dim3(256), 0, s, a);
+  hipLaunchKernelGGL((k_packing_wide<MT, NTW, NT, D, WM>), dim3((unsigned)blocks), dim3(256 * WM), 0, s, a);
 }
 
 template <int MT, int NT, int WM, int WN>
@@ -636,9 +663,9 @@ int launch_packing_mfma(uint32_t* S, int64_t s_client_stride, const uint8_t* til
                        dim3(256), 0, s, keys, y_body, z_body, key_client_stride, count, bpad);
     PackWideArgs a{S, tiled, keys, term_src, instances, tiled_ntiles(instances), bpad, count, 0, 0, s_client_stride};
     if (bpad == 32)
-      launch_wide<8, 6, 22, 4>(a, s);   // 32 clients: waves of 8 x 6 tiles over 22 N-tiles
+      launch_wide<8, 6, 22, 4, 1>(a, s);   // 32 clients: 4 waves of 8 x 6 tiles over 22 N-tiles
     else
-      launch_wide<16, 3, 11, 4>(a, s);  // 64-client groups: waves of 16 x 3 tiles over 11 N-tiles
+      launch_wide<16, 4, 16, 4, 2>(a, s);  // 64-client groups: 8 waves of 8 x 4 tiles over 16 N-tiles
     return 0;
   }
   uint32_t* ykeys = keys;

@@ -23,3 +23,24 @@ struct DeviceConstants {
 const DeviceConstants* device_constants();
 
 }  // namespace inspire
+
+struct inspire_server;
+
+namespace inspire {
+
+// Server facts the serving engine needs (engine.cpp).
+struct ServerInfo {
+  int device;          // HIP device the server's state lives on
+  size_t query_bytes;  // serialized keyword query with nq hash queries
+  size_t blob;         // response bytes per keyword query on device (nq * c * 12288)
+  size_t response;     // serialized response bytes, stash included
+  int batch_max;       // current batch capacity (0: none allocated)
+};
+int server_info(const inspire_server* s, int nq, ServerInfo* out);
+// Device bytes the batch buffers for `max` keyword queries take.
+size_t server_batch_bytes(const inspire_server* s, int max);
+// serialize_keyword_response (wire.rs:708-726) of nq device blobs (host copy)
+// plus the stash; returns the bytes written.
+size_t server_write_response(const inspire_server* s, int nq, const uint8_t* blobs, uint8_t* resp);
+
+}  // namespace inspire

@@ -698,7 +698,15 @@ extern "C" int inspire_respond_serialized(inspire_server_t* s, const uint8_t* qu
             "download responses");
     HIP_TRY(hipStreamSynchronize(s->stream), "respond");
   }
+  server_write_response(s, nq, s->host_out.data(), resp);
+  return INSPIRE_OK;
+}
+
+namespace inspire {
+
+size_t server_write_response(const inspire_server* s, int nq, const uint8_t* blobs, uint8_t* resp) {
   // serialize_keyword_response (wire.rs:708-726)
+  const size_t blob = (size_t)s->c * kRespBytes;
   size_t o = 0;
   auto put32 = [&](uint32_t v) {
     std::memcpy(resp + o, &v, 4);
@@ -707,7 +715,7 @@ extern "C" int inspire_respond_serialized(inspire_server_t* s, const uint8_t* qu
   put32((uint32_t)nq);
   for (int h = 0; h < nq; ++h) {
     put32((uint32_t)blob);
-    std::memcpy(resp + o, s->host_out.data() + (size_t)h * blob, blob);
+    std::memcpy(resp + o, blobs + (size_t)h * blob, blob);
     o += blob;
   }
   put32(s->stash_count);
@@ -716,8 +724,34 @@ extern "C" int inspire_respond_serialized(inspire_server_t* s, const uint8_t* qu
     std::memcpy(resp + o, s->stash.data() + (size_t)i * s->stash_entry, s->stash_entry);
     o += s->stash_entry;
   }
+  return o;
+}
+
+int server_info(const inspire_server* s, int nq, ServerInfo* out) {
+  if (!s || !out || nq < 1 || nq > s->shape.max_queries) return set_error(INSPIRE_ERR_INVALID, "bad server info request");
+  out->device = s->device;
+  out->query_bytes = query_bytes(s->shape.db_rows, nq);
+  out->blob = (size_t)nq * s->c * kRespBytes;
+  out->response = inspire_server_response_bytes(s, nq);
+  out->batch_max = s->batch.max;
   return INSPIRE_OK;
 }
+
+size_t server_batch_bytes(const inspire_server* s, int max) {
+  const size_t inst = s->shape.instances, rows = s->shape.db_rows, mq = s->shape.max_queries;
+  size_t b = 0;
+  b += 2 * (size_t)max * kTExp * kN * 8;        // y_body, z_body
+  b += (size_t)max * mq * rows * 8;             // query rows
+  b += (size_t)max * mq * 6 * 2 * kN * 4;       // RGSW bodies
+  b += (size_t)max * inst * 2 * kN * 4;         // packing output
+  b += (size_t)max * inst * kN * 8;             // post output
+  b += (size_t)max * mq * s->db_cols * 8;       // GEMV output
+  b += packing_keys_bytes(max);
+  b += gemv_mfma_batch_scratch_bytes((int)rows, max);
+  return b;
+}
+
+}  // namespace inspire
 
 // ---------------------------------------------------------------------------
 // Stage entry points

@@ -87,6 +87,12 @@ def lib():
         L.inspire_packing_device.argtypes = [vp, vp, vp, vp, vp, vp, i64, vp]
         L.inspire_post_device.argtypes = [vp, vp, i64, vp]
         L.inspire_fold_device.argtypes = [vp, vp, vp, vp, vp, i64, i32, i32, vp]
+        L.inspire_engine_create.argtypes = [vp, i32, i32, ctypes.c_double, i32, ctypes.POINTER(vp)]
+        L.inspire_engine_submit.argtypes = [vp, vp, sz, ctypes.POINTER(u64)]
+        L.inspire_engine_wait.argtypes = [vp, u64, vp, sz, ctypes.POINTER(sz), ctypes.POINTER(ctypes.c_double)]
+        L.inspire_engine_stats.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i32)]
+        L.inspire_engine_destroy.argtypes = [vp]
+        L.inspire_engine_destroy.restype = None
         L.gpu_available_memory_bytes.restype = u64
         L.gpu_packing_online_get_d_y_all.restype = vp
         L.gpu_packing_online_get_d_y_bar_all.restype = vp
@@ -161,6 +167,10 @@ class Server:
     def stream(self):
         return lib().inspire_server_stream(self.h)
 
+    def response_bytes(self, nq):
+        """Serialized response size for a query with nq hash queries (stash included)."""
+        return lib().inspire_server_response_bytes(self.h, nq)
+
     def respond_serialized(self, query):
         """respond_to_serialized_query (keyword.rs:636-640): bytes in, bytes out."""
         nq = query[0] if query else 0
@@ -200,6 +210,46 @@ class Server:
 
     def respond_device(self, d_query, length, nq, d_out, stream=None):
         check(lib().inspire_respond_device(self.h, _ptr(d_query), length, nq, _ptr(d_out), stream))
+
+
+class Engine:
+    """Continuous-batching front end over a Server (inspire_engine_* in the C ABI).
+
+    submit() may be called from any thread and returns a ticket; wait(ticket)
+    blocks until that query's batch is done and returns (response bytes,
+    submit-to-ready latency in ms).  Every ticket is waited exactly once."""
+
+    def __init__(self, server, nq=2, max_batch=64, window_ms=0.0, capacity=1024):
+        self.server = server
+        h = ctypes.c_void_p()
+        check(lib().inspire_engine_create(server.h, nq, max_batch, float(window_ms), capacity, ctypes.byref(h)))
+        self.h = h
+        self.nq = nq
+        self._resp_cap = server.response_bytes(nq)
+
+    def submit(self, query):
+        q = bytes(query)
+        t = ctypes.c_uint64()
+        check(lib().inspire_engine_submit(self.h, q, len(q), ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket):
+        cap = self.server.response_bytes(self.nq)
+        buf = ctypes.create_string_buffer(cap)
+        n = ctypes.c_size_t()
+        lat = ctypes.c_double()
+        check(lib().inspire_engine_wait(self.h, ticket, buf, cap, ctypes.byref(n), ctypes.byref(lat)))
+        return buf.raw[:n.value], lat.value
+
+    def stats(self):
+        b, q, m = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
+        check(lib().inspire_engine_stats(self.h, ctypes.byref(b), ctypes.byref(q), ctypes.byref(m)))
+        return {"batches": b.value, "queries": q.value, "max_batch": m.value}
+
+    def close(self):
+        if self.h:
+            lib().inspire_engine_destroy(self.h)
+            self.h = None
 
 
 def synth_fill(d_out, seed, stream_id, count, kind, start=0, stream=None):

@@ -373,3 +373,61 @@ def test_batch_results_equal_solo_results(torch, count, inst, d):
     exp = O.respond_raw(st[0], db_rows, inst, d, st[1], st[2], st[3], st[4], y, z, rows, bodies)
     assert out[:blob] == b"".join(exp)
     srv.close()
+
+
+# ---------------------------------------------------------------------------
+# Continuous-batching engine
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("window_ms,max_batch", [(5.0, 8), (0.0, 32)])
+def test_engine_responses_equal_solo_responses(torch, window_ms, max_batch):
+    say("engine vs solo", window_ms, max_batch)
+    import threading
+    db_rows, inst, d = 2048, 8, 4
+    srv = IA.Server(db_rows, inst, d)
+    srv.fill_synthetic(51)
+    srv.set_stash([bytes([i]) * 16 for i in range(3)])
+    qs = [q for q, _ in _queries(db_rows, [200 + i for i in range(24)])]
+    solo = [srv.respond_serialized(q) for q in qs]
+    eng = IA.Engine(srv, nq=2, max_batch=max_batch, window_ms=window_ms, capacity=64)
+    tickets = [None] * len(qs)
+
+    def submit(lo, hi):
+        for i in range(lo, hi):
+            tickets[i] = eng.submit(qs[i])
+
+    th = [threading.Thread(target=submit, args=(k * 12, k * 12 + 12)) for k in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for i, t in enumerate(tickets):
+        resp, lat = eng.wait(t)
+        assert resp == solo[i], i
+        assert lat > 0
+    st = eng.stats()
+    assert st["queries"] == len(qs) and 1 <= st["batches"] <= len(qs)
+    assert st["max_batch"] == max_batch
+    eng.close()
+    srv.close()
+
+
+def test_engine_rejects_bad_queries_and_full_queue(torch):
+    say("engine errors")
+    db_rows, inst, d = 2048, 8, 4
+    srv = IA.Server(db_rows, inst, d)
+    srv.fill_synthetic(52)
+    q = _queries(db_rows, [300])[0][0]
+    eng = IA.Engine(srv, nq=2, max_batch=4, window_ms=50.0, capacity=2)
+    with pytest.raises(IA.InspireError):
+        eng.submit(q[:100])                      # truncated
+    with pytest.raises(IA.InspireError):
+        eng.submit(bytes([1]) + q[1:])           # hash-query count differs
+    t1, t2 = eng.submit(q), eng.submit(q)
+    with pytest.raises(IA.InspireError):
+        eng.submit(q)                            # capacity 2
+    assert eng.wait(t1)[0] == eng.wait(t2)[0]
+    with pytest.raises(IA.InspireError):
+        eng.wait(t1)                             # already waited
+    eng.close()
+    srv.close()
