@@ -23,6 +23,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "inspire-rs_amd"))
 
 METRIC = "PIR queries/sec + p50/p99 respond latency, 1M keys, batched vs single-query"
+INT8_PEAK_TOPS = 5000.0  # MI355X dense int8 MFMA
 HBM_PEAK_GBS = 8000.0
 RECORDS, KEY, VALUE = 1_000_000, 32, 1184
 
@@ -327,6 +328,22 @@ def main():
             log("batch %d: %.2f ms/batch, %.1f q/s (%.1fx single)" % (B, tot / reps, qps_b, qps_b / qps))
             del bq, bout
     result["batched"] = batched
+    # int8 MFMA rooflines of the batched GEMMs at the largest batch <= 64
+    # (stage times from hipEvents on the batch stream).
+    rb = [int(b) for b in batched if int(b) <= 64]
+    if rb:
+        B = max(rb)
+        st_b = batched[str(B)]["stage_ms"]
+        pack_ops = B * 2 * inst * IA.N * 6141 * 16 * 2          # limb-pair MACs x 2
+        gemv_ops = B * 16 * plan["db_cols"] * db_rows * 2 * 2    # 16 rows/query, 2 byte planes, MAC x 2
+        mk = lambda ops, ms, kern: {"bound": "mfma", "kernel": kern, "achieved": ops / (ms / 1e3) / 1e12,
+                                    "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
+                                    "frac": ops / (ms / 1e3) / 1e12 / INT8_PEAK_TOPS, "ops": ops, "ms": ms}
+        result["batched_roofline"] = {"B": B, "packing": mk(pack_ops, st_b["packing"], "k_packing_wide"),
+                                      "gemv": mk(gemv_ops, st_b["gemv"], "k_gemv_wide")}
+        log("batched MFMA at B=%d: packing %.0f TOP/s (%.1f%%), gemv %.0f TOP/s (%.1f%%)" % (
+            B, result["batched_roofline"]["packing"]["achieved"], 100 * result["batched_roofline"]["packing"]["frac"],
+            result["batched_roofline"]["gemv"]["achieved"], 100 * result["batched_roofline"]["gemv"]["frac"]))
 
     # Serving: continuous batching under open-loop Poisson arrivals.
     serving = []

@@ -431,3 +431,41 @@ def test_engine_rejects_bad_queries_and_full_queue(torch):
         eng.wait(t1)                             # already waited
     eng.close()
     srv.close()
+
+
+def test_engines_on_two_databases(torch):
+    # Two resident databases of different shapes, each behind its own engine on
+    # the one GPU; queries to both are submitted interleaved from two threads.
+    say("engines on two databases")
+    import threading
+    shapes = [(2048, 8, 4, 61), (4096, 12, 4, 62)]
+    srvs, engs, qs, solo = [], [], [], []
+    for rows, inst, d, seed in shapes:
+        srv = IA.Server(rows, inst, d)
+        srv.fill_synthetic(seed)
+        q = [x for x, _ in _queries(rows, [seed * 10 + i for i in range(10)])]
+        solo.append([srv.respond_serialized(x) for x in q])
+        srvs.append(srv)
+        qs.append(q)
+    for srv in srvs:
+        engs.append(IA.Engine(srv, nq=2, max_batch=8, window_ms=2.0, capacity=32))
+    tickets = [[None] * 10 for _ in shapes]
+
+    def submit(k):
+        for i in range(10):
+            for s in range(len(shapes)):
+                if (i + s) % 2 == k:
+                    tickets[s][i] = engs[s].submit(qs[s][i])
+
+    th = [threading.Thread(target=submit, args=(k,)) for k in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for s in range(len(shapes)):
+        for i in range(10):
+            assert engs[s].wait(tickets[s][i])[0] == solo[s][i], (s, i)
+    for e in engs:
+        e.close()
+    for srv in srvs:
+        srv.close()
