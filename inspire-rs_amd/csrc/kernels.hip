@@ -438,15 +438,15 @@ void launch_rotation_legacy(uint64_t* y_all, uint64_t* y_bar_all, const uint64_t
 
 __global__ __launch_bounds__(1024) void k_post(uint64_t* __restrict__ P, const uint32_t* __restrict__ S,
                                                const uint32_t* __restrict__ tw_tab) {
-  __shared__ uint32_t buf[2][kN];
+  __shared__ uint32_t buf[2][kNPad];
   const NttTables tw{tw_tab};
   const int j = blockIdx.x;
-  for (int t = threadIdx.x; t < 2 * kN; t += 1024) buf[t / kN][t % kN] = S[(size_t)j * 2 * kN + t];
+  for (int t = threadIdx.x; t < 2 * kN; t += 1024) buf[t / kN][lds_index(t % kN)] = S[(size_t)j * 2 * kN + t];
   __syncthreads();
   uint32_t* const polys[2] = {buf[0], buf[1]};
   const int pr[2] = {0, 1};
   ntt_inverse_r4<1024, 2>(polys, pr, tw);
-  for (int z = threadIdx.x; z < kN; z += 1024) P[(size_t)j * kN + z] = crt_compose(buf[0][z], buf[1][z]);
+  for (int z = threadIdx.x; z < kN; z += 1024) P[(size_t)j * kN + z] = crt_compose(buf[0][lds_index(z)], buf[1][lds_index(z)]);
 }
 
 void launch_post(uint64_t* P, const uint32_t* S, const uint32_t* tw, int count, hipStream_t s) {
@@ -482,11 +482,11 @@ struct FoldArgs {
 };
 
 __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
-  // LDS: the 6 gadget-digit polynomials for both primes (96 KiB) and the two
-  // product rows of both primes (32 KiB).
+  // LDS (padded polynomial layout of ntt_device.hpp): the 6 gadget-digit
+  // polynomials for both primes and the two product rows of both primes.
   extern __shared__ uint32_t lds[];
-  uint32_t* dig = lds;                 // [2 (prime)][6][N]
-  uint32_t* prod = lds + 12 * kN;      // [2 (prime)][2 (row)][N]
+  uint32_t* dig = lds;                 // [2 (prime)][6][kNPad]
+  uint32_t* prod = lds + 12 * kNPad;   // [2 (prime)][2 (row)][kNPad]
   const NttTables tw{args.tw};
   const int per_client = args.nq * args.c;
   const int b = blockIdx.x / per_client;
@@ -510,11 +510,11 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
     }
   }
 
-  uint32_t* const dpolys[12] = {dig,          dig + kN,     dig + 2 * kN, dig + 3 * kN,
-                                dig + 4 * kN, dig + 5 * kN, dig + 6 * kN, dig + 7 * kN,
-                                dig + 8 * kN, dig + 9 * kN, dig + 10 * kN, dig + 11 * kN};
+  uint32_t* const dpolys[12] = {dig,             dig + kNPad,     dig + 2 * kNPad, dig + 3 * kNPad,
+                                dig + 4 * kNPad, dig + 5 * kNPad, dig + 6 * kNPad, dig + 7 * kNPad,
+                                dig + 8 * kNPad, dig + 9 * kNPad, dig + 10 * kNPad, dig + 11 * kNPad};
   const int dprime[12] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1};
-  uint32_t* const ipolys[4] = {prod, prod + kN, prod + 2 * kN, prod + 3 * kN};
+  uint32_t* const ipolys[4] = {prod, prod + kNPad, prod + 2 * kNPad, prod + 3 * kNPad};
   const int iprime[4] = {0, 0, 1, 1};
   const uint64_t mask = (1ull << kGadgetBits) - 1;
 
@@ -523,14 +523,14 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
     // digits (< 2^19) are canonical for both primes.
 #pragma unroll
     for (int s = 0; s < kFoldSlots; ++s) {
-      const int z = tid + s * kFoldThreads;
+      const int lz = lds_index(tid + s * kFoldThreads);
 #pragma unroll
       for (int r = 0; r < 2; ++r)
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
           const uint32_t dv = (uint32_t)((sum[r][s] >> (kGadgetBits * k)) & mask);
-          dig[(r + 2 * k) * kN + z] = dv;
-          dig[(6 + r + 2 * k) * kN + z] = dv;
+          dig[(r + 2 * k) * kNPad + lz] = dv;
+          dig[(6 + r + 2 * k) * kNPad + lz] = dv;
         }
     }
     __syncthreads();
@@ -546,14 +546,14 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
         uint64_t acc = 0;
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
-          const uint32_t gk = dig[(6 * c + k) * kN + z];
+          const uint32_t gk = dig[(6 * c + k) * kNPad + lds_index(z)];
           gsum += gk;
           acc += (uint64_t)body[(k * 2 + c) * kN + z] * gk;
         }
         const uint32_t gs = gsum % q;
         const uint32_t na = args.neg_a[c * kN + z];
-        prod[(c * 2 + 0) * kN + z] = c == 0 ? mulmod<0>(na, gs) : mulmod<1>(na, gs);
-        prod[(c * 2 + 1) * kN + z] = (uint32_t)barrett64(acc, cr1(c), q);
+        prod[(c * 2 + 0) * kNPad + lds_index(z)] = c == 0 ? mulmod<0>(na, gs) : mulmod<1>(na, gs);
+        prod[(c * 2 + 1) * kNPad + lds_index(z)] = (uint32_t)barrett64(acc, cr1(c), q);
       }
     }
     __syncthreads();
@@ -562,8 +562,9 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
 #pragma unroll
     for (int s = 0; s < kFoldSlots; ++s) {
       const int z = tid + s * kFoldThreads;
-      const uint64_t x0 = crt_compose(prod[0 * kN + z], prod[2 * kN + z]);
-      const uint64_t x1 = crt_compose(prod[1 * kN + z], prod[3 * kN + z]);
+      const int lz = lds_index(z);
+      const uint64_t x0 = crt_compose(prod[0 * kNPad + lz], prod[2 * kNPad + lz]);
+      const uint64_t x1 = crt_compose(prod[1 * kNPad + lz], prod[3 * kNPad + lz]);
       sum[0][s] = reduce_small_q(x0 + args.a_hat[j * kN + z]);
       sum[1][s] = reduce_small_q(x1 + cv[j * kN + z] + Pc[j * kN + z]);
     }
@@ -573,12 +574,12 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
   // switch_and_keep(2^20, 268369921, 2048): row 0 -> 28 bits, row 1 -> 20 bits.
   __syncthreads();
   uint32_t* r0 = dig;
-  uint32_t* r1 = dig + kN;
+  uint32_t* r1 = dig + kNPad;
 #pragma unroll
   for (int s = 0; s < kFoldSlots; ++s) {
-    const int z = tid + s * kFoldThreads;
-    r0[z] = (uint32_t)rescale_from_q(sum[0][s], kQPrime2);
-    r1[z] = (uint32_t)rescale_from_q(sum[1][s], kQPrime1);
+    const int lz = lds_index(tid + s * kFoldThreads);
+    r0[lz] = (uint32_t)rescale_from_q(sum[0][s], kQPrime2);
+    r1[lz] = (uint32_t)rescale_from_q(sum[1][s], kQPrime1);
   }
   __syncthreads();
   uint32_t* out32 =
@@ -592,7 +593,7 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
       const int vlast = (min(hi, kRow0Bits) - 1) / 28;
       for (int v = lo / 28; v <= vlast; ++v) {
         const int sh = 28 * v - lo;
-        const uint32_t val = r0[v];
+        const uint32_t val = r0[lds_index(v)];
         word |= sh >= 0 ? (val << sh) : (val >> (-sh));
       }
     }
@@ -602,7 +603,7 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
       const int vlast = (bhi - 1) / 20;
       for (int v = blo / 20; v <= vlast; ++v) {
         const int sh = kRow0Bits + 20 * v - lo;
-        const uint32_t val = r1[v];
+        const uint32_t val = r1[lds_index(v)];
         word |= sh >= 0 ? (val << sh) : (val >> (-sh));
       }
     }
@@ -610,7 +611,7 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
   }
 }
 
-size_t fold_lds_bytes() { return (size_t)(12 * kN + 4 * kN) * sizeof(uint32_t); }
+size_t fold_lds_bytes() { return (size_t)(12 + 4) * kNPad * sizeof(uint32_t); }
 
 int launch_fold(uint8_t* out, const uint64_t* a_hat, const uint64_t* cvec, const uint64_t* P,
                 const uint32_t* bodies, const uint32_t* neg_a, const uint32_t* tw, int64_t instances, int d,
@@ -625,7 +626,7 @@ int launch_fold_batch(uint8_t* out, int64_t out_stride, const uint64_t* a_hat, c
   FoldArgs a{out, a_hat, cvec, P, bodies, neg_a, tw, instances, out_stride, d, (int)(instances / d), nq};
   static bool attr_set = false;
   if (!attr_set) {
-    // Raises the dynamic-LDS ceiling for this kernel (128 KB of the 160 KB per CU).
+    // Raises the dynamic-LDS ceiling for this kernel (132 KiB of the 160 KiB per CU).
     (void)hipFuncSetAttribute((const void*)k_fold, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)fold_lds_bytes());
     attr_set = true;
