@@ -167,36 +167,18 @@ void build_rgsw_neg_a(uint32_t* out, const uint32_t* tables) {
 
 namespace inspire {
 
-// For NTT slot z, term n of the packing GEMM reads key row kTermRowZero (a zero
-// row) or row plane*N + slot of the client's keys, planes 0-2 holding the y rows
-// k and 3-5 the z_body rows k.  Terms n = 6i + 2k + w (i < 1023) read y row k
-// at the image of z under the automorphism X -> X^(5^i mod 2N)
-// (automorph_ntt_table, packing.rs:1639-1652), w = 1 its conjugate slot
-// N-1-s; n = 6138 + k reads z_body row k at z itself.
-void build_packing_term_sources(uint16_t* out) {
+// Discrete logs for the slot-ordered packing GEMM: for the odd exponent
+// e = (-1)^w 5^i mod 2N (i < N/2), out[e >> 1] = i | (w << 15).  The packing
+// term of rotation i (packing.rs:1639-1652, automorphism X -> X^(5^i)) reads
+// key slot sigma at NTT slot z exactly when the exponent of sigma equals the
+// exponent of z times 5^i (w = 0) or minus it (w = 1).
+void build_slot_dlog(uint16_t* out) {
   constexpr uint32_t kMask = 2 * kN - 1;
-  static_assert(kTermRowZero < 65536, "term rows must fit 16 bits");
-  uint32_t g5[kNumRot];
   uint32_t g = 1;
-  for (int i = 0; i < kNumRot; ++i) {
-    g5[i] = g;
+  for (int i = 0; i < kN / 2; ++i) {
+    out[g >> 1] = (uint16_t)i;
+    out[((2 * kN - g) & kMask) >> 1] = (uint16_t)(i | 0x8000);
     g = (g * 5u) & kMask;
-  }
-  for (int z = 0; z < kN; ++z) {
-    const uint32_t ez = 2u * rev11((uint32_t)z) + 1u;
-    uint16_t* row = out + (size_t)z * kPackTerms;
-    for (int n = 0; n < kPackTerms; ++n) {
-      uint32_t v = kTermRowZero;
-      if (n < 6 * kNumRot) {
-        const int i = n / 6, r = n % 6;
-        const uint32_t s1 = rev11(((ez * g5[i]) & kMask) >> 1);
-        const uint32_t slot = (r & 1) ? (uint32_t)(kN - 1) - s1 : s1;
-        v = (uint32_t)(r >> 1) * kN + slot;
-      } else if (n < 6 * kNumRot + kTExp) {
-        v = (uint32_t)(3 + n - 6 * kNumRot) * kN + (uint32_t)z;
-      }
-      row[n] = (uint16_t)v;
-    }
   }
 }
 

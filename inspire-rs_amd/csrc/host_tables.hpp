@@ -19,11 +19,7 @@ void chacha20_u64(const uint8_t seed[32], uint64_t* out, size_t n);
 // out: [2][kN] = NTT(-a mod q_c) for the constant RGSW row-0 polynomial a.
 void build_rgsw_neg_a(uint32_t* out, const uint32_t* tables);
 
-// Packing GEMM term order: kPackTerms = 6141 terms padded to 96 steps of 64.
-constexpr int kPackTerms = 6144;
-// Key rows: 6 planes of kN slots, then one zero row for the padding terms.
-constexpr uint32_t kTermRowZero = 6 * kN;
-// out: [kN][kPackTerms] key row read by term n at NTT slot z.
-void build_packing_term_sources(uint16_t* out);
+// out: [N] discrete logs of the odd exponents mod 2N, (-1)^w 5^i -> i | w << 15.
+void build_slot_dlog(uint16_t* out);
 
 }  // namespace inspire

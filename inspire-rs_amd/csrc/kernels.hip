@@ -453,6 +453,73 @@ void launch_post(uint64_t* P, const uint32_t* S, const uint32_t* tw, int count, 
   hipLaunchKernelGGL(k_post, dim3(count), dim3(1024), 0, s, P, S, tw);
 }
 
+// Post-process of the MFMA packing.  S holds the rotation-term sums z-major,
+// S[b][p][z][j]; this kernel adds the z_body terms
+//     sum_k z_body_b[k][z] * bold_t_hat[j][k][z]   (packing_online_kernel.cu:70-88)
+// while staging kPostJ consecutive outputs j of one client into LDS, then runs
+// the inverse NTT and the CRT composition of all of them.  One block per
+// (group of kPostJ outputs, client); P[b][j][z].
+constexpr int kPostJ = 4;
+constexpr int kPostThreads = 512;
+
+__global__ __launch_bounds__(kPostThreads) void k_post_zmajor(uint64_t* __restrict__ P,
+                                                             const uint32_t* __restrict__ S,
+                                                             const uint64_t* __restrict__ z_body,
+                                                             int64_t key_client_stride,
+                                                             const uint64_t* __restrict__ bth,
+                                                             const uint32_t* __restrict__ tw_tab, int64_t instances) {
+  __shared__ uint32_t buf[2 * kPostJ][kNPad];
+  const NttTables tw{tw_tab};
+  const int64_t j0 = (int64_t)blockIdx.x * kPostJ;
+  const int64_t b = blockIdx.y;
+  const int nj = (int)(instances - j0 < kPostJ ? instances - j0 : kPostJ);
+  const uint32_t* Sb = S + (size_t)b * 2 * kN * instances;
+  const uint64_t* zb = z_body + b * key_client_stride;
+  for (int t = threadIdx.x; t < kN * kPostJ; t += kPostThreads) {
+    const int jj = t % kPostJ, z = t / kPostJ;
+    uint32_t v0 = 0, v1 = 0;
+    if (jj < nj) {
+      const int64_t j = j0 + jj;
+      const uint64_t* th = bth + (size_t)j * kTExp * kN + z;
+      uint64_t h0 = 0, h1 = 0;
+#pragma unroll
+      for (int k = 0; k < kTExp; ++k) {
+        const uint64_t zv = zb[k * kN + z], tv = th[k * kN];
+        h0 += (uint64_t)(uint32_t)zv * (uint32_t)tv;
+        h1 += (zv >> 32) * (tv >> 32);
+      }
+      v0 = (uint32_t)barrett64(h0, cr1(0), prime(0)) + Sb[(size_t)z * instances + j];
+      v1 = (uint32_t)barrett64(h1, cr1(1), prime(1)) + Sb[((size_t)kN + z) * instances + j];
+      v0 = v0 >= prime(0) ? v0 - prime(0) : v0;
+      v1 = v1 >= prime(1) ? v1 - prime(1) : v1;
+    }
+    buf[jj][lds_index(z)] = v0;
+    buf[kPostJ + jj][lds_index(z)] = v1;
+  }
+  __syncthreads();
+  uint32_t* polys[2 * kPostJ];
+  int pr[2 * kPostJ];
+#pragma unroll
+  for (int i = 0; i < 2 * kPostJ; ++i) {
+    polys[i] = buf[i];
+    pr[i] = i / kPostJ;
+  }
+  ntt_inverse_r4<kPostThreads, 2 * kPostJ>(polys, pr, tw);
+  for (int t = threadIdx.x; t < kN * kPostJ; t += kPostThreads) {
+    const int jj = t / kN, z = t % kN;
+    if (jj < nj)
+      P[((size_t)b * instances + j0 + jj) * kN + z] =
+          crt_compose(buf[jj][lds_index(z)], buf[kPostJ + jj][lds_index(z)]);
+  }
+}
+
+void launch_post_zmajor(uint64_t* P, const uint32_t* S, const uint64_t* z_body, int64_t key_client_stride,
+                        const uint64_t* bth, const uint32_t* tw, int clients, int64_t instances, hipStream_t s) {
+  const unsigned groups = (unsigned)((instances + kPostJ - 1) / kPostJ);
+  hipLaunchKernelGGL(k_post_zmajor, dim3(groups, (unsigned)clients), dim3(kPostThreads), 0, s, P, S, z_body,
+                     key_client_stride, bth, tw, instances);
+}
+
 // ---------------------------------------------------------------------------
 // RGSW fold + modulus switch (keyword.rs:543-581, modulus_switch.rs:45-73).
 //

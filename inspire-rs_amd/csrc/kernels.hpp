@@ -38,20 +38,20 @@ int launch_gemv_mfma_batch(uint64_t* c, int64_t c_client_stride, const uint64_t*
                            int nq, const uint16_t* db, int db_rows, int64_t db_cols, int count, void* scratch,
                            hipStream_t s);
 
-// Tiled int8-limb key-switch material for the MFMA packing (packing_mfma.hip).
+// Tiled int8-limb key-switch material for the MFMA packing (packing_mfma.hip),
+// rotation terms in key-slot order per NTT slot; dlog: DeviceConstants::slot_dlog.
 size_t tiled_bold_t_bytes(int64_t instances);
-void launch_tile_bold_t(uint8_t* out, const uint64_t* bt, const uint64_t* btb, const uint64_t* bth, int64_t instances,
-                        hipStream_t s);
-void launch_tile_bold_t_synth(uint8_t* out, uint64_t seed, int64_t instances, hipStream_t s);
+void launch_tile_bold_t(uint8_t* out, const uint64_t* bt, const uint64_t* btb, const uint16_t* dlog,
+                        int64_t instances, hipStream_t s);
 int batch_key_pad(int count);
-size_t batch_keys_bytes(int bpad);
-// Device bytes of the client-key buffer for up to `count` clients.
-size_t packing_keys_bytes(int count);
-// S[client][instances][2][N]; y_body/z_body: [count][3][N] (key_client_stride
-// elements apart); term_src: the per-device [N][6144] term-source table.
-int launch_packing_mfma(uint32_t* S, int64_t s_client_stride, const uint8_t* tiled, const uint16_t* term_src,
-                        uint32_t* keys, const uint64_t* y_body, const uint64_t* z_body, int64_t key_client_stride,
-                        int count, int64_t instances, hipStream_t s);
+// Device bytes of the client-key fragments for up to `count` clients.
+size_t packing_afrag_bytes(int count);
+// Rotation-term sums of the packing, z-major: S[client][2][N][instances];
+// y_body: [count][3][N] (key_client_stride elements apart); afrag: scratch of
+// packing_afrag_bytes(count) bytes.  k_post_zmajor adds the z_body terms.
+int launch_packing_mfma(uint32_t* S, int64_t s_client_stride, const uint8_t* tiled, uint8_t* afrag,
+                        const uint64_t* y_body, int64_t key_client_stride, int count, int64_t instances,
+                        hipStream_t s);
 
 // c: [nq][db_cols] canonical mod q; rows: [nq][db_rows] condensed; db: [db_cols][db_rows].
 int launch_gemv(uint64_t* c, const uint64_t* rows, int nq, const uint16_t* db, int db_rows, int64_t db_cols,
@@ -67,8 +67,14 @@ void launch_rotation_legacy(uint64_t* y_all, uint64_t* y_bar_all, const uint64_t
                             const uint32_t* gen_pows, int num_rotations, int t_exp, int poly_len,
                             hipStream_t s);
 
-// P: [count][N] canonical mod q.
+// P: [count][N] canonical mod q from S [count][2][N].
 void launch_post(uint64_t* P, const uint32_t* S, const uint32_t* tw, int count, hipStream_t s);
+// P: [clients][instances][N] from the z-major rotation-term sums S
+// [clients][2][N][instances] plus the z_body terms sum_k z_body[b][k] *
+// bold_t_hat[j][k] (z_body: [clients][3][N], key_client_stride apart).
+void launch_post_zmajor(uint64_t* P, const uint32_t* S, const uint64_t* z_body, int64_t key_client_stride,
+                        const uint64_t* bth, const uint32_t* tw, int clients, int64_t instances, hipStream_t s);
+
 
 size_t fold_lds_bytes();
 // out: [nq][instances/d][12288] response blobs.

@@ -57,10 +57,10 @@ const DeviceConstants* device_constants() {
   DeviceConstants& dc = g_consts[dev];
   if (dc.tw) return &dc;
   std::vector<uint32_t> tw(2 * 4 * kN), na(2 * kN);
-  std::vector<uint16_t> ts((size_t)kN * kPackTerms);
+  std::vector<uint16_t> ts(kN);
   build_ntt_tables(tw.data());
   build_rgsw_neg_a(na.data(), tw.data());
-  build_packing_term_sources(ts.data());
+  build_slot_dlog(ts.data());
   uint32_t *dtw = nullptr, *dna = nullptr;
   uint16_t* dts = nullptr;
   auto release = [&]() {
@@ -79,7 +79,7 @@ const DeviceConstants* device_constants() {
     release();
     return nullptr;
   }
-  dc.term_src = dts;
+  dc.slot_dlog = dts;
   dc.neg_a = dna;
   dc.tw = dtw;
   return &dc;
@@ -225,7 +225,7 @@ struct Slot {
   uint32_t *bodies = nullptr, *S = nullptr;
   uint8_t* out = nullptr;
   void* gemv_scratch = nullptr;  // MFMA query fragments
-  uint32_t* keys = nullptr;      // MFMA packing: client keys per prime (y then z)
+  uint8_t* keys = nullptr;       // MFMA packing: client-key fragments
   hipEvent_t done = nullptr;
 };
 
@@ -233,7 +233,8 @@ struct Slot {
 struct Batch {
   int max = 0;
   uint64_t *y_body = nullptr, *z_body = nullptr, *rows = nullptr, *P = nullptr, *cvec = nullptr;
-  uint32_t *bodies = nullptr, *S = nullptr, *keys = nullptr;
+  uint32_t *bodies = nullptr, *S = nullptr;
+  uint8_t* keys = nullptr;
   void* gemv_scratch = nullptr;
   hipEvent_t done = nullptr;
 };
@@ -310,7 +311,7 @@ int alloc_batch(inspire_server_t* s, int max) {
   if (!rc) rc = dev_alloc(&b.S, (size_t)max * inst * 2 * kN, "batch packing output");
   if (!rc) rc = dev_alloc(&b.P, (size_t)max * inst * kN, "batch post output");
   if (!rc) rc = dev_alloc(&b.cvec, (size_t)max * mq * s->db_cols, "batch gemv output");
-  if (!rc) rc = dev_alloc((uint8_t**)&b.keys, packing_keys_bytes(max), "batch keys");
+  if (!rc) rc = dev_alloc(&b.keys, packing_afrag_bytes(max), "batch key fragments");
   if (!rc)
     rc = dev_alloc((uint8_t**)&b.gemv_scratch, gemv_mfma_batch_scratch_bytes((int)rows, max), "batch gemv scratch");
   if (!rc && hipEventCreateWithFlags(&b.done, hipEventDisableTiming) != hipSuccess)
@@ -326,7 +327,7 @@ int ensure_tiled(inspire_server_t* s) {
     s->tiled_valid = false;
   }
   if (!s->tiled_valid && s->have_precomp) {
-    launch_tile_bold_t(s->tiled, s->bt, s->btb, s->bth, s->shape.instances, s->stream);
+    launch_tile_bold_t(s->tiled, s->bt, s->btb, s->consts->slot_dlog, s->shape.instances, s->stream);
     HIP_TRY(hipGetLastError(), "tile bold_t launch");
     HIP_TRY(hipStreamSynchronize(s->stream), "tile bold_t");
     s->tiled_valid = true;
@@ -362,7 +363,7 @@ int alloc_slots(inspire_server_t* s, int count) {
     if (!rc) rc = dev_alloc(&sl.cvec, (size_t)mq * s->db_cols, "gemv output");
     if (!rc) rc = dev_alloc(&sl.out, (size_t)mq * s->c * kRespBytes, "responses");
     if (!rc) rc = dev_alloc((uint8_t**)&sl.gemv_scratch, gemv_mfma_scratch_bytes((int)rows), "gemv scratch");
-    if (!rc) rc = dev_alloc((uint8_t**)&sl.keys, packing_keys_bytes(1), "packing keys");
+    if (!rc) rc = dev_alloc(&sl.keys, packing_afrag_bytes(1), "packing key fragments");
     if (!rc && hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess)
       rc = set_error(INSPIRE_ERR_HIP, "hipEventCreate failed");
     if (rc) return rc;
@@ -486,10 +487,9 @@ extern "C" int inspire_respond_batch_device(inspire_server_t* s, const uint8_t* 
   launch_unpack_query_batch(d_queries, (int64_t)q_stride, nq, (int)rows, count, b.y_body, b.z_body, b.rows, b.bodies,
                             st);
   mark(1);
-  launch_packing_mfma(b.S, inst * 2 * kN, s->tiled, s->consts->term_src, b.keys, b.y_body, b.z_body,
-                      (int64_t)kTExp * kN, count, inst, st);
+  launch_packing_mfma(b.S, inst * 2 * kN, s->tiled, b.keys, b.y_body, (int64_t)kTExp * kN, count, inst, st);
   mark(2);
-  launch_post(b.P, b.S, s->consts->tw, (int)(count * inst), st);
+  launch_post_zmajor(b.P, b.S, b.z_body, (int64_t)kTExp * kN, s->bth, s->consts->tw, count, inst, st);
   mark(3);
   if (launch_gemv_mfma_batch(b.cvec, (int64_t)nq * s->db_cols, b.rows, (int64_t)nq * rows, nq, s->db, (int)rows,
                              s->db_cols, count, b.gemv_scratch, st))
@@ -618,13 +618,14 @@ int run_pipeline(inspire_server_t* s, Slot& sl, const uint8_t* d_query, int nq, 
   mark(1);
   if (s->packing_kind == INSPIRE_PACKING_MFMA) {
     if (!s->tiled_valid) return set_error(INSPIRE_ERR_STATE, "tiled key-switch material not built");
-    launch_packing_mfma(sl.S, inst * 2 * kN, s->tiled, s->consts->term_src, sl.keys, sl.y_body, sl.z_body,
-                        (int64_t)kTExp * kN, 1, inst, st);
+    launch_packing_mfma(sl.S, inst * 2 * kN, s->tiled, sl.keys, sl.y_body, (int64_t)kTExp * kN, 1, inst, st);
+    mark(2);
+    launch_post_zmajor(sl.P, sl.S, sl.z_body, (int64_t)kTExp * kN, s->bth, s->consts->tw, 1, inst, st);
   } else {
     launch_packing(sl.S, s->bt, s->btb, s->bth, sl.y_body, sl.z_body, (int)inst, st);
+    mark(2);
+    launch_post(sl.P, sl.S, s->consts->tw, (int)inst, st);
   }
-  mark(2);
-  launch_post(sl.P, sl.S, s->consts->tw, (int)inst, st);
   mark(3);
   if (s->gemv_kind == INSPIRE_GEMV_MFMA) {
     // two hash queries per MFMA tile; larger nq runs in pairs
@@ -746,7 +747,7 @@ size_t server_batch_bytes(const inspire_server* s, int max) {
   b += (size_t)max * inst * 2 * kN * 4;         // packing output
   b += (size_t)max * inst * kN * 8;             // post output
   b += (size_t)max * mq * s->db_cols * 8;       // GEMV output
-  b += packing_keys_bytes(max);
+  b += packing_afrag_bytes(max);
   b += gemv_mfma_batch_scratch_bytes((int)rows, max);
   return b;
 }
