@@ -17,6 +17,8 @@
 // fragments use the same row order, so the contraction is over matching k.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.hpp"
 #include "modarith.hpp"
 
@@ -27,8 +29,15 @@ typedef unsigned int u4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMfmaChunkRows = 2048;                        // rows staged per LDS chunk
 constexpr int kMfmaStepsPerChunk = kMfmaChunkRows / 64;     // 32
-constexpr int kTilesPerWave = 2;                             // 2 x 16 columns per wave
-constexpr int kColsPerBlock = 4 * kTilesPerWave * 16;        // 128
+#ifndef GEMV_TPW
+#define GEMV_TPW 1
+#endif
+#ifndef GEMV_D
+#define GEMV_D 8
+#endif
+constexpr int kTilesPerWave = GEMV_TPW;                      // column tiles of 16 per wave
+constexpr int kGemvRing = GEMV_D;                            // DB steps in flight per wave
+constexpr int kColsPerBlock = 4 * kTilesPerWave * 16;
 
 // Row k of step s held by lane group g at byte t.
 __host__ __device__ inline int frag_row(int s, int g, int t) { return 64 * s + 8 * g + t + (t >= 8 ? 24 : 0); }
@@ -80,50 +89,121 @@ __device__ inline uint32_t plane_hi(uint32_t a, uint32_t b) {
   return __builtin_amdgcn_perm(b, a, 0x07050301u) ^ 0x80808080u;
 }
 
+// Fragment-ordered copy of the DB for the MFMA GEMVs:
+//   dbfrag[tile T][step s][plane][lane (j, g)][16 B],
+// plane 0 = low bytes, plane 1 = high bytes of the u16 values, each stored as
+// the signed byte x - 128; byte t of lane (j, g) is DB row frag_row(s, g, t)
+// of column 16T + j.  Same size as the DB; every MFMA B operand of a
+// (tile, step) is 2 KiB contiguous.
+__global__ void k_tile_db(u4* __restrict__ out, const uint16_t* __restrict__ db, int db_rows, int64_t db_cols) {
+  const int steps = db_rows / 64;
+  const int64_t total = db_cols / 16 * steps * 64;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int lane = (int)(e % 64);
+    const int64_t ts = e / 64;
+    const int s = (int)(ts % steps);
+    const int64_t tile = ts / steps;
+    const int j = lane & 15, g = lane >> 4;
+    const uint16_t* p = db + (tile * 16 + j) * (int64_t)db_rows + 64 * s + 8 * g;
+    const u4 d0 = *reinterpret_cast<const u4*>(p);
+    const u4 d1 = *reinterpret_cast<const u4*>(p + 32);
+    out[ts * 128 + lane] = u4{plane_lo(d0.x, d0.y), plane_lo(d0.z, d0.w), plane_lo(d1.x, d1.y), plane_lo(d1.z, d1.w)};
+    out[ts * 128 + 64 + lane] =
+        u4{plane_hi(d0.x, d0.y), plane_hi(d0.z, d0.w), plane_hi(d1.x, d1.y), plane_hi(d1.z, d1.w)};
+  }
+}
+
+size_t tiled_db_bytes(int db_rows, int64_t db_cols) { return (size_t)db_rows * (size_t)db_cols * 2; }
+
+int launch_tile_db(void* out, const uint16_t* db, int db_rows, int64_t db_cols, hipStream_t s) {
+  if (db_rows <= 0 || db_rows % 64 != 0 || db_cols <= 0 || db_cols % 16 != 0) return -1;
+  const int64_t total = db_cols / 16 * (db_rows / 64) * 64;
+  const int64_t blocks = std::min<int64_t>((total + 255) / 256, 1 << 20);
+  hipLaunchKernelGGL(k_tile_db, dim3((unsigned)blocks), dim3(256), 0, s, static_cast<u4*>(out), db, db_rows,
+                     db_cols);
+  return 0;
+}
+
+// Block = 4 waves x TPW column tiles of 16.  The query fragments of one
+// 2048-row chunk sit in LDS; the DB stream runs D steps ahead through a
+// register ring with clamped (unconditional) loads, so every wave keeps
+// about (D - 1) * TPW * 2 KiB of DB reads in flight.  FRAG: the DB is read
+// from its fragment-ordered copy (k_tile_db), one contiguous KiB per load
+// instruction; otherwise from the column-major DB.
+template <int TPW, int D, bool FRAG>
 __global__ __launch_bounds__(256) void k_gemv_mfma(uint64_t* __restrict__ c, const int8_t* __restrict__ afrag,
                                                    const int32_t* __restrict__ rsum,
-                                                   const uint16_t* __restrict__ db, int nq, int db_rows,
-                                                   int64_t db_cols) {
+                                                   const uint16_t* __restrict__ db, const u4* __restrict__ dbfrag,
+                                                   int nq, int db_rows, int64_t db_cols) {
+  static_assert(kMfmaStepsPerChunk % D == 0, "ring depth divides the chunk");
   __shared__ v4i a_lds[kMfmaStepsPerChunk][64];  // 32 KB
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int j = lane & 15, g = lane >> 4;
-  const int64_t col_base = (int64_t)blockIdx.x * kColsPerBlock + wave * (kTilesPerWave * 16);
+  const int64_t col_base = (int64_t)blockIdx.x * (4 * TPW * 16) + wave * (TPW * 16);
+  const int steps = db_rows / 64;
 
-  v4i acc_lo[kTilesPerWave], acc_hi[kTilesPerWave];
-  const uint16_t* colp[kTilesPerWave];
-  bool valid[kTilesPerWave];
+  v4i acc_lo[TPW], acc_hi[TPW];
+  const uint16_t* colp[TPW];
+  const u4* fragp[TPW];
 #pragma unroll
-  for (int tt = 0; tt < kTilesPerWave; ++tt) {
+  for (int tt = 0; tt < TPW; ++tt) {
     acc_lo[tt] = v4i{0, 0, 0, 0};
     acc_hi[tt] = v4i{0, 0, 0, 0};
-    const int64_t col = col_base + tt * 16 + j;
-    valid[tt] = col < db_cols;
-    colp[tt] = db + (valid[tt] ? col : 0) * (int64_t)db_rows + 8 * g;
+    const int64_t col = min(col_base + tt * 16 + j, db_cols - 1);
+    colp[tt] = db + col * (int64_t)db_rows + 8 * g;
+    const int64_t tile = min(col_base / 16 + tt, db_cols / 16 - 1);
+    fragp[tt] = FRAG ? dbfrag + tile * steps * 128 + lane : nullptr;
   }
+  u4 ring[D][TPW][2];
+  auto load = [&](int st, u4 (&r)[TPW][2]) {
+    const int sc = min(st, steps - 1);
+#pragma unroll
+    for (int tt = 0; tt < TPW; ++tt) {
+      if (FRAG) {
+        r[tt][0] = __builtin_nontemporal_load(fragp[tt] + sc * 128);
+        r[tt][1] = __builtin_nontemporal_load(fragp[tt] + sc * 128 + 64);
+      } else {
+        const uint16_t* p = colp[tt] + 64 * sc;
+        r[tt][0] = __builtin_nontemporal_load(reinterpret_cast<const u4*>(p));
+        r[tt][1] = __builtin_nontemporal_load(reinterpret_cast<const u4*>(p + 32));
+      }
+    }
+  };
 
-  for (int k0 = 0; k0 < db_rows; k0 += kMfmaChunkRows) {
+  for (int k0 = 0; k0 < steps; k0 += kMfmaStepsPerChunk) {
     __syncthreads();
-    const v4i* src = reinterpret_cast<const v4i*>(afrag) + (int64_t)(k0 / 64) * 64;
+    const v4i* src = reinterpret_cast<const v4i*>(afrag) + (int64_t)k0 * 64;
     for (int e = threadIdx.x; e < kMfmaStepsPerChunk * 64; e += 256) a_lds[e / 64][e % 64] = src[e];
     __syncthreads();
-#pragma unroll 4
-    for (int s = 0; s < kMfmaStepsPerChunk; ++s) {
-      const v4i a = a_lds[s][lane];
+    if (k0 == 0) {
 #pragma unroll
-      for (int tt = 0; tt < kTilesPerWave; ++tt) {
-        const uint16_t* p = colp[tt] + k0 + 64 * s;
-        u4 d0 = {0, 0, 0, 0}, d1 = {0, 0, 0, 0};
-        if (valid[tt]) {
-          d0 = __builtin_nontemporal_load(reinterpret_cast<const u4*>(p));
-          d1 = __builtin_nontemporal_load(reinterpret_cast<const u4*>(p + 32));
+      for (int r = 0; r < D; ++r) load(r, ring[r]);
+    }
+    for (int s = 0; s < kMfmaStepsPerChunk; s += D) {
+#pragma unroll
+      for (int r = 0; r < D; ++r) {
+        const v4i a = a_lds[s + r][lane];
+        v4i lo[TPW], hi[TPW];
+#pragma unroll
+        for (int tt = 0; tt < TPW; ++tt) {
+          const u4 d0 = ring[r][tt][0], d1 = ring[r][tt][1];
+          if (FRAG) {
+            lo[tt] = __builtin_bit_cast(v4i, d0);
+            hi[tt] = __builtin_bit_cast(v4i, d1);
+          } else {
+            lo[tt] = v4i{(int)plane_lo(d0.x, d0.y), (int)plane_lo(d0.z, d0.w), (int)plane_lo(d1.x, d1.y),
+                         (int)plane_lo(d1.z, d1.w)};
+            hi[tt] = v4i{(int)plane_hi(d0.x, d0.y), (int)plane_hi(d0.z, d0.w), (int)plane_hi(d1.x, d1.y),
+                         (int)plane_hi(d1.z, d1.w)};
+          }
         }
-        const v4i lo = {(int)plane_lo(d0.x, d0.y), (int)plane_lo(d0.z, d0.w), (int)plane_lo(d1.x, d1.y),
-                        (int)plane_lo(d1.z, d1.w)};
-        const v4i hi = {(int)plane_hi(d0.x, d0.y), (int)plane_hi(d0.z, d0.w), (int)plane_hi(d1.x, d1.y),
-                        (int)plane_hi(d1.z, d1.w)};
-        acc_lo[tt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, lo, acc_lo[tt], 0, 0, 0);
-        acc_hi[tt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, hi, acc_hi[tt], 0, 0, 0);
+        load(k0 + s + r + D, ring[r]);
+#pragma unroll
+        for (int tt = 0; tt < TPW; ++tt) {
+          acc_lo[tt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, lo[tt], acc_lo[tt], 0, 0, 0);
+          acc_hi[tt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, hi[tt], acc_hi[tt], 0, 0, 0);
+        }
       }
     }
   }
@@ -134,7 +214,7 @@ __global__ __launch_bounds__(256) void k_gemv_mfma(uint64_t* __restrict__ c, con
 #pragma unroll
   for (int r = 0; r < 4; ++r) rs[r] = (int64_t)rsum[4 * g + r] * 32896;
 #pragma unroll
-  for (int tt = 0; tt < kTilesPerWave; ++tt) {
+  for (int tt = 0; tt < TPW; ++tt) {
     int64_t v = 0;
 #pragma unroll
     for (int r = 0; r < 4; ++r)
@@ -157,11 +237,12 @@ __global__ __launch_bounds__(256) void k_gemv_mfma(uint64_t* __restrict__ c, con
 // meet in L2 and the DB leaves HBM about once per batch.
 // ---------------------------------------------------------------------------
 
-template <int CPW, int TW, int D>
+template <int CPW, int TW, int D, bool FRAG>
 __global__ __launch_bounds__(256) void k_gemv_wide(uint64_t* __restrict__ c, int64_t c_client_stride,
                                                    const int8_t* __restrict__ afrag, const int32_t* __restrict__ rsum,
-                                                   const uint16_t* __restrict__ db, int nq, int db_rows,
-                                                   int64_t db_cols, int count, int ngroups, int64_t col_blocks) {
+                                                   const uint16_t* __restrict__ db, const u4* __restrict__ dbfrag,
+                                                   int nq, int db_rows, int64_t db_cols, int count, int ngroups,
+                                                   int64_t col_blocks) {
   static_assert(TW * 64 == 256, "one DB staging item per thread");
   __shared__ v4i planes[2][TW][2][64];
   const int64_t bid = blockIdx.x;
@@ -178,6 +259,7 @@ __global__ __launch_bounds__(256) void k_gemv_wide(uint64_t* __restrict__ c, int
   const int st = tid >> 6, sj = lane & 15, sg = lane >> 4;
   const int64_t scol = min(col0 + 16 * st + sj, db_cols - 1);
   const uint16_t* dbp = db + scol * (int64_t)db_rows + 8 * sg;
+  const u4* fp = FRAG ? dbfrag + min(col0 / 16 + st, db_cols / 16 - 1) * steps * 128 + lane : nullptr;
   // Query fragments of this wave's clients (clamped to the last real one).
   const int client0 = grp * 4 * CPW + wave * CPW;
   const u4* ap[CPW];
@@ -186,15 +268,25 @@ __global__ __launch_bounds__(256) void k_gemv_wide(uint64_t* __restrict__ c, int
     ap[q] = reinterpret_cast<const u4*>(afrag) + (int64_t)min(client0 + q, count - 1) * steps * 64 + lane;
 
   auto load_db = [&](int s, u4 (&d)[2]) {
-    const uint16_t* p = dbp + 64 * s;
-    d[0] = __builtin_nontemporal_load(reinterpret_cast<const u4*>(p));
-    d[1] = __builtin_nontemporal_load(reinterpret_cast<const u4*>(p + 32));
+    if (FRAG) {
+      d[0] = __builtin_nontemporal_load(fp + (int64_t)s * 128);
+      d[1] = __builtin_nontemporal_load(fp + (int64_t)s * 128 + 64);
+    } else {
+      const uint16_t* p = dbp + 64 * s;
+      d[0] = __builtin_nontemporal_load(reinterpret_cast<const u4*>(p));
+      d[1] = __builtin_nontemporal_load(reinterpret_cast<const u4*>(p + 32));
+    }
   };
   auto stage_db = [&](int buf, const u4 (&d)[2]) {
-    planes[buf][st][0][lane] = v4i{(int)plane_lo(d[0].x, d[0].y), (int)plane_lo(d[0].z, d[0].w),
-                                   (int)plane_lo(d[1].x, d[1].y), (int)plane_lo(d[1].z, d[1].w)};
-    planes[buf][st][1][lane] = v4i{(int)plane_hi(d[0].x, d[0].y), (int)plane_hi(d[0].z, d[0].w),
-                                   (int)plane_hi(d[1].x, d[1].y), (int)plane_hi(d[1].z, d[1].w)};
+    if (FRAG) {
+      planes[buf][st][0][lane] = __builtin_bit_cast(v4i, d[0]);
+      planes[buf][st][1][lane] = __builtin_bit_cast(v4i, d[1]);
+    } else {
+      planes[buf][st][0][lane] = v4i{(int)plane_lo(d[0].x, d[0].y), (int)plane_lo(d[0].z, d[0].w),
+                                     (int)plane_lo(d[1].x, d[1].y), (int)plane_lo(d[1].z, d[1].w)};
+      planes[buf][st][1][lane] = v4i{(int)plane_hi(d[0].x, d[0].y), (int)plane_hi(d[0].z, d[0].w),
+                                     (int)plane_hi(d[1].x, d[1].y), (int)plane_hi(d[1].z, d[1].w)};
+    }
   };
   auto load_a = [&](int s, u4 (&a)[CPW]) {
 #pragma unroll
@@ -280,8 +372,8 @@ size_t gemv_mfma_batch_scratch_bytes(int db_rows, int count) {
 }
 
 int launch_gemv_mfma_batch(uint64_t* c, int64_t c_client_stride, const uint64_t* rows, int64_t rows_client_stride,
-                           int nq, const uint16_t* db, int db_rows, int64_t db_cols, int count, void* scratch,
-                           hipStream_t s) {
+                           int nq, const uint16_t* db, const void* dbfrag, int db_rows, int64_t db_cols, int count,
+                           void* scratch, hipStream_t s) {
   if (nq < 1 || nq > 2 || count < 1 || db_rows <= 0 || db_rows % 256 != 0 || db_rows > (1 << 17) || db_cols <= 0)
     return -1;
   int8_t* afrag = static_cast<int8_t*>(scratch);
@@ -297,15 +389,21 @@ int launch_gemv_mfma_batch(uint64_t* c, int64_t c_client_stride, const uint64_t*
   const int ngroups = (count + 4 * kCpw - 1) / (4 * kCpw);
   const int64_t col_blocks = (db_cols + 16 * kTw - 1) / (16 * kTw);
   const int64_t blocks = (col_blocks + 7) / 8 * 8 * ngroups;
-  hipLaunchKernelGGL((k_gemv_wide<kCpw, kTw, 2>), dim3((unsigned)blocks), dim3(256), 0, s, c, c_client_stride, afrag,
-                     rsum, db, nq, db_rows, db_cols, count, ngroups, col_blocks);
+  if (dbfrag && db_cols % 16 == 0)
+    hipLaunchKernelGGL((k_gemv_wide<kCpw, kTw, 2, true>), dim3((unsigned)blocks), dim3(256), 0, s, c, c_client_stride,
+                       afrag, rsum, db, static_cast<const u4*>(dbfrag), nq, db_rows, db_cols, count, ngroups,
+                       col_blocks);
+  else
+    hipLaunchKernelGGL((k_gemv_wide<kCpw, kTw, 2, false>), dim3((unsigned)blocks), dim3(256), 0, s, c, c_client_stride,
+                       afrag, rsum, db, static_cast<const u4*>(nullptr), nq, db_rows, db_cols, count, ngroups,
+                       col_blocks);
   return 0;
 }
 
 size_t gemv_mfma_scratch_bytes(int db_rows) { return (size_t)(db_rows / 64) * 64 * 16 + 16 * sizeof(int32_t); }
 
-int launch_gemv_mfma(uint64_t* c, const uint64_t* rows, int nq, const uint16_t* db, int db_rows, int64_t db_cols,
-                     void* scratch, hipStream_t s) {
+int launch_gemv_mfma(uint64_t* c, const uint64_t* rows, int nq, const uint16_t* db, const void* dbfrag,
+                     int db_rows, int64_t db_cols, void* scratch, hipStream_t s) {
   if (nq < 1 || nq > 2 || db_rows <= 0 || db_rows % kMfmaChunkRows != 0 || db_rows > (1 << 17) || db_cols <= 0)
     return -1;
   int8_t* afrag = static_cast<int8_t*>(scratch);
@@ -315,7 +413,12 @@ int launch_gemv_mfma(uint64_t* c, const uint64_t* rows, int nq, const uint16_t* 
   const int prep_threads = (db_rows / 64) * 64;
   hipLaunchKernelGGL(k_gemv_prep, dim3((prep_threads + 255) / 256), dim3(256), 0, s, rows, nq, db_rows, afrag, rsum);
   const unsigned blocks = (unsigned)((db_cols + kColsPerBlock - 1) / kColsPerBlock);
-  hipLaunchKernelGGL(k_gemv_mfma, dim3(blocks), dim3(256), 0, s, c, afrag, rsum, db, nq, db_rows, db_cols);
+  if (dbfrag && db_cols % 16 == 0)
+    hipLaunchKernelGGL((k_gemv_mfma<kTilesPerWave, kGemvRing, true>), dim3(blocks), dim3(256), 0, s, c, afrag, rsum,
+                       db, static_cast<const u4*>(dbfrag), nq, db_rows, db_cols);
+  else
+    hipLaunchKernelGGL((k_gemv_mfma<kTilesPerWave, kGemvRing, false>), dim3(blocks), dim3(256), 0, s, c, afrag, rsum,
+                       db, static_cast<const u4*>(nullptr), nq, db_rows, db_cols);
   return 0;
 }
 

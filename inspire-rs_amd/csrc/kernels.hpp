@@ -26,17 +26,23 @@ int launch_fold_batch(uint8_t* out, int64_t out_stride, const uint64_t* a_hat, c
                       const uint64_t* P, const uint32_t* bodies, const uint32_t* neg_a, const uint32_t* tw,
                       int64_t instances, int d, int nq, int clients, hipStream_t s);
 
+// Fragment-ordered copy of the DB (signed byte planes in MFMA B-operand
+// order, tiled_db_bytes bytes); needs db_rows % 64 == 0 and db_cols % 16 == 0.
+size_t tiled_db_bytes(int db_rows, int64_t db_cols);
+int launch_tile_db(void* out, const uint16_t* db, int db_rows, int64_t db_cols, hipStream_t s);
+
 // Exact int8-limb MFMA GEMV for nq <= 2 hash queries; scratch holds the query
-// fragments (gemv_mfma_scratch_bytes(db_rows) bytes, 16-byte aligned).
+// fragments (gemv_mfma_scratch_bytes(db_rows) bytes, 16-byte aligned).  Reads
+// dbfrag (launch_tile_db) when given, else the column-major DB.
 size_t gemv_mfma_scratch_bytes(int db_rows);
-int launch_gemv_mfma(uint64_t* c, const uint64_t* rows, int nq, const uint16_t* db, int db_rows, int64_t db_cols,
-                     void* scratch, hipStream_t s);
+int launch_gemv_mfma(uint64_t* c, const uint64_t* rows, int nq, const uint16_t* db, const void* dbfrag,
+                     int db_rows, int64_t db_cols, void* scratch, hipStream_t s);
 
 // Batched: `count` keyword queries of nq <= 2 rows each; c[client][h][db_cols].
 size_t gemv_mfma_batch_scratch_bytes(int db_rows, int count);
 int launch_gemv_mfma_batch(uint64_t* c, int64_t c_client_stride, const uint64_t* rows, int64_t rows_client_stride,
-                           int nq, const uint16_t* db, int db_rows, int64_t db_cols, int count, void* scratch,
-                           hipStream_t s);
+                           int nq, const uint16_t* db, const void* dbfrag, int db_rows, int64_t db_cols, int count,
+                           void* scratch, hipStream_t s);
 
 // Tiled int8-limb key-switch material for the MFMA packing (packing_mfma.hip),
 // rotation terms in key-slot order per NTT slot; dlog: DeviceConstants::slot_dlog.

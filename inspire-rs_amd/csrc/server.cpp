@@ -248,6 +248,7 @@ struct inspire_server {
   const DeviceConstants* consts = nullptr;
   // resident state
   uint16_t* db = nullptr;
+  void* dbfrag = nullptr;  // fragment-ordered DB for the MFMA GEMVs (nullptr: column-major kernels)
   uint64_t *bt = nullptr, *btb = nullptr, *bth = nullptr, *a_hat = nullptr;
   bool have_db = false, have_precomp = false;
   std::vector<uint8_t> stash;  // entries back to back
@@ -399,6 +400,11 @@ extern "C" int inspire_server_create(const inspire_shape_t* shape, inspire_serve
   s->max_query_bytes = query_bytes(rows, shape->max_queries);
   int rc = INSPIRE_OK;
   if (!rc) rc = dev_alloc(&s->db, (size_t)rows * s->db_cols, "database");
+  if (!rc && rows % 64 == 0 && s->db_cols % 16 == 0 &&
+      hipMalloc(&s->dbfrag, tiled_db_bytes((int)rows, s->db_cols)) != hipSuccess) {
+    s->dbfrag = nullptr;  // the GEMVs then read the column-major DB
+    (void)hipGetLastError();
+  }
   if (!rc) rc = dev_alloc(&s->bt, (size_t)inst * kNInner * kN, "bold_t");
   if (!rc) rc = dev_alloc(&s->btb, (size_t)inst * kNInner * kN, "bold_t_bar");
   if (!rc) rc = dev_alloc(&s->bth, (size_t)inst * kTExp * kN, "bold_t_hat");
@@ -419,7 +425,7 @@ extern "C" void inspire_server_destroy(inspire_server_t* s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   free_slots(s);
   free_batch(s);
-  void* ptrs[] = {s->db, s->bt, s->btb, s->bth, s->a_hat, s->tiled};
+  void* ptrs[] = {s->db, s->dbfrag, s->bt, s->btb, s->bth, s->a_hat, s->tiled};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -491,8 +497,8 @@ extern "C" int inspire_respond_batch_device(inspire_server_t* s, const uint8_t* 
   mark(2);
   launch_post_zmajor(b.P, b.S, b.z_body, (int64_t)kTExp * kN, s->bth, s->consts->tw, count, inst, st);
   mark(3);
-  if (launch_gemv_mfma_batch(b.cvec, (int64_t)nq * s->db_cols, b.rows, (int64_t)nq * rows, nq, s->db, (int)rows,
-                             s->db_cols, count, b.gemv_scratch, st))
+  if (launch_gemv_mfma_batch(b.cvec, (int64_t)nq * s->db_cols, b.rows, (int64_t)nq * rows, nq, s->db, s->dbfrag,
+                             (int)rows, s->db_cols, count, b.gemv_scratch, st))
     return set_error(INSPIRE_ERR_INVALID, "unsupported GEMV shape");
   mark(4);
   if (launch_fold_batch(d_out, (int64_t)out_stride, s->a_hat, b.cvec, b.P, b.bodies, s->consts->neg_a, s->consts->tw,
@@ -545,6 +551,8 @@ extern "C" int inspire_server_set_db(inspire_server_t* s, const uint16_t* db, in
   const size_t bytes = (size_t)s->shape.db_rows * s->db_cols * sizeof(uint16_t);
   HIP_TRY(hipMemcpyAsync(s->db, db, bytes, host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, s->stream),
           "copy database");
+  if (s->dbfrag) launch_tile_db(s->dbfrag, s->db, (int)s->shape.db_rows, s->db_cols, s->stream);
+  HIP_TRY(hipGetLastError(), "tile database");
   HIP_TRY(hipStreamSynchronize(s->stream), "copy database");
   s->have_db = true;
   return INSPIRE_OK;
@@ -585,6 +593,7 @@ extern "C" int inspire_server_fill_synthetic(inspire_server_t* s, uint64_t seed)
   launch_synth_fill(s->btb, seed, kStreamBtb, 0, inst * kNInner * kN, kSynthCondensed, s->stream);
   launch_synth_fill(s->bth, seed, kStreamBth, 0, inst * kTExp * kN, kSynthCondensed, s->stream);
   launch_synth_fill(s->a_hat, seed, kStreamAhat, 0, inst * kN, kSynthModQ, s->stream);
+  if (s->dbfrag) launch_tile_db(s->dbfrag, s->db, (int)s->shape.db_rows, s->db_cols, s->stream);
   HIP_TRY(hipGetLastError(), "synthetic fill launch");
   HIP_TRY(hipStreamSynchronize(s->stream), "synthetic fill");
   s->have_db = s->have_precomp = true;
@@ -631,8 +640,8 @@ int run_pipeline(inspire_server_t* s, Slot& sl, const uint8_t* d_query, int nq, 
     // two hash queries per MFMA tile; larger nq runs in pairs
     for (int h = 0; h < nq; h += 2) {
       const int m = nq - h < 2 ? nq - h : 2;
-      if (launch_gemv_mfma(sl.cvec + (size_t)h * s->db_cols, sl.rows + (size_t)h * rows, m, s->db, (int)rows,
-                           s->db_cols, sl.gemv_scratch, st))
+      if (launch_gemv_mfma(sl.cvec + (size_t)h * s->db_cols, sl.rows + (size_t)h * rows, m, s->db, s->dbfrag,
+                           (int)rows, s->db_cols, sl.gemv_scratch, st))
         return set_error(INSPIRE_ERR_INVALID, "unsupported GEMV shape");
     }
   } else if (launch_gemv(sl.cvec, sl.rows, nq, s->db, (int)rows, s->db_cols, st)) {
@@ -790,7 +799,7 @@ extern "C" int inspire_gemv_mfma_device(uint64_t* d_c, const uint64_t* d_rows, i
                                         int64_t db_rows, int64_t db_cols, void* d_scratch, void* stream) {
   if (!d_c || !d_rows || !d_db || !d_scratch || nq < 1 || nq > 2)
     return set_error(INSPIRE_ERR_INVALID, "bad gemv argument");
-  if (launch_gemv_mfma(d_c, d_rows, nq, d_db, (int)db_rows, db_cols, d_scratch, (hipStream_t)stream))
+  if (launch_gemv_mfma(d_c, d_rows, nq, d_db, nullptr, (int)db_rows, db_cols, d_scratch, (hipStream_t)stream))
     return set_error(INSPIRE_ERR_INVALID, "db_rows must be a multiple of 2048 and at most 2^17");
   HIP_TRY(hipGetLastError(), "gemv launch");
   return INSPIRE_OK;
