@@ -548,6 +548,41 @@ struct FoldArgs {
   int nq;
 };
 
+// Bit-packs the rescaled rows r0 (28 bits) and r1 (20 bits), padded
+// polynomial layout in LDS, into the 12288-byte chain blob of (client b, hash
+// query h, chain w) (modulus_switch.rs:45-73, util.rs:303-321).
+template <int NT>
+__device__ inline void fold_write_response(const FoldArgs& args, int b, int h, int w, const uint32_t* r0,
+                                           const uint32_t* r1) {
+  uint32_t* out32 =
+      reinterpret_cast<uint32_t*>(args.out + (size_t)b * args.out_stride + ((size_t)h * args.c + w) * kRespBytes);
+  constexpr int kRow0Bits = 28 * kN;
+  constexpr int kTotalBits = 28 * kN + 20 * kN;
+  for (int wd = threadIdx.x; wd < kRespBytes / 4; wd += NT) {
+    const int lo = wd * 32, hi = lo + 32;
+    uint32_t word = 0;
+    if (lo < kRow0Bits) {
+      const int vlast = (min(hi, kRow0Bits) - 1) / 28;
+      for (int v = lo / 28; v <= vlast; ++v) {
+        const int sh = 28 * v - lo;
+        const uint32_t val = r0[lds_index(v)];
+        word |= sh >= 0 ? (val << sh) : (val >> (-sh));
+      }
+    }
+    if (hi > kRow0Bits && lo < kTotalBits) {
+      const int blo = max(lo, kRow0Bits) - kRow0Bits;
+      const int bhi = min(hi, kTotalBits) - kRow0Bits;
+      const int vlast = (bhi - 1) / 20;
+      for (int v = blo / 20; v <= vlast; ++v) {
+        const int sh = kRow0Bits + 20 * v - lo;
+        const uint32_t val = r1[lds_index(v)];
+        word |= sh >= 0 ? (val << sh) : (val >> (-sh));
+      }
+    }
+    out32[wd] = word;
+  }
+}
+
 __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
   // LDS (padded polynomial layout of ntt_device.hpp): the 6 gadget-digit
   // polynomials for both primes and the two product rows of both primes.
@@ -640,42 +675,118 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
 
   // switch_and_keep(2^20, 268369921, 2048): row 0 -> 28 bits, row 1 -> 20 bits.
   __syncthreads();
-  uint32_t* r0 = dig;
-  uint32_t* r1 = dig + kNPad;
 #pragma unroll
   for (int s = 0; s < kFoldSlots; ++s) {
     const int lz = lds_index(tid + s * kFoldThreads);
-    r0[lz] = (uint32_t)rescale_from_q(sum[0][s], kQPrime2);
-    r1[lz] = (uint32_t)rescale_from_q(sum[1][s], kQPrime1);
+    dig[lz] = (uint32_t)rescale_from_q(sum[0][s], kQPrime2);
+    dig[kNPad + lz] = (uint32_t)rescale_from_q(sum[1][s], kQPrime1);
   }
   __syncthreads();
-  uint32_t* out32 =
-      reinterpret_cast<uint32_t*>(args.out + (size_t)b * args.out_stride + ((size_t)h * args.c + w) * kRespBytes);
-  constexpr int kRow0Bits = 28 * kN;
-  constexpr int kTotalBits = 28 * kN + 20 * kN;
-  for (int wd = tid; wd < kRespBytes / 4; wd += kFoldThreads) {
-    const int lo = wd * 32, hi = lo + 32;
-    uint32_t word = 0;
-    if (lo < kRow0Bits) {
-      const int vlast = (min(hi, kRow0Bits) - 1) / 28;
-      for (int v = lo / 28; v <= vlast; ++v) {
-        const int sh = 28 * v - lo;
-        const uint32_t val = r0[lds_index(v)];
-        word |= sh >= 0 ? (val << sh) : (val >> (-sh));
-      }
+  fold_write_response<kFoldThreads>(args, b, h, w, dig, dig + kNPad);
+}
+
+// The same fold with 512 threads and the two CRT primes taken one after the
+// other: per Horner step and prime, the six gadget digits are transformed, the
+// two product rows are formed in place of digits 0 and 1 and transformed back;
+// prime 0's result waits in registers for the CRT.  50 KiB of LDS, so three
+// workgroups share a CU and every thread has a radix-4 unit in each pass.
+constexpr int kFold2Threads = 512;
+constexpr int kFoldSeqMinBlocks = 512;
+constexpr int kFold2Slots = kN / kFold2Threads;
+
+__global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_fold_seq(FoldArgs args) {
+  __shared__ uint32_t dig[6][kNPad];
+  const NttTables tw{args.tw};
+  const int per_client = args.nq * args.c;
+  const int b = blockIdx.x / per_client;
+  const int h = (blockIdx.x % per_client) / args.c;
+  const int w = blockIdx.x % args.c;
+  const int tid = threadIdx.x;
+  const int d = args.d;
+  const uint64_t* cv = args.cvec + ((size_t)b * args.nq + h) * args.instances * kN;
+  const uint64_t* Pc = args.P + (size_t)b * args.instances * kN;
+  const uint32_t* body = args.bodies + ((size_t)b * args.nq + h) * 6 * 2 * kN;
+
+  uint64_t sum[2][kFold2Slots];
+  {
+    const size_t j = (size_t)w * d + (d - 1);
+#pragma unroll
+    for (int s = 0; s < kFold2Slots; ++s) {
+      const int z = tid + s * kFold2Threads;
+      sum[0][s] = args.a_hat[j * kN + z];
+      sum[1][s] = reduce_small_q(cv[j * kN + z] + Pc[j * kN + z]);
     }
-    if (hi > kRow0Bits && lo < kTotalBits) {
-      const int blo = max(lo, kRow0Bits) - kRow0Bits;
-      const int bhi = min(hi, kTotalBits) - kRow0Bits;
-      const int vlast = (bhi - 1) / 20;
-      for (int v = blo / 20; v <= vlast; ++v) {
-        const int sh = kRow0Bits + 20 * v - lo;
-        const uint32_t val = r1[lds_index(v)];
-        word |= sh >= 0 ? (val << sh) : (val >> (-sh));
-      }
-    }
-    out32[wd] = word;
   }
+  uint32_t* const dpolys[6] = {dig[0], dig[1], dig[2], dig[3], dig[4], dig[5]};
+  uint32_t* const ipolys[2] = {dig[0], dig[1]};
+  const uint64_t mask = (1ull << kGadgetBits) - 1;
+
+  for (int i = d - 2; i >= 0; --i) {
+    uint32_t res0[2][kFold2Slots];
+#pragma unroll 1  // one prime at a time keeps the NTT within 80 VGPRs (6 waves per SIMD)
+    for (int c = 0; c < 2; ++c) {
+      // G^-1(sum): digit k of row r is gadget row r + 2k (gadget.rs:35-61).
+      // Each thread rewrites only its own slots, which it alone read last.
+#pragma unroll
+      for (int s = 0; s < kFold2Slots; ++s) {
+        const int lz = lds_index(tid + s * kFold2Threads);
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) dig[r + 2 * k][lz] = (uint32_t)((sum[r][s] >> (kGadgetBits * k)) & mask);
+      }
+      __syncthreads();
+      ntt_forward_r4_1p<kFold2Threads, 6>(dpolys, c, tw);
+      const uint32_t q = prime(c);
+#pragma unroll
+      for (int s = 0; s < kFold2Slots; ++s) {
+        const int z = tid + s * kFold2Threads;
+        const int lz = lds_index(z);
+        uint32_t gsum = 0;
+        uint64_t acc = 0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          const uint32_t gk = dig[k][lz];
+          gsum += gk;
+          acc += (uint64_t)body[(k * 2 + c) * kN + z] * gk;
+        }
+        const uint32_t gs = gsum % q;
+        const uint32_t na = args.neg_a[c * kN + z];
+        dig[0][lz] = c == 0 ? mulmod<0>(na, gs) : mulmod<1>(na, gs);
+        dig[1][lz] = (uint32_t)barrett64(acc, cr1(c), q);
+      }
+      __syncthreads();
+      ntt_inverse_r4_1p<kFold2Threads, 2>(ipolys, c, tw);
+      if (c == 0) {
+#pragma unroll
+        for (int s = 0; s < kFold2Slots; ++s) {
+          const int lz = lds_index(tid + s * kFold2Threads);
+          res0[0][s] = dig[0][lz];
+          res0[1][s] = dig[1][lz];
+        }
+      }
+    }
+    const size_t j = (size_t)w * d + i;
+#pragma unroll
+    for (int s = 0; s < kFold2Slots; ++s) {
+      const int z = tid + s * kFold2Threads;
+      const int lz = lds_index(z);
+      const uint64_t x0 = crt_compose(res0[0][s], dig[0][lz]);
+      const uint64_t x1 = crt_compose(res0[1][s], dig[1][lz]);
+      sum[0][s] = reduce_small_q(x0 + args.a_hat[j * kN + z]);
+      sum[1][s] = reduce_small_q(x1 + cv[j * kN + z] + Pc[j * kN + z]);
+    }
+  }
+
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < kFold2Slots; ++s) {
+    const int lz = lds_index(tid + s * kFold2Threads);
+    dig[0][lz] = (uint32_t)rescale_from_q(sum[0][s], kQPrime2);
+    dig[1][lz] = (uint32_t)rescale_from_q(sum[1][s], kQPrime1);
+  }
+  __syncthreads();
+  fold_write_response<kFold2Threads>(args, b, h, w, dig[0], dig[1]);
 }
 
 size_t fold_lds_bytes() { return (size_t)(12 + 4) * kNPad * sizeof(uint32_t); }
@@ -698,7 +809,13 @@ int launch_fold_batch(uint8_t* out, int64_t out_stride, const uint64_t* a_hat, c
                               (int)fold_lds_bytes());
     attr_set = true;
   }
-  hipLaunchKernelGGL(k_fold, dim3(clients * nq * a.c), dim3(kFoldThreads), fold_lds_bytes(), s, a);
+  const int blocks = clients * nq * a.c;
+  // Below two workgroups per CU the 1024-thread form (both primes at once)
+  // has the shorter chain latency; above, three k_fold_seq per CU win.
+  if (blocks >= kFoldSeqMinBlocks)
+    hipLaunchKernelGGL(k_fold_seq, dim3(blocks), dim3(kFold2Threads), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_fold, dim3(blocks), dim3(kFoldThreads), fold_lds_bytes(), s, a);
   return 0;
 }
 

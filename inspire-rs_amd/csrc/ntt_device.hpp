@@ -179,4 +179,129 @@ __device__ inline void ntt_inverse_r4(uint32_t* const (&polys)[NP], const int (&
   }
 }
 
+// Single-prime forms: NP polynomials, all over prime c (twiddles of c only).
+// Same transforms, layouts and sync contract as the forms above.
+template <int NT, int NP>
+__device__ inline void ntt_forward_r4_1p(uint32_t* const (&polys)[NP], int c, const NttTables& tw) {
+  const int tid = threadIdx.x;
+  const uint32_t q = prime(c), q2 = 2 * q;
+  const uint32_t* fw = tw.fw(c);
+  const uint32_t* fwp = tw.fwp(c);
+  auto bf = [q2, q](uint32_t& x, uint32_t& y, uint32_t w, uint32_t wp) {
+    const uint32_t xr = x >= q2 ? x - q2 : x;
+    const uint32_t qn = shoup_lazy(y, w, wp, q);
+    x = xr + qn;
+    y = xr + q2 - qn;
+  };
+  for (int mm = 0; mm + 1 < kLogN; mm += 2) {
+    const int m = 1 << mm, t = (kN / 2) >> mm, th = t >> 1;
+    for (int u = tid; u < kN / 4; u += NT) {
+      int i, j;
+      if (th >= 32) {
+        i = u / th;
+        j = u % th;
+      } else {
+        i = u % m;
+        j = u / m;
+      }
+      const int x0 = 2 * t * i + j;
+      const int l0 = lds_index(x0), l1 = lds_index(x0 + th), l2 = lds_index(x0 + t), l3 = lds_index(x0 + t + th);
+      const uint32_t w0 = fw[m + i], wp0 = fwp[m + i];
+      const uint32_t w1 = fw[2 * m + 2 * i], wp1 = fwp[2 * m + 2 * i];
+      const uint32_t w2 = fw[2 * m + 2 * i + 1], wp2 = fwp[2 * m + 2 * i + 1];
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        uint32_t* a = polys[p];
+        uint32_t a0 = a[l0], a1 = a[l1], a2 = a[l2], a3 = a[l3];
+        bf(a0, a2, w0, wp0);
+        bf(a1, a3, w0, wp0);
+        bf(a0, a1, w1, wp1);
+        bf(a2, a3, w2, wp2);
+        a[l0] = a0;
+        a[l1] = a1;
+        a[l2] = a2;
+        a[l3] = a3;
+      }
+    }
+    __syncthreads();
+  }
+  for (int u = tid; u < kN / 2; u += NT) {
+    const uint32_t w = fw[1024 + u], wp = fwp[1024 + u];
+    const int l0 = lds_index(2 * u), l1 = lds_index(2 * u + 1);
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      uint32_t* a = polys[p];
+      uint32_t x = a[l0], y = a[l1];
+      bf(x, y, w, wp);
+      x = x >= q2 ? x - q2 : x;
+      x = x >= q ? x - q : x;
+      y = y >= q2 ? y - q2 : y;
+      y = y >= q ? y - q : y;
+      a[l0] = x;
+      a[l1] = y;
+    }
+  }
+  __syncthreads();
+}
+
+template <int NT, int NP>
+__device__ inline void ntt_inverse_r4_1p(uint32_t* const (&polys)[NP], int c, const NttTables& tw) {
+  const int tid = threadIdx.x;
+  const uint32_t q = prime(c), q2 = 2 * q;
+  const uint32_t* iw = tw.iw(c);
+  const uint32_t* iwp = tw.iwp(c);
+  auto bf = [q2, q](uint32_t& x, uint32_t& y, uint32_t w, uint32_t wp) {
+    const uint32_t tt = q2 - y + x;
+    uint32_t cx = x + y;
+    cx = cx >= q2 ? cx - q2 : cx;
+    x = (cx + ((cx & 1u) ? q : 0u)) >> 1;
+    y = w * tt - mulhi32(tt, wp) * q;
+  };
+  for (int u = tid; u < kN / 2; u += NT) {
+    const int l0 = lds_index(2 * u), l1 = lds_index(2 * u + 1);
+    const uint32_t w = iw[1024 + u], wp = iwp[1024 + u];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      uint32_t* a = polys[p];
+      uint32_t x = a[l0], y = a[l1];
+      bf(x, y, w, wp);
+      a[l0] = x;
+      a[l1] = y;
+    }
+  }
+  __syncthreads();
+  for (int mm = kLogN - 2; mm >= 1; mm -= 2) {
+    const int h = 1 << mm, t = (kN / 2) >> mm, ni = (kN / 4) / t;
+    for (int u = tid; u < kN / 4; u += NT) {
+      int i, j;
+      if (t >= 32) {
+        i = u / t;
+        j = u % t;
+      } else {
+        i = u % ni;
+        j = u / ni;
+      }
+      const int x0 = 4 * t * i + j;
+      const int l0 = lds_index(x0), l1 = lds_index(x0 + t), l2 = lds_index(x0 + 2 * t), l3 = lds_index(x0 + 3 * t);
+      const uint32_t wa = iw[h + 2 * i], wpa = iwp[h + 2 * i];
+      const uint32_t wb = iw[h + 2 * i + 1], wpb = iwp[h + 2 * i + 1];
+      const uint32_t wc = iw[(h >> 1) + i], wpc = iwp[(h >> 1) + i];
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        uint32_t* a = polys[p];
+        uint32_t a0 = a[l0], a1 = a[l1], a2 = a[l2], a3 = a[l3];
+        bf(a0, a1, wa, wpa);
+        bf(a2, a3, wb, wpb);
+        bf(a0, a2, wc, wpc);
+        bf(a1, a3, wc, wpc);
+        a[l0] = a0 >= q ? a0 - q : a0;
+        a[l1] = a1 >= q ? a1 - q : a1;
+        a[l2] = a2 >= q ? a2 - q : a2;
+        a[l3] = a3 >= q ? a3 - q : a3;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace inspire
