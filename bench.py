@@ -181,6 +181,9 @@ def main():
     ap.add_argument("--rates", default="100,500,1000,2000,3000",
                     help="Poisson arrival rates (q/s) for the serving run ('' to skip)")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrivals per rate span about this long")
+    ap.add_argument("--packing", choices=("mfma", "valu"), default="mfma",
+                    help="single-query packing kernel: int8 MFMA over the fragment-ordered bold_t, or the VALU "
+                         "kernel over the reference layout")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)",
@@ -197,6 +200,10 @@ def main():
     t0 = time.time()
     srv.fill_synthetic(11)
     log("state resident in %.1fs" % (time.time() - t0))
+    if args.packing == "mfma":
+        t0 = time.time()
+        srv.set_packing_kernel(IA.PACKING_MFMA)
+        log("fragment-ordered bold_t built in %.1fs" % (time.time() - t0))
     dq, qlen, stride = make_queries(IA, torch, db_rows, args.queries, 13)
     # One output buffer and one stream per request slot: requests alternate
     # between the streams, so one request's fold (22 workgroups) overlaps the
@@ -258,7 +265,11 @@ def main():
     srv.profile(False)
     names = ["unpack", "packing", "post", "gemv", "fold"]
     stage = {n: stage_ms[i] / max(cnt, 1) for i, n in enumerate(names)}
-    pack_bytes = plan["bold_t_bytes"] + inst * 3 * IA.N * 8
+    # Algorithmic bytes of the packing kernel: bold_t and bold_t_bar once; the
+    # VALU kernel also reads bold_t_hat (the MFMA path adds those terms in the
+    # post-process).
+    pack_bytes = plan["bold_t_bytes"] + (inst * 3 * IA.N * 8 if args.packing == "valu" else 0)
+    pack_kernel = "k_packing_mfma<1, 44>" if args.packing == "mfma" else "k_packing<8, true>"
     gemv_bytes = plan["db_bytes"]
     pack_gbs = pack_bytes / (stage["packing"] / 1e3) / 1e9
     gemv_gbs = gemv_bytes / (stage["gemv"] / 1e3) / 1e9
@@ -272,15 +283,16 @@ def main():
                    "records": RECORDS, "entry_bytes": KEY + VALUE, "buckets": plan["num_items"],
                    "dim0": plan["dim0"], "db_rows": db_rows, "db_cols": plan["db_cols"], "instances": inst,
                    "interpolate_degree": d, "chains": plan["c"], "hash_queries": 2,
+                   "packing_kernel": args.packing,
                    "parallelism": "1 GPU"},
         "p50_ms": p50, "p99_ms": p99,
         "serial": {"qps": ser_qps, "ms_per_query": ser_ms / max(10, args.steps // 4), "p50_ms": ser_p50,
                    "p99_ms": ser_p99},
         "slots": nslots,
         "stage_ms": stage,
-        "roofline": {"bound": "hbm", "kernel": "k_packing (InspiRING key-switch inner product)",
+        "roofline": {"bound": "hbm", "kernel": pack_kernel + " (InspiRING key-switch inner product)",
                      "achieved": pack_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": pack_gbs / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic("inspire::k_packing"), "algorithmic_bytes": pack_bytes,
+                     "traffic": profiled_traffic("inspire::" + pack_kernel), "algorithmic_bytes": pack_bytes,
                      "gemv": {"achieved": gemv_gbs, "frac": gemv_gbs / HBM_PEAK_GBS, "algorithmic_bytes": gemv_bytes}},
     }
     # Batched respond: B keyword queries per pass over the DB and bold_t.

@@ -134,7 +134,10 @@ typedef struct {
 
 typedef struct inspire_server inspire_server_t;
 
-/* Allocates device state and per-request buffers on the current device. */
+/* Allocates device state and per-request buffers on the current device,
+ * including a fragment-ordered copy of the database (the database's size)
+ * that the MFMA GEMVs read; if that copy cannot be allocated they read the
+ * column-major database. */
 int inspire_server_create(const inspire_shape_t* shape, inspire_server_t** out);
 void inspire_server_destroy(inspire_server_t* srv);
 
