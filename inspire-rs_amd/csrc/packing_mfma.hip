@@ -247,28 +247,36 @@ __global__ __launch_bounds__(256) void k_packing_mfma(PackArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Large batches: block = 4 waves, each with all MT M-tiles of a client group
-// and NTW N-tiles, so every B fragment leaves HBM once per group and feeds MT
-// MFMAs.  B fragments go through a D-step register ring; the A fragments
-// (MT KiB per step, the same for every slot) are copied into an LDS ring of
-// D+1 steps by the LDS DMA path (global_load_lds_dwordx4), so they take no
-// registers.  A step waits with an explicit vector-memory count for its own A
-// copies (issued D steps earlier) and a barrier for everyone else's; each A
-// fragment is read from LDS three fragments' MFMAs ahead of its own.
+// Large batches: block = W waves, each with all MT M-tiles of a client group
+// and NTW N-tiles (W * NTW N-tiles per block), so every B fragment leaves HBM
+// once per group and feeds MT MFMAs.  B fragments go through a D-step register
+// ring; the A fragments (MT KiB per step, the same for every slot) are copied
+// into an LDS ring of D+1 steps, shared by all W waves, by the LDS DMA path
+// (global_load_lds_dwordx4), so they take no registers.  A step waits with an
+// explicit vector-memory count for its own A copies (issued D steps earlier)
+// and a barrier for everyone else's; each A fragment is read from LDS four
+// fragments' MFMAs ahead of its use.  WPE = waves per SIMD the register budget
+// is sized for.
 // ---------------------------------------------------------------------------
 
-template <int MT, int NTW, int NTG, int D>
-__global__ __launch_bounds__(256) void k_packing_wide(PackArgs a) {
-  static_assert(NTG <= 4 * NTW && NTG > 4 * NTW - NTW, "four waves cover the group's N-tiles");
-  constexpr int kAPerWave = MT / 4;  // A fragments each wave copies per step
+template <int MT, int NTW, int W, int D, int WPE>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_packing_wide(PackArgs a) {
+  constexpr int NTG = W * NTW;                 // N-tiles per block
+  constexpr int kAPerWave = (MT + W - 1) / W;  // A fragments each wave copies per step
   constexpr int kGroups = MT / 4;
-  static_assert(MT % 4 == 0 && kKSteps % D == 0, "A copies split over 4 waves; D divides the steps");
+  static_assert(MT % 4 == 0 && kKSteps % D == 0, "A reads in fours; D divides the steps");
   // Vector-memory operations a wave issues after its A copies of step s:
   // that step's B loads and D-1 later steps.
   constexpr int kWait = (kAPerWave + NTW) * (D - 1) + NTW;
   static_assert(kWait < 64, "vector-memory count field");
   // Slots 0..D hold the ring; slot D+1 takes the copies issued past the last step.
   __shared__ u4 aring[D + 2][MT][64];
+  // Epilogue rounds: all waves park kRoundX M-tiles x NTW tiles at once.
+  constexpr int kRoundX = (W * MT * NTW <= (D + 2) * MT) ? MT
+                          : (W * (MT / 2) * NTW <= (D + 2) * MT) ? MT / 2
+                          : (W * (MT / 4) * NTW <= (D + 2) * MT) ? MT / 4
+                                                                  : MT / 8;
+  static_assert(kRoundX >= 1 && MT % kRoundX == 0 && W * kRoundX * NTW <= (D + 2) * MT, "epilogue fits the A ring");
 
   // Blocks of one (p, z) are consecutive on one XCD (block id mod 8), client
   // groups innermost so the groups' reads of a B fragment meet in L2.
@@ -284,14 +292,13 @@ __global__ __launch_bounds__(256) void k_packing_wide(PackArgs a) {
 
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  // Tiles past the group (the last wave's extra tile) or the matrix repeat a
-  // real tile's loads and are not stored.
+  // Tiles past the matrix repeat the last real tile's loads and are not stored.
   const int nt0 = ng * NTG;
   const uint8_t* tb = a.tiled + ((size_t)p * kN + z) * a.ntiles * kKSteps * 1024;
   const uint8_t* bbase[NTW];
 #pragma unroll
   for (int y = 0; y < NTW; ++y) {
-    const int64_t nt = min((int64_t)(nt0 + min(wave * NTW + y, NTG - 1)), a.ntiles - 1);
+    const int64_t nt = min((int64_t)(nt0 + wave * NTW + y), a.ntiles - 1);
     bbase[y] = tb + (size_t)nt * kKSteps * 1024 + lane * 16;
   }
   const u4* abase = reinterpret_cast<const u4*>(a.afrag) + ((size_t)p * (a.bpad / 4) + mg * MT) * kKSteps * 64 + lane;
@@ -301,13 +308,15 @@ __global__ __launch_bounds__(256) void k_packing_wide(PackArgs a) {
     for (int y = 0; y < NTW; ++y)
       regs[y] = __builtin_nontemporal_load(reinterpret_cast<const u4*>(bbase[y] + (size_t)s * 1024));
   };
-  // A fragments of step s (or, past the last step, a repeat into the spare slot).
+  // A fragments of step s (or, past the last step, a repeat into the spare
+  // slot).  When W does not divide MT the last waves copy fragment MT-1 again:
+  // the same bytes to the same place.
   auto copy_a = [&](int s) {
     const int slot = s < kKSteps ? s % (D + 1) : D + 1;
     const int ss = min(s, kKSteps - 1);
 #pragma unroll
     for (int f = 0; f < kAPerWave; ++f) {
-      const int x = wave * kAPerWave + f;
+      const int x = min(wave * kAPerWave + f, MT - 1);
       __builtin_amdgcn_global_load_lds((const void*)(abase + ((size_t)x * kKSteps + ss) * 64),
                                        (lds_void*)&aring[slot][x][0], 16, 0, 0);
     }
@@ -366,21 +375,24 @@ __global__ __launch_bounds__(256) void k_packing_wide(PackArgs a) {
     }
   }
 
-  // Epilogue through the A ring, now idle: each wave parks half of its tiles
-  // there (static register indices), then reduces them in a rolled loop.
-  constexpr int kHalf = MT / 2 * NTW;  // tiles per wave per round
-  static_assert(4 * kHalf <= (D + 2) * MT, "the A ring holds a round of all four waves' tiles");
-  __syncthreads();
-  v4i* park = reinterpret_cast<v4i*>(&aring[0][0][0]) + wave * kHalf * 64 + lane;
+  // Epilogue through the A ring, now idle: each wave parks kRoundX x NTW of
+  // its tiles there per round (static register indices), then reduces them in
+  // a rolled loop.
+  constexpr int kRound = kRoundX * NTW;
+  // The copies issued past the last step write the spare slot: let them land
+  // before the ring is reused.
+  __builtin_amdgcn_s_waitcnt(0x3f70);  // vmcnt(0)
+  v4i* park = reinterpret_cast<v4i*>(&aring[0][0][0]) + wave * kRound * 64 + lane;
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int h = 0; h < MT / kRoundX; ++h) {
+    __syncthreads();
 #pragma unroll
-    for (int x = 0; x < MT / 2; ++x)
+    for (int x = 0; x < kRoundX; ++x)
 #pragma unroll
-      for (int y = 0; y < NTW; ++y) park[(x * NTW + y) * 64] = acc[h * (MT / 2) + x][y];
-    for (int t = 0; t < kHalf; ++t) {
-      const int x = h * (MT / 2) + t / NTW, y = t % NTW;
-      if (wave * NTW + y < NTG) store_tile(a, park[t * 64], p, z, mg * MT + x, nt0 + wave * NTW + y);
+      for (int y = 0; y < NTW; ++y) park[(x * NTW + y) * 64] = acc[h * kRoundX + x][y];
+    for (int t = 0; t < kRound; ++t) {
+      const int x = h * kRoundX + t / NTW, y = t % NTW;
+      store_tile(a, park[t * 64], p, z, mg * MT + x, nt0 + wave * NTW + y);
     }
   }
 }
@@ -407,12 +419,12 @@ static void launch_small(PackArgs a, hipStream_t s) {
   hipLaunchKernelGGL((k_packing_mfma<MT, NT>), dim3((unsigned)blocks), dim3(256), 0, s, a);
 }
 
-template <int MT, int NTW, int NTG, int D>
+template <int MT, int NTW, int W, int D, int WPE>
 static void launch_wide(PackArgs a, hipStream_t s) {
-  a.ngroups = (int)((a.ntiles + NTG - 1) / NTG);
+  a.ngroups = (int)((a.ntiles + W * NTW - 1) / (W * NTW));
   a.mgroups = a.bpad / (4 * MT);
   const int64_t blocks = (int64_t)kN * 2 * a.ngroups * a.mgroups;
-  hipLaunchKernelGGL((k_packing_wide<MT, NTW, NTG, D>), dim3((unsigned)blocks), dim3(256), 0, s, a);
+  hipLaunchKernelGGL((k_packing_wide<MT, NTW, W, D, WPE>), dim3((unsigned)blocks), dim3(64 * W), 0, s, a);
 }
 
 int launch_packing_mfma(uint32_t* S, int64_t s_client_stride, const uint8_t* tiled, uint8_t* afrag,
@@ -429,9 +441,11 @@ int launch_packing_mfma(uint32_t* S, int64_t s_client_stride, const uint8_t* til
   else if (bpad <= 16)
     launch_small<4, 44>(a, s);    // 4 M-tiles x 44 N-tiles
   else if (bpad == 32)
-    launch_wide<8, 4, 16, 6>(a, s);   // one 32-client group: waves of 8 x 4 tiles, 16 N-tiles per block
+    launch_wide<8, 4, 4, 6, 1>(a, s);     // one 32-client group: 4 waves of 8 x 4 tiles
   else
-    launch_wide<16, 4, 16, 6>(a, s);  // 64-client groups: waves of 16 x 4 tiles, 16 N-tiles per block
+    // 64-client groups: 4 waves of 16 x 2 tiles, 3-step ring, two workgroups
+    // per CU (80 KiB of LDS and 184 VGPRs each); 88 N-tiles = 11 groups of 8.
+    launch_wide<16, 2, 4, 3, 2>(a, s);
   return 0;
 }
 
