@@ -247,9 +247,10 @@ __global__ __launch_bounds__(256) void k_packing_mfma(PackArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Large batches: block = W waves, each with all MT M-tiles of a client group
-// and NTW N-tiles (W * NTW N-tiles per block), so every B fragment leaves HBM
-// once per group and feeds MT MFMAs.  B fragments go through a D-step register
+// Large batches: block = W waves over MT M-tiles (a client group) x
+// (W / WM) * NTW N-tiles; wave w owns M-tiles [(w % WM) * MT/WM, ...) and
+// N-tiles (w / WM) * NTW ..., so every B fragment leaves HBM once per group
+// (the WM waves that share it meet in L2) and feeds MT MFMAs.  B fragments go through a D-step register
 // ring; the A fragments (MT KiB per step, the same for every slot) are copied
 // into an LDS ring of D+1 steps, shared by all W waves, by the LDS DMA path
 // (global_load_lds_dwordx4), so they take no registers.  A step waits with an
@@ -259,12 +260,13 @@ __global__ __launch_bounds__(256) void k_packing_mfma(PackArgs a) {
 // is sized for.
 // ---------------------------------------------------------------------------
 
-template <int MT, int NTW, int W, int D, int WPE>
+template <int MT, int NTW, int W, int D, int WPE, int WM>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_packing_wide(PackArgs a) {
-  constexpr int NTG = W * NTW;                 // N-tiles per block
+  constexpr int MTW = MT / WM;                 // M-tiles per wave
+  constexpr int NTG = W / WM * NTW;            // N-tiles per block
   constexpr int kAPerWave = (MT + W - 1) / W;  // A fragments each wave copies per step
-  constexpr int kGroups = MT / 4;
-  static_assert(MT % 4 == 0 && kKSteps % D == 0, "A reads in fours; D divides the steps");
+  constexpr int kGroups = MTW / 4;
+  static_assert(W % WM == 0 && MTW % 4 == 0 && kKSteps % D == 0, "A reads in fours; D divides the steps");
   // Vector-memory operations a wave issues after its A copies of step s:
   // that step's B loads and D-1 later steps.
   constexpr int kWait = (kAPerWave + NTW) * (D - 1) + NTW;
@@ -272,11 +274,11 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE, WPE
   // Slots 0..D hold the ring; slot D+1 takes the copies issued past the last step.
   __shared__ u4 aring[D + 2][MT][64];
   // Epilogue rounds: all waves park kRoundX M-tiles x NTW tiles at once.
-  constexpr int kRoundX = (W * MT * NTW <= (D + 2) * MT) ? MT
-                          : (W * (MT / 2) * NTW <= (D + 2) * MT) ? MT / 2
-                          : (W * (MT / 4) * NTW <= (D + 2) * MT) ? MT / 4
-                                                                  : MT / 8;
-  static_assert(kRoundX >= 1 && MT % kRoundX == 0 && W * kRoundX * NTW <= (D + 2) * MT, "epilogue fits the A ring");
+  constexpr int kRoundX = (W * MTW * NTW <= (D + 2) * MT) ? MTW
+                          : (W * (MTW / 2) * NTW <= (D + 2) * MT) ? MTW / 2
+                          : (W * (MTW / 4) * NTW <= (D + 2) * MT) ? MTW / 4
+                                                                   : MTW / 8;
+  static_assert(kRoundX >= 1 && MTW % kRoundX == 0 && W * kRoundX * NTW <= (D + 2) * MT, "epilogue fits the A ring");
 
   // Blocks of one (p, z) are consecutive on one XCD (block id mod 8), client
   // groups innermost so the groups' reads of a B fragment meet in L2.
@@ -292,13 +294,14 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE, WPE
 
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wave % WM, wn = wave / WM;
   // Tiles past the matrix repeat the last real tile's loads and are not stored.
   const int nt0 = ng * NTG;
   const uint8_t* tb = a.tiled + ((size_t)p * kN + z) * a.ntiles * kKSteps * 1024;
   const uint8_t* bbase[NTW];
 #pragma unroll
   for (int y = 0; y < NTW; ++y) {
-    const int64_t nt = min((int64_t)(nt0 + wave * NTW + y), a.ntiles - 1);
+    const int64_t nt = min((int64_t)(nt0 + wn * NTW + y), a.ntiles - 1);
     bbase[y] = tb + (size_t)nt * kKSteps * 1024 + lane * 16;
   }
   const u4* abase = reinterpret_cast<const u4*>(a.afrag) + ((size_t)p * (a.bpad / 4) + mg * MT) * kKSteps * 64 + lane;
@@ -322,9 +325,9 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE, WPE
     }
   };
 
-  v4i acc[MT][NTW];
+  v4i acc[MTW][NTW];
 #pragma unroll
-  for (int x = 0; x < MT; ++x)
+  for (int x = 0; x < MTW; ++x)
 #pragma unroll
     for (int y = 0; y < NTW; ++y) acc[x][y] = v4i{0, 0, 0, 0};
 
@@ -344,7 +347,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE, WPE
       // waves' copies and their reads of the slot refilled below (step s-1's).
       __builtin_amdgcn_s_waitcnt((kWait & 15) | (((kWait >> 4) & 3) << 14) | 0x3f70);
       __builtin_amdgcn_s_barrier();
-      const u4* at = &aring[s % (D + 1)][0][0];
+      const u4* at = &aring[s % (D + 1)][wm * MTW][0];
       // A fragments four tiles at a time: fragment e of the next group is
       // read as soon as this group's MFMAs on fragment e have been issued.
       v4i ga[4];
@@ -384,7 +387,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE, WPE
   __builtin_amdgcn_s_waitcnt(0x3f70);  // vmcnt(0)
   v4i* park = reinterpret_cast<v4i*>(&aring[0][0][0]) + wave * kRound * 64 + lane;
 #pragma unroll
-  for (int h = 0; h < MT / kRoundX; ++h) {
+  for (int h = 0; h < MTW / kRoundX; ++h) {
     __syncthreads();
 #pragma unroll
     for (int x = 0; x < kRoundX; ++x)
@@ -392,7 +395,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE, WPE
       for (int y = 0; y < NTW; ++y) park[(x * NTW + y) * 64] = acc[h * kRoundX + x][y];
     for (int t = 0; t < kRound; ++t) {
       const int x = h * kRoundX + t / NTW, y = t % NTW;
-      store_tile(a, park[t * 64], p, z, mg * MT + x, nt0 + wave * NTW + y);
+      store_tile(a, park[t * 64], p, z, mg * MT + wm * MTW + x, nt0 + wn * NTW + y);
     }
   }
 }
@@ -419,12 +422,13 @@ static void launch_small(PackArgs a, hipStream_t s) {
   hipLaunchKernelGGL((k_packing_mfma<MT, NT>), dim3((unsigned)blocks), dim3(256), 0, s, a);
 }
 
-template <int MT, int NTW, int W, int D, int WPE>
+template <int MT, int NTW, int W, int D, int WPE, int WM>
 static void launch_wide(PackArgs a, hipStream_t s) {
-  a.ngroups = (int)((a.ntiles + W * NTW - 1) / (W * NTW));
+  constexpr int NTG = W / WM * NTW;
+  a.ngroups = (int)((a.ntiles + NTG - 1) / NTG);
   a.mgroups = a.bpad / (4 * MT);
   const int64_t blocks = (int64_t)kN * 2 * a.ngroups * a.mgroups;
-  hipLaunchKernelGGL((k_packing_wide<MT, NTW, W, D, WPE>), dim3((unsigned)blocks), dim3(64 * W), 0, s, a);
+  hipLaunchKernelGGL((k_packing_wide<MT, NTW, W, D, WPE, WM>), dim3((unsigned)blocks), dim3(64 * W), 0, s, a);
 }
 
 int launch_packing_mfma(uint32_t* S, int64_t s_client_stride, const uint8_t* tiled, uint8_t* afrag,
@@ -441,11 +445,11 @@ int launch_packing_mfma(uint32_t* S, int64_t s_client_stride, const uint8_t* til
   else if (bpad <= 16)
     launch_small<4, 44>(a, s);    // 4 M-tiles x 44 N-tiles
   else if (bpad == 32)
-    launch_wide<8, 4, 4, 6, 1>(a, s);     // one 32-client group: 4 waves of 8 x 4 tiles
+    launch_wide<8, 4, 4, 6, 1, 1>(a, s);     // one 32-client group: 4 waves of 8 x 4 tiles
   else
-    // 64-client groups: 4 waves of 16 x 2 tiles, 3-step ring, two workgroups
-    // per CU (80 KiB of LDS and 184 VGPRs each); 88 N-tiles = 11 groups of 8.
-    launch_wide<16, 2, 4, 3, 2>(a, s);
+    // 64-client groups: 11 waves of 16 x 2 tiles sharing one client-key ring,
+    // 2-step B ring, 3 waves per SIMD; 88 N-tiles = 4 groups of 22.
+    launch_wide<16, 2, 11, 2, 3, 1>(a, s);
   return 0;
 }
 
