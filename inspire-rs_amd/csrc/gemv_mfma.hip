@@ -357,6 +357,146 @@ __global__ __launch_bounds__(256) void k_gemv_wide(uint64_t* __restrict__ c, int
   }
 }
 
+// ---------------------------------------------------------------------------
+// Batched form over the fragment-ordered DB: a block covers QB = WQ*QW keyword
+// queries x TB = WT*TW column tiles with WQ*WT waves; wave (wq, wt) owns
+// queries wq*QW.. and tiles wt*TW...  Per K step the block's QB query
+// fragments and 2*TB DB plane fragments (1 KiB each) are copied into an LDS
+// ring by the LDS DMA path and read by every wave that needs them, so each
+// query fragment feeds TB*2 MFMAs per block and each DB fragment QB.
+// ---------------------------------------------------------------------------
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+template <int QW, int TW, int WQ, int WT, int D>
+__global__ __launch_bounds__(64 * WQ * WT) void k_gemv_lds(uint64_t* __restrict__ c, int64_t c_client_stride,
+                                                           const int8_t* __restrict__ afrag,
+                                                           const int32_t* __restrict__ rsum,
+                                                           const u4* __restrict__ dbfrag, int nq, int db_rows,
+                                                           int64_t db_cols, int count, int ngroups,
+                                                           int64_t col_blocks) {
+  constexpr int W = WQ * WT, QB = WQ * QW, TB = WT * TW;
+  constexpr int F = QB + 2 * TB;                 // fragments per step
+  constexpr int kPerWave = (F + W - 1) / W;      // copies per wave per step
+  static_assert(kPerWave * (D - 1) < 64, "vector-memory count field");
+  __shared__ u4 ring[D + 2][F][64];
+  const int64_t bid = blockIdx.x;
+  const int xcd = (int)(bid % 8);
+  const int64_t rest = bid / 8;
+  const int grp = (int)(rest % ngroups);
+  const int64_t cb = (rest / ngroups) * 8 + xcd;
+  if (cb >= col_blocks) return;  // whole block: the grid rounds column blocks up to a multiple of 8
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wq = wave % WQ, wt = wave / WQ;
+  const int steps = db_rows / 64;
+  const int64_t ntiles = db_cols / 16;
+  const int client0 = grp * QB;
+
+  // Copy sources of this wave (clamped: past-the-end queries and tiles repeat
+  // the last real one).
+  const u4* src[kPerWave];
+  int64_t sstep[kPerWave];
+  int dst[kPerWave];
+#pragma unroll
+  for (int i = 0; i < kPerWave; ++i) {
+    const int f = min(wave * kPerWave + i, F - 1);
+    dst[i] = f;
+    if (f < QB) {
+      src[i] = reinterpret_cast<const u4*>(afrag) + (int64_t)min(client0 + f, count - 1) * steps * 64 + lane;
+      sstep[i] = 64;
+    } else {
+      const int t = (f - QB) >> 1, pl = (f - QB) & 1;
+      const int64_t tile = min(cb * TB + t, ntiles - 1);
+      src[i] = dbfrag + tile * steps * 128 + pl * 64 + lane;
+      sstep[i] = 128;
+    }
+  }
+  auto copy = [&](int s) {
+    const int slot = s < steps ? s % (D + 1) : D + 1;
+    const int ss = min(s, steps - 1);
+#pragma unroll
+    for (int i = 0; i < kPerWave; ++i)
+      __builtin_amdgcn_global_load_lds((const void*)(src[i] + ss * sstep[i]), (lds_void*)&ring[slot][dst[i]][0], 16,
+                                       0, 0);
+  };
+
+  v4i acc[QW][TW][2];
+#pragma unroll
+  for (int q = 0; q < QW; ++q)
+#pragma unroll
+    for (int t = 0; t < TW; ++t) acc[q][t][0] = acc[q][t][1] = v4i{0, 0, 0, 0};
+
+  // Operands of one step, read from its ring slot.
+  auto read = [&](int s, v4i (&a)[QW], v4i (&b)[TW][2]) {
+    const u4* slot = &ring[s % (D + 1)][0][0];
+#pragma unroll
+    for (int q = 0; q < QW; ++q) a[q] = __builtin_bit_cast(v4i, slot[(wq * QW + q) * 64 + lane]);
+#pragma unroll
+    for (int t = 0; t < TW; ++t)
+#pragma unroll
+      for (int pl = 0; pl < 2; ++pl)
+        b[t][pl] = __builtin_bit_cast(v4i, slot[(QB + 2 * (wt * TW + t) + pl) * 64 + lane]);
+  };
+  // Step s's operands are read from LDS during step s-1's MFMAs.  At the top
+  // of step s a wave needs its own copies of step s+1 (at most kWait newer
+  // copies outstanding) and, after the barrier, everyone else's; the copy
+  // issued then refills the slot of step s-1, whose reads finished in step
+  // s-2 before every wave reached this barrier.
+  constexpr int kWait1 = kPerWave * (D - 1);
+  constexpr int kWait2 = kPerWave * (D - 2);
+  static_assert(D >= 2, "one step read ahead");
+#pragma unroll
+  for (int t = 0; t < D; ++t) copy(t);
+  __builtin_amdgcn_s_waitcnt((kWait1 & 15) | (((kWait1 >> 4) & 3) << 14) | 0x3f70);
+  __builtin_amdgcn_s_barrier();
+  v4i a[QW], b[TW][2];
+  read(0, a, b);
+  for (int s = 0; s < steps; ++s) {
+    __builtin_amdgcn_s_waitcnt((kWait2 & 15) | (((kWait2 >> 4) & 3) << 14) | 0x3f70);
+    __builtin_amdgcn_s_barrier();
+    copy(s + D);
+    v4i an[QW], bn[TW][2];
+    read(min(s + 1, steps - 1), an, bn);
+#pragma unroll
+    for (int t = 0; t < TW; ++t)
+#pragma unroll
+      for (int q = 0; q < QW; ++q)
+#pragma unroll
+        for (int pl = 0; pl < 2; ++pl)
+          acc[q][t][pl] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[q], b[t][pl], acc[q][t][pl], 0, 0, 0);
+#pragma unroll
+    for (int q = 0; q < QW; ++q) a[q] = an[q];
+#pragma unroll
+    for (int t = 0; t < TW; ++t) b[t][0] = bn[t][0], b[t][1] = bn[t][1];
+  }
+  __builtin_amdgcn_s_waitcnt(0x3f70);  // vmcnt(0): the copies past the last step have landed
+  __syncthreads();
+
+  // Lane (j, G) holds rows 4G..4G+3 = limbs 0..3 of (h, p) = (G >> 1, G & 1).
+  const int j = lane & 15, g = lane >> 4;
+  const int h = g >> 1, pr = g & 1;
+#pragma unroll
+  for (int q = 0; q < QW; ++q) {
+    const int client = client0 + wq * QW + q;
+    const int rc = min(client, count - 1);
+    int64_t rs[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) rs[r] = (int64_t)rsum[rc * 16 + 4 * g + r] * 32896;
+#pragma unroll
+    for (int t = 0; t < TW; ++t) {
+      int64_t v = 0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) v += ((int64_t)256 * acc[q][t][1][r] + (int64_t)acc[q][t][0][r] + rs[r]) << (7 * r);
+      const uint64_t red = pr ? barrett64((uint64_t)v, kCr1Q1, kQ1) : barrett64((uint64_t)v, kCr1Q0, kQ0);
+      const uint64_t other = __shfl_down(red, 16, 64);
+      const int64_t col = (cb * TB + wt * TW + t) * 16 + j;
+      if (pr == 0 && h < nq && col < db_cols && client < count)
+        c[(int64_t)client * c_client_stride + (int64_t)h * db_cols + col] = crt_compose((uint32_t)red, (uint32_t)other);
+    }
+  }
+}
+
 // Fragments of `count` clients, each with nq <= 2 rows of db_rows (stride
 // rows_client_stride elements between clients).
 __global__ void k_gemv_prep_batch(const uint64_t* __restrict__ rows, int64_t rows_client_stride, int nq, int db_rows,
@@ -389,11 +529,17 @@ int launch_gemv_mfma_batch(uint64_t* c, int64_t c_client_stride, const uint64_t*
   const int ngroups = (count + 4 * kCpw - 1) / (4 * kCpw);
   const int64_t col_blocks = (db_cols + 16 * kTw - 1) / (16 * kTw);
   const int64_t blocks = (col_blocks + 7) / 8 * 8 * ngroups;
-  if (dbfrag && db_cols % 16 == 0)
-    hipLaunchKernelGGL((k_gemv_wide<kCpw, kTw, 2, true>), dim3((unsigned)blocks), dim3(256), 0, s, c, c_client_stride,
-                       afrag, rsum, db, static_cast<const u4*>(dbfrag), nq, db_rows, db_cols, count, ngroups,
-                       col_blocks);
-  else
+  if (dbfrag && db_cols % 16 == 0) {
+    // 16 queries x 8 column tiles per block: 8 waves of 4 queries x 4 tiles,
+    // a 3-step LDS ring of 32 KiB per step, one block per CU.
+    constexpr int kQW = 4, kTW = 4, kWQ = 4, kWT = 2, kD = 3;
+    const int ng = (count + kQW * kWQ - 1) / (kQW * kWQ);
+    const int64_t cbs = (db_cols / 16 + kTW * kWT - 1) / (kTW * kWT);
+    const int64_t nb = (cbs + 7) / 8 * 8 * ng;
+    hipLaunchKernelGGL((k_gemv_lds<kQW, kTW, kWQ, kWT, kD>), dim3((unsigned)nb), dim3(64 * kWQ * kWT), 0, s, c,
+                       c_client_stride, afrag, rsum, static_cast<const u4*>(dbfrag), nq, db_rows, db_cols, count, ng,
+                       cbs);
+  } else
     hipLaunchKernelGGL((k_gemv_wide<kCpw, kTw, 2, false>), dim3((unsigned)blocks), dim3(256), 0, s, c, c_client_stride,
                        afrag, rsum, db, static_cast<const u4*>(nullptr), nq, db_rows, db_cols, count, ngroups,
                        col_blocks);
