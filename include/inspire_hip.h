@@ -51,6 +51,30 @@ void gpu_init_memory_pool(void);
 /* encode_kernel.cu:258-260: free device memory in bytes (0 on failure). */
 uint64_t gpu_available_memory_bytes(void);
 
+/* encode.rs:7-18 / encode_kernel.cu:386-424 (gpu_encode_db_from_host):
+ * encodes a raw host DB (u16, column-major [db_cols][db_rows], cuckoo.rs:
+ * 314-353) by polynomial interpolation mod `modulus` (65535) and keeps it on
+ * the device for gpu_encode_get_device_db; h_db_out is not written (the
+ * reference leaves it untouched too).  gamma must be 2048, interp_degree a
+ * power of two <= 32, num_inv = interp_degree^-1 mod modulus. */
+void gpu_encode_db_from_host(const uint16_t* h_db_in, uint16_t* h_db_out, int db_rows, int db_cols,
+                             int interp_degree, int gamma, int c, uint32_t modulus, uint32_t num_inv);
+/* encode.rs:16 / encode_kernel.cu:427-430: the device DB of the last encode (NULL if none). */
+const uint16_t* gpu_encode_get_device_db(void);
+/* encode.rs:17 / encode_kernel.cu:432-436. */
+void gpu_encode_free_device_db(void);
+/* encode.rs:19-28 / encode_kernel.cu:439-494: the same encode into a second
+ * slot, leaving the primary DB serving. */
+void gpu_encode_db_from_host_fold(const uint16_t* h_db_in, int db_rows, int db_cols, int interp_degree, int gamma,
+                                  int c, uint32_t modulus, uint32_t num_inv);
+/* encode.rs:29 / encode_kernel.cu:497-499. */
+const uint16_t* gpu_encode_get_device_db_fold(void);
+/* encode.rs:30 / encode_kernel.cu:502-510: free the primary, promote the second slot. */
+void gpu_encode_swap_primary(void);
+/* encode.rs:5 / encode_kernel.cu:233-238: waits for the device (no pooled
+ * allocator to trim here). */
+void gpu_trim_memory_pool(void);
+
 /* gemv.rs:6 / gemv_kernel.cu:116-136: copy a host DB (u16, column-major
  * [db_cols][db_rows]) to the device; *d_db_out = NULL on failure. */
 void gpu_gemv_upload_db(const uint16_t* h_db, uint16_t** d_db_out, size_t db_size_bytes);
@@ -145,6 +169,10 @@ void inspire_server_destroy(inspire_server_t* srv);
  * encode_kernel.cu).  host != 0: copy from host memory; host == 0: copy from
  * a device pointer. */
 int inspire_server_set_db(inspire_server_t* srv, const uint16_t* db, int host);
+/* Raw DB, column-major [db_cols][db_rows] u16 (cuckoo.rs:314-353), encoded on
+ * the device (encode_kernel.cu:263-380 with interp_degree = the shape's d,
+ * modulus 65535) into the server's DB: keyword.rs:404-424 in one call. */
+int inspire_server_encode_db(inspire_server_t* srv, const uint16_t* raw, int host);
 /* InspiRING precomputation (PrecompInsPIR, packing.rs:1718-1736) in the
  * layout of gpu_packing_online_upload_precomp: bold_t, bold_t_bar
  * [instances][3069][2048] condensed, bold_t_hat [instances][3][2048]
@@ -226,6 +254,12 @@ int inspire_synth_fill(void* d_out, uint64_t seed, uint32_t stream_id, uint64_t 
  * [nq][db_rows] condensed; bodies [nq][6][2][2048] u32. */
 int inspire_unpack_query_device(const uint8_t* d_query, int nq, int64_t db_rows, uint64_t* d_y_body,
                                 uint64_t* d_z_body, uint64_t* d_rows, uint32_t* d_bodies, void* stream);
+/* encode_kernel.cu:263-380 (gpu_encode_db) on device buffers: d_out and d_in
+ * column-major [db_cols][db_rows] u16 (d_out may equal d_in); interp_degree a
+ * power of two <= 32; db_cols a multiple of interp_degree * 2048, db_rows of
+ * 64.  Allocates its work buffer (up to the DB's size) and returns when done. */
+int inspire_encode_db_device(uint16_t* d_out, const uint16_t* d_in, int64_t db_rows, int64_t db_cols,
+                             int interp_degree, void* stream);
 /* gemv_kernel.cu:46-103 for nq query rows at once: d_c [nq][db_cols]. */
 int inspire_gemv_device(uint64_t* d_c, const uint64_t* d_rows, int nq, const uint16_t* d_db, int64_t db_rows,
                         int64_t db_cols, void* stream);

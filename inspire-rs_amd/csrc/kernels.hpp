@@ -26,6 +26,15 @@ int launch_fold_batch(uint8_t* out, int64_t out_stride, const uint64_t* a_hat, c
                       const uint64_t* P, const uint32_t* bodies, const uint32_t* neg_a, const uint32_t* tw,
                       int64_t instances, int d, int nq, int clients, hipStream_t s);
 
+// DB encode (interpolation of each row's column groups, p = 65535): in and out
+// column-major [db_cols][db_rows] (out may equal in); work holds
+// groups_per_batch column groups of work units (encode_work_bytes_per_group
+// bytes each).  d a power of two <= 32, db_rows % 64 == 0, db_cols % (d*2048)
+// == 0; returns -1 otherwise.
+size_t encode_work_bytes_per_group(int64_t db_rows, int d);
+int launch_encode_db(uint16_t* out, const uint16_t* in, int64_t db_rows, int64_t db_cols, int d, uint32_t num_inv,
+                     uint16_t* work, int64_t groups_per_batch, hipStream_t s);
+
 // Fragment-ordered copy of the DB (signed byte planes in MFMA B-operand
 // order, tiled_db_bytes bytes); needs db_rows % 64 == 0 and db_cols % 16 == 0.
 size_t tiled_db_bytes(int db_rows, int64_t db_cols);

@@ -79,6 +79,80 @@ extern "C" uint64_t gpu_available_memory_bytes(void) {
 }
 
 // ---------------------------------------------------------------------------
+// DB encode (encode.rs:3-32, encode_kernel.cu:386-510): a primary encoded DB
+// and a second slot for a rebuild that swaps in when done.
+// ---------------------------------------------------------------------------
+
+namespace {
+uint16_t* d_encoded_db = nullptr;
+uint16_t* d_encoded_db_new = nullptr;
+
+// Copies and encodes a host DB into a new device buffer; nullptr on failure.
+uint16_t* encode_from_host(const char* tag, const uint16_t* h_db_in, int db_rows, int db_cols, int interp_degree,
+                           int gamma, int c, uint32_t modulus, uint32_t num_inv) {
+  if (!h_db_in || db_rows <= 0 || db_cols <= 0 || gamma != kN || modulus != 65535 || interp_degree < 1 ||
+      (int64_t)interp_degree * gamma * c != db_cols || (uint64_t)num_inv * (uint32_t)interp_degree % modulus != 1) {
+    std::fprintf(stderr, "[%s] unsupported arguments (db_rows=%d db_cols=%d interp_degree=%d gamma=%d c=%d "
+                 "modulus=%u num_inv=%u)\n", tag, db_rows, db_cols, interp_degree, gamma, c, modulus, num_inv);
+    return nullptr;
+  }
+  const size_t bytes = (size_t)db_rows * db_cols * sizeof(uint16_t);
+  uint16_t* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e == hipSuccess) e = hipMemcpy(p, h_db_in, bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    std::fprintf(stderr, "[%s] HIP error: %s\n", tag, hipGetErrorString(e));
+    free_ptr(p);
+    return nullptr;
+  }
+  if (inspire_encode_db_device(p, p, db_rows, db_cols, interp_degree, nullptr) != INSPIRE_OK) {
+    std::fprintf(stderr, "[%s] %s\n", tag, inspire_last_error());
+    free_ptr(p);
+    return nullptr;
+  }
+  return p;
+}
+}  // namespace
+
+extern "C" void gpu_encode_db_from_host(const uint16_t* h_db_in, uint16_t*, int db_rows, int db_cols,
+                                        int interp_degree, int gamma, int c, uint32_t modulus, uint32_t num_inv) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  uint16_t* p = encode_from_host("GPU Encode", h_db_in, db_rows, db_cols, interp_degree, gamma, c, modulus, num_inv);
+  if (!p) return;
+  if (d_encoded_db && d_encoded_db != d_encoded_db_new) free_ptr(d_encoded_db);
+  d_encoded_db = p;
+}
+
+extern "C" const uint16_t* gpu_encode_get_device_db(void) { return d_encoded_db; }
+
+extern "C" void gpu_encode_free_device_db(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  free_ptr(d_encoded_db);
+  d_encoded_db = nullptr;
+}
+
+extern "C" void gpu_encode_db_from_host_fold(const uint16_t* h_db_in, int db_rows, int db_cols, int interp_degree,
+                                             int gamma, int c, uint32_t modulus, uint32_t num_inv) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  uint16_t* p =
+      encode_from_host("GPU Encode Fold", h_db_in, db_rows, db_cols, interp_degree, gamma, c, modulus, num_inv);
+  if (!p) return;
+  if (d_encoded_db_new && d_encoded_db_new != d_encoded_db) free_ptr(d_encoded_db_new);
+  d_encoded_db_new = p;
+}
+
+extern "C" const uint16_t* gpu_encode_get_device_db_fold(void) { return d_encoded_db_new; }
+
+extern "C" void gpu_encode_swap_primary(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (d_encoded_db && d_encoded_db != d_encoded_db_new) free_ptr(d_encoded_db);
+  d_encoded_db = d_encoded_db_new;
+  d_encoded_db_new = nullptr;
+}
+
+extern "C" void gpu_trim_memory_pool(void) { (void)hipDeviceSynchronize(); }
+
+// ---------------------------------------------------------------------------
 // GEMV (gemv.rs:5-25, gemv_kernel.cu:105-194)
 // ---------------------------------------------------------------------------
 

@@ -3,6 +3,8 @@
 // stage entry points declared in include/inspire_hip.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -584,6 +586,62 @@ extern "C" int inspire_server_set_stash(inspire_server_t* s, const uint8_t* entr
 
 // Stream ids of the synthetic generator (the oracle uses the same numbers).
 enum { kStreamDb = 1, kStreamBt = 2, kStreamBtb = 3, kStreamBth = 4, kStreamAhat = 5 };
+
+// d^-1 mod 65535 for the encode's power-of-two degrees.
+static uint32_t p16_inverse(int d) {
+  for (uint32_t x = 1; x < 65535; ++x)
+    if ((uint64_t)x * (uint32_t)d % 65535 == 1) return x;
+  return 0;
+}
+
+// Encodes in place (or from d_in) with a work buffer of as many column groups
+// as fit in free memory less 1 GiB (all of them when possible).
+static int encode_db(uint16_t* d_out, const uint16_t* d_in, int64_t db_rows, int64_t db_cols, int d,
+                     hipStream_t st) {
+  if (d < 1 || d > 32 || (d & (d - 1)) || db_rows <= 0 || db_rows % 64 || db_cols <= 0 ||
+      db_cols % ((int64_t)d * kN))
+    return set_error(INSPIRE_ERR_INVALID, "encode shape: interp_degree a power of two <= 32, db_rows % 64 == 0, "
+                                          "db_cols % (interp_degree * 2048) == 0");
+  const int64_t c = db_cols / ((int64_t)d * kN);
+  const size_t per_group = encode_work_bytes_per_group(db_rows, d);
+  size_t free_b = 0, total_b = 0;
+  HIP_TRY(hipMemGetInfo(&free_b, &total_b), "memory info");
+  const size_t spare = free_b > (1ull << 30) ? free_b - (1ull << 30) : 0;
+  const int64_t groups = std::min<int64_t>(c, (int64_t)(spare / per_group));
+  if (groups < 1) return set_error(INSPIRE_ERR_NOMEM, "no room for one column group of encode work units");
+  uint16_t* work = nullptr;
+  int rc = dev_alloc(&work, (size_t)groups * per_group / sizeof(uint16_t), "encode work buffer");
+  if (rc) return rc;
+  if (launch_encode_db(d_out, d_in, db_rows, db_cols, d, p16_inverse(d), work, groups, st)) {
+    (void)hipFree(work);
+    return set_error(INSPIRE_ERR_INVALID, "unsupported encode shape");
+  }
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(work);
+  if (e != hipSuccess) return set_error(INSPIRE_ERR_HIP, std::string("encode: ") + hipGetErrorString(e));
+  return INSPIRE_OK;
+}
+
+extern "C" int inspire_encode_db_device(uint16_t* d_out, const uint16_t* d_in, int64_t db_rows, int64_t db_cols,
+                                        int interp_degree, void* stream) {
+  if (!d_out || !d_in) return set_error(INSPIRE_ERR_INVALID, "null argument");
+  return encode_db(d_out, d_in, db_rows, db_cols, interp_degree, (hipStream_t)stream);
+}
+
+extern "C" int inspire_server_encode_db(inspire_server_t* s, const uint16_t* raw, int host) {
+  if (!s || !raw) return set_error(INSPIRE_ERR_INVALID, "null argument");
+  const size_t bytes = (size_t)s->shape.db_rows * s->db_cols * sizeof(uint16_t);
+  HIP_TRY(hipMemcpyAsync(s->db, raw, bytes, host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, s->stream),
+          "copy raw database");
+  int rc = encode_db(s->db, s->db, s->shape.db_rows, s->db_cols, s->shape.interpolate_degree, s->stream);
+  if (rc) return rc;
+  if (s->dbfrag) launch_tile_db(s->dbfrag, s->db, (int)s->shape.db_rows, s->db_cols, s->stream);
+  HIP_TRY(hipGetLastError(), "tile database");
+  HIP_TRY(hipStreamSynchronize(s->stream), "tile database");
+  s->have_db = true;
+  return INSPIRE_OK;
+}
 
 extern "C" int inspire_server_fill_synthetic(inspire_server_t* s, uint64_t seed) {
   if (!s) return set_error(INSPIRE_ERR_INVALID, "null server");

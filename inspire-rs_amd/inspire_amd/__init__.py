@@ -62,6 +62,8 @@ def lib():
         L.inspire_server_destroy.argtypes = [vp]
         L.inspire_server_destroy.restype = None
         L.inspire_server_set_db.argtypes = [vp, vp, i32]
+        L.inspire_server_encode_db.argtypes = [vp, vp, i32]
+        L.inspire_encode_db_device.argtypes = [vp, vp, i64, i64, i32, vp]
         L.inspire_server_set_precomp.argtypes = [vp, vp, vp, vp, vp, i32]
         L.inspire_server_set_stash.argtypes = [vp, vp, u32, u32]
         L.inspire_server_fill_synthetic.argtypes = [vp, u64]
@@ -152,6 +154,11 @@ class Server:
 
     def set_db(self, db, host=True):
         check(lib().inspire_server_set_db(self.h, _ptr(db), 1 if host else 0))
+
+    def encode_db(self, raw, host=True):
+        """Raw column-major DB in, encoded on the device into the server's DB
+        (keyword.rs:404-424)."""
+        check(lib().inspire_server_encode_db(self.h, _ptr(raw), 1 if host else 0))
 
     def set_precomp(self, bt, btb, bth, a_hat, host=True):
         check(lib().inspire_server_set_precomp(self.h, _ptr(bt), _ptr(btb), _ptr(bth), _ptr(a_hat),
@@ -276,6 +283,11 @@ def gemv_mfma_scratch_bytes(db_rows):
 def gemv_mfma_device(d_c, d_rows, nq, d_db, db_rows, db_cols, d_scratch, stream=None):
     check(lib().inspire_gemv_mfma_device(_ptr(d_c), _ptr(d_rows), nq, _ptr(d_db), db_rows, db_cols,
                                          _ptr(d_scratch), stream))
+
+
+def encode_db_device(d_out, d_in, db_rows, db_cols, interp_degree, stream=None):
+    """encode_kernel.cu:263-380 on device buffers (column-major u16)."""
+    check(lib().inspire_encode_db_device(_ptr(d_out), _ptr(d_in), db_rows, db_cols, interp_degree, stream))
 
 
 def packing_device(d_S, d_bt, d_btb, d_bth, d_y, d_z, instances, stream=None):

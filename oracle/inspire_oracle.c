@@ -21,6 +21,9 @@
  *   - Gadget: KAT gadget.rs:79-96.
  *   - Rescale / bit I/O: exact rational definition, python.
  *   - Fold: one external product against a schoolbook negacyclic product.
+ *   - DB encode: evaluating the encoded polynomials at Y = X^(2g*j/d)
+ *     recovers the raw polynomials (python, direct O(d^2) evaluation); d = 1
+ *     and d = 2 closed forms.
  *   - ChaCha20 block function: RFC 8439 section 2.3.2 / A.1 vectors.
  *     The rand_chacha word order (64-bit counter in words 12-13, u64 = lo|hi<<32)
  *     is parity unpinned: nothing in the reference's tests fixes it.
@@ -452,6 +455,72 @@ void orc_rgsw_neg_a_ntt(uint64_t* out, uint64_t* a_out) {
             out[c * N + i] = barrett_raw_u64(neg, C.cr1[c], MODULI[c]); /* poly.rs:857-863 */
         }
         ntt_forward_one(out + c * N, c);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* DB encode (encode_kernel.cu:18-228, driven by gpu_encode_db at       */
+/* encode_kernel.cu:263-380; called from keyword.rs:404-420)           */
+/* ------------------------------------------------------------------ */
+
+/* Per DB row r and column group wp (c = db_cols / (d * gamma) groups), the d
+ * polynomials P_j[k] = raw[col = (wp*d + j)*gamma + k][r] of Z_p[X]/(X^gamma + 1)
+ * are interpolated: placed at bit-reversed positions (gather_tiled_kernel,
+ * encode_kernel.cu:150-196), run through log2(d) radix-2 levels whose
+ * twiddle is the monomial X^e, e = (2g - 2g*i/size) mod 2g for butterfly i of
+ * a group of `size` (compute_twiddle_exponents, encode_kernel.cu:220-228;
+ * butterfly_kernel, encode_kernel.cu:18-76), then scaled by num_inv
+ * (scale_scatter_tiled_kernel, encode_kernel.cu:85-146) and written back to
+ * the same columns, column-major [db_cols][db_rows].  p = modulus (65535);
+ * every output is canonical in [0, p). */
+void orc_encode(const uint16_t* raw, uint16_t* out, int64_t db_rows, int64_t db_cols, int d, int gamma,
+                uint32_t modulus, uint32_t num_inv) {
+    int logd = 0;
+    while ((1 << logd) < d) logd++;
+    const int64_t c = db_cols / ((int64_t)d * gamma);
+    const int two_g = 2 * gamma;
+#pragma omp parallel
+    {
+        uint32_t* a = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)d * gamma);
+        uint32_t* b = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)d * gamma);
+#pragma omp for schedule(dynamic, 1) collapse(2)
+        for (int64_t r = 0; r < db_rows; r++) {
+            for (int64_t wp = 0; wp < c; wp++) {
+                for (int j = 0; j < d; j++) {
+                    const int dj = (int)rev_bits((uint32_t)j, logd);
+                    for (int k = 0; k < gamma; k++)
+                        a[(size_t)dj * gamma + k] = raw[(((wp * d + j) * (int64_t)gamma) + k) * db_rows + r] % modulus;
+                }
+                for (int level = 0; level < logd; level++) {
+                    const int size = 1 << (level + 1), half = 1 << level;
+                    for (int g = 0; g < d / size; g++) {
+                        for (int i = 0; i < half; i++) {
+                            const int e = (two_g - two_g * i / size) % two_g;
+                            const uint32_t* ev = a + (size_t)(g * size + i) * gamma;
+                            const uint32_t* od = a + (size_t)(g * size + i + half) * gamma;
+                            uint32_t* oe = b + (size_t)(g * size + i) * gamma;
+                            uint32_t* oo = b + (size_t)(g * size + i + half) * gamma;
+                            for (int k = 0; k < gamma; k++) {
+                                /* coefficient k of X^e * odd in Z_p[X]/(X^gamma + 1) */
+                                const int src = ((k - e) % two_g + two_g) % two_g;
+                                const uint32_t t = src < gamma ? od[src] : (modulus - od[src - gamma]) % modulus;
+                                oe[k] = (ev[k] + t) % modulus;
+                                oo[k] = (ev[k] + modulus - t) % modulus;
+                            }
+                        }
+                    }
+                    uint32_t* tmp = a;
+                    a = b;
+                    b = tmp;
+                }
+                for (int j = 0; j < d; j++)
+                    for (int k = 0; k < gamma; k++)
+                        out[(((wp * d + j) * (int64_t)gamma) + k) * db_rows + r] =
+                            (uint16_t)(((uint64_t)a[(size_t)j * gamma + k] * num_inv) % modulus);
+            }
+        }
+        free(a);
+        free(b);
     }
 }
 

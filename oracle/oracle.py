@@ -311,3 +311,15 @@ def synth_query(seed, db_rows, nq=2, small=False):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+def encode(raw, db_rows, db_cols, d, gamma=N, modulus=65535):
+    """encode_kernel.cu:18-228 (gpu_encode_db): interpolation of each DB row's
+    column groups; raw and the result are column-major u16 [db_cols][db_rows]."""
+    raw = np.ascontiguousarray(raw, dtype=np.uint16).reshape(-1)
+    assert raw.size == db_rows * db_cols and db_cols % (d * gamma) == 0
+    num_inv = pow(d, -1, modulus)
+    out = np.zeros_like(raw)
+    lib().orc_encode(_p(raw), _p(out), ctypes.c_int64(db_rows), ctypes.c_int64(db_cols), ctypes.c_int(d),
+                     ctypes.c_int(gamma), ctypes.c_uint32(modulus), ctypes.c_uint32(num_inv))
+    return out

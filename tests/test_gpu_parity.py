@@ -469,3 +469,119 @@ def test_engines_on_two_databases(torch):
         e.close()
     for srv in srvs:
         srv.close()
+
+
+# --- DB encode (encode_kernel.cu:18-228) -------------------------------------
+
+@pytest.mark.parametrize("db_rows,d,groups", [(64, 1, 2), (64, 2, 3), (128, 8, 2), (64, 32, 2)])
+def test_encode_matches_oracle(torch, db_rows, d, groups):
+    say("encode", db_rows, d, groups)
+    db_cols = groups * d * N
+    rng = np.random.default_rng(db_rows + d)
+    raw = rng.integers(0, 65536, size=db_rows * db_cols, dtype=np.uint16)
+    raw[::131] = 65535  # non-canonical words (0 mod p)
+    exp = O.encode(raw, db_rows, db_cols, d)
+    dr = dev_from(torch, raw)
+    out = empty(torch, raw.size, np.uint16)
+    IA.encode_db_device(out, dr, db_rows, db_cols, d)
+    assert (host(torch, out, np.uint16) == exp).all()
+    IA.encode_db_device(dr, dr, db_rows, db_cols, d)  # in place
+    assert (host(torch, dr, np.uint16) == exp).all()
+
+
+def test_encode_rejects_bad_shapes(torch):
+    say("encode bad shapes")
+    dr = empty(torch, 64 * 4 * N, np.uint16)
+    for rows, cols, d in [(64, 3 * N, 3), (60, 4 * N, 2), (64, 3 * N, 2), (64, 64 * N, 64)]:
+        with pytest.raises(IA.InspireError):
+            IA.encode_db_device(dr, dr, rows, cols, d)
+
+
+def test_server_encode_db_equals_set_db_of_oracle_encoding(torch):
+    say("server encode_db 2048x8 d4")
+    db_rows, inst, d = 2048, 8, 4
+    st = O.synth_state(23, db_rows, inst)
+    rng = np.random.default_rng(29)
+    raw = rng.integers(0, 65536, size=db_rows * inst * N, dtype=np.uint16)
+    enc = O.encode(raw, db_rows, inst * N, d)
+    y, z, rows, bodies = O.synth_query(24, db_rows, nq=2)
+    q = O.serialize_query(y, z, rows, bodies)
+    out = []
+    for how in ("encode", "set"):
+        srv = IA.Server(db_rows, inst, d)
+        if how == "encode":
+            srv.encode_db(raw)
+        else:
+            srv.set_db(enc)
+        srv.set_precomp(st[1], st[2], st[3], st[4])
+        out.append(srv.respond_serialized(q))
+        srv.close()
+    assert out[0] == out[1]
+    exp = O.respond_raw(enc, db_rows, inst, d, st[1], st[2], st[3], st[4], y, z, rows, bodies)
+    assert out[0] == O.serialize_response(exp, [])
+
+
+def test_encode_full_size_column_group_matches_oracle(torch):
+    # The 1M-key shape: db_rows 2048, 720,896 columns, d = 32 (11 column
+    # groups); group 7 is checked against the oracle.
+    say("encode 1M shape")
+    db_rows, d, c = 2048, 32, 11
+    db_cols = c * d * N
+    raw = empty(torch, db_rows * db_cols, np.uint16)
+    IA.synth_fill(raw, 19, IA.STREAM_DB, db_rows * db_cols, IA.KIND_U16)
+    out = empty(torch, db_rows * db_cols, np.uint16)
+    IA.encode_db_device(out, raw, db_rows, db_cols, d)
+    wp = 7
+    lo, hi = wp * d * N * db_rows, (wp + 1) * d * N * db_rows
+    slab = host(torch, raw[lo:hi], np.uint16)
+    exp = O.encode(slab, db_rows, d * N, d)
+    assert (host(torch, out[lo:hi], np.uint16) == exp).all()
+    del raw, out
+    torch.cuda.empty_cache()
+
+
+def test_legacy_encode_then_gemv(torch):
+    # keyword.rs:404-424: gpu_encode_db_from_host, then the GEMV on the device
+    # DB it keeps (gpu_encode_get_device_db); the fold slot and swap.
+    say("legacy encode + gemv")
+    import ctypes
+    L = IA.lib()
+    L.gpu_encode_get_device_db.restype = ctypes.c_void_p
+    L.gpu_encode_get_device_db_fold.restype = ctypes.c_void_p
+    db_rows, d, c = 2048, 4, 2
+    db_cols = c * d * N
+    rng = np.random.default_rng(41)
+    raws = [rng.integers(0, 65536, size=db_rows * db_cols, dtype=np.uint16) for _ in range(2)]
+    a = O.synth(83, O.STREAM_ROW, db_rows, O.KIND_CONDENSED)
+    k = O.constants()
+    u = ctypes.c_uint64
+    vp = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+
+    def gemv(ptr):
+        out = np.zeros(db_cols, dtype=np.uint64)
+        L.gpu_gemv_query(vp(out), vp(a), ctypes.c_void_p(ptr), db_rows, db_cols, u(O.Q0), u(O.Q1),
+                         u(k["cr1_q0"]), u(k["cr1_q1"]), u(k["mod0_inv_mod1"]), u(k["mod1_inv_mod0"]), u(O.Q),
+                         u(k["cr0_q"]), u(k["cr1_q"]))
+        return out
+
+    num_inv = pow(d, -1, 65535)
+    L.gpu_encode_db_from_host(vp(raws[0]), None, db_rows, db_cols, d, N, c, ctypes.c_uint32(65535),
+                              ctypes.c_uint32(num_inv))
+    p0 = L.gpu_encode_get_device_db()
+    assert p0
+    exp0 = O.gemv(a, O.encode(raws[0], db_rows, db_cols, d), db_rows, db_cols)
+    assert (gemv(p0) == exp0).all()
+    L.gpu_encode_db_from_host_fold(vp(raws[1]), db_rows, db_cols, d, N, c, ctypes.c_uint32(65535),
+                                   ctypes.c_uint32(num_inv))
+    p1 = L.gpu_encode_get_device_db_fold()
+    assert p1 and L.gpu_encode_get_device_db() == p0
+    L.gpu_encode_swap_primary()
+    assert L.gpu_encode_get_device_db() == p1 and not L.gpu_encode_get_device_db_fold()
+    assert (gemv(p1) == O.gemv(a, O.encode(raws[1], db_rows, db_cols, d), db_rows, db_cols)).all()
+    # a wrong num_inv is refused, leaving the primary in place
+    L.gpu_encode_db_from_host(vp(raws[0]), None, db_rows, db_cols, d, N, c, ctypes.c_uint32(65535),
+                              ctypes.c_uint32(num_inv + 1))
+    assert L.gpu_encode_get_device_db() == p1
+    L.gpu_encode_free_device_db()
+    assert not L.gpu_encode_get_device_db()
+    L.gpu_trim_memory_pool()

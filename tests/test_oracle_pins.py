@@ -349,3 +349,62 @@ def test_respond_raw_small_is_deterministic_digest(tmp_path):
         y, z, rows, bodies = O.synth_query(case["query_seed"], db_rows, nq=case["nq"], small=case["small"])
         resp = O.respond_raw(st[0], db_rows, inst, d, st[1], st[2], st[3], st[4], y, z, rows, bodies)
         assert [hashlib.sha256(r).hexdigest() for r in resp] == case["sha256"]
+
+
+# --- DB encode (encode_kernel.cu:18-228) ------------------------------------
+
+P16 = 65535
+
+
+def _mono_mul(poly, e, g):
+    """X^e * poly in Z_p[X]/(X^g + 1), e taken mod 2g (X^g = -1)."""
+    e %= 2 * g
+    sign = 1
+    if e >= g:
+        e -= g
+        sign = -1
+    out = np.zeros(g, dtype=np.int64)
+    out[e:] = poly[:g - e]
+    out[:e] = -poly[g - e:]
+    return (sign * out) % P16
+
+
+def _cols(flat, rows, cols):
+    # column-major [cols][rows] -> array indexed [col, row]
+    return flat.reshape(cols, rows).astype(np.int64)
+
+
+@pytest.mark.parametrize("d,g,rows,groups", [(4, 16, 3, 2), (8, 8, 2, 1), (2, 32, 2, 3)])
+def test_encode_evaluation_recovers_raw(d, g, rows, groups):
+    # Encoded polynomials E_t of a (row, group) interpolate the raw ones:
+    # sum_t E_t * X^(2g*j*t/d) = P_j in Z_p[X]/(X^g + 1), evaluated directly
+    # (O(d^2) monomial products), independent of the butterfly order.
+    cols = groups * d * g
+    rng = np.random.default_rng(d * 100 + g)
+    raw = rng.integers(0, 65536, size=rows * cols, dtype=np.uint16)
+    raw[::97] = 65535  # non-canonical words reduce to 0 (mod p)
+    enc = _cols(O.encode(raw, rows, cols, d, g), rows, cols)
+    r = _cols(raw, rows, cols) % P16
+    assert enc.max() < P16
+    for row in range(rows):
+        for wp in range(groups):
+            E = [enc[(wp * d + t) * g:(wp * d + t + 1) * g, row] for t in range(d)]
+            for j in range(d):
+                acc = np.zeros(g, dtype=np.int64)
+                for t in range(d):
+                    acc = (acc + _mono_mul(E[t], 2 * g * j * t // d, g)) % P16
+                assert np.array_equal(acc, r[(wp * d + j) * g:(wp * d + j + 1) * g, row]), (row, wp, j)
+
+
+def test_encode_degree_one_and_two_closed_forms():
+    g, rows = 16, 2
+    rng = np.random.default_rng(3)
+    raw = rng.integers(0, 65536, size=rows * g, dtype=np.uint16)
+    raw[5] = 65535
+    assert np.array_equal(O.encode(raw, rows, g, 1, g).astype(np.int64), raw.astype(np.int64) % P16)
+    raw2 = rng.integers(0, 65536, size=rows * 2 * g, dtype=np.uint16)
+    enc = _cols(O.encode(raw2, rows, 2 * g, 2, g), rows, 2 * g)
+    r = _cols(raw2, rows, 2 * g) % P16
+    half = pow(2, -1, P16)
+    assert np.array_equal(enc[:g], (r[:g] + r[g:]) * half % P16)
+    assert np.array_equal(enc[g:], (r[:g] - r[g:]) % P16 * half % P16)
