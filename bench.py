@@ -140,13 +140,20 @@ def cpu_baseline(plan, out_samples=32, fold_chains=2):
     t_fold = (time.perf_counter() - t0) / fold_chains
     scale = inst / J
     per_query = (t_pack + t_post + t_gemv) * scale + t_fold * 2 * plan["c"]
+    # DB encode: 64 rows of one column group, scaled to the whole DB.
+    er = 64
+    raw = O.synth(seed, O.STREAM_DB, er * d * N, O.KIND_U16)
+    t0 = time.perf_counter()
+    O.encode(raw, er, d * N, d)
+    t_enc = (time.perf_counter() - t0) * (rows / er) * plan["c"]
     sample = ("packing+INTT for %d of %d outputs, GEMV of %d of %d columns x 2 hash queries, "
               "fold+modswitch of %d of %d chains; linearly scaled to one keyword query"
               % (J, inst, J * N, plan["db_cols"], fold_chains, 2 * plan["c"]))
     return {"value": 1.0 / per_query, "unit": "queries/s", "cores": O.num_threads(), "kind": "port",
             "sample": sample, "seconds_per_query": per_query,
             "stage_s": {"packing": t_pack * scale, "post": t_post * scale, "gemv": t_gemv * scale,
-                        "fold": t_fold * 2 * plan["c"]}}
+                        "fold": t_fold * 2 * plan["c"]},
+            "encode_s": t_enc, "encode_sample": "64 of %d rows of 1 of %d column groups, scaled" % (rows, plan["c"])}
 
 
 def profiled_traffic(kernel_prefix):
@@ -181,6 +188,7 @@ def main():
     ap.add_argument("--rates", default="100,500,1000,2000,3000",
                     help="Poisson arrival rates (q/s) for the serving run ('' to skip)")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrivals per rate span about this long")
+    ap.add_argument("--no-encode", action="store_true", help="skip the DB encode measurement")
     ap.add_argument("--packing", choices=("mfma", "valu"), default="mfma",
                     help="single-query packing kernel: int8 MFMA over the fragment-ordered bold_t, or the VALU "
                          "kernel over the reference layout")
@@ -356,6 +364,24 @@ def main():
         log("batched MFMA at B=%d: packing %.0f TOP/s (%.1f%%), gemv %.0f TOP/s (%.1f%%)" % (
             B, result["batched_roofline"]["packing"]["achieved"], 100 * result["batched_roofline"]["packing"]["frac"],
             result["batched_roofline"]["gemv"]["achieved"], 100 * result["batched_roofline"]["gemv"]["frac"]))
+
+    # Setup: the DB encode (encode_kernel.cu) of a raw DB of the same shape,
+    # device to device, in place; one synchronous call (work buffer included).
+    if not args.no_encode:
+        ncell = db_rows * plan["db_cols"]
+        raw = torch.empty(ncell, dtype=torch.int16, device="cuda")
+        IA.synth_fill(raw, 31, IA.STREAM_DB, ncell, IA.KIND_U16)
+        IA.encode_db_device(raw, raw, db_rows, plan["db_cols"], d)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        IA.encode_db_device(raw, raw, db_rows, plan["db_cols"], d)
+        enc_s = time.perf_counter() - t0
+        del raw
+        torch.cuda.empty_cache()
+        result["setup"] = {"encode_ms": enc_s * 1e3, "encode_db_bytes": 2 * ncell,
+                           "encode_gbs": 2 * ncell * 2 / enc_s / 1e9,
+                           "note": "raw DB read once and encoded DB written once (2 x 2.95 GB) over the call's wall time"}
+        log("encode %.1f ms (%.0f GB/s of read+write DB bytes)" % (enc_s * 1e3, 4 * ncell / enc_s / 1e9))
 
     # Serving: continuous batching under open-loop Poisson arrivals.
     serving = []
