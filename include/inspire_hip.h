@@ -226,6 +226,31 @@ int inspire_server_set_max_batch(inspire_server_t* srv, int max_batch);
 int inspire_respond_batch_device(inspire_server_t* srv, const uint8_t* d_queries, size_t q_stride, size_t len,
                                  int nq, int count, uint8_t* d_out, size_t out_stride, void* stream);
 
+/* Batched kernel variants.  packing_variant: tile configuration of the
+ * 64-client-group packing kernel (0..3); gemv_variant: 0 = query and DB
+ * fragments shared through LDS, 1 = query fragments from L2 (0..1, when the
+ * server holds the fragment-ordered DB); fold_variant: 0 = by launch size,
+ * 1 = one 1024-thread workgroup per chain (both primes at once), 2 = 512
+ * threads, primes in turn.  All variants give identical results. */
+int inspire_server_set_batch_kernels(inspire_server_t* srv, int packing_variant, int gemv_variant, int fold_variant);
+
+/* Autotuner.  Times every batched kernel variant on this device for `count`
+ * keyword queries of nq hash queries (reps runs each, after a warm-up), keeps
+ * the fastest of each stage for the server's batched respond, then measures
+ * the batch stages' throughput at batch sizes 1, 2, 4, ... up to max_batch and
+ * suggests the smallest batch within 10% of the best.  Needs set_max_batch
+ * >= count; uses the batch buffers (no other call on the server meanwhile). */
+typedef struct {
+  int packing_variant, gemv_variant, fold_variant; /* chosen */
+  int packing_variants, gemv_variants;             /* variants timed */
+  double packing_ms[8], gemv_ms[8], fold_ms[3];    /* per variant (fold: [1], [2]) */
+  int nbatch;
+  int batch_sizes[16];
+  double batch_qps[16]; /* keyword queries per second of the batch stages */
+  int suggested_batch;
+} inspire_tune_t;
+int inspire_server_autotune(inspire_server_t* srv, int count, int nq, int reps, inspire_tune_t* out);
+
 /* First-pass kernel used by the server: INSPIRE_GEMV_MFMA (default, exact
  * int8-limb MFMA) or INSPIRE_GEMV_VALU (64-bit multiply-accumulate). */
 #define INSPIRE_GEMV_VALU 0

@@ -511,9 +511,14 @@ size_t gemv_mfma_batch_scratch_bytes(int db_rows, int count) {
   return (size_t)count * ((size_t)(db_rows / 64) * 64 * 16 + 16 * sizeof(int32_t));
 }
 
+int gemv_batch_variants() { return 2; }
+const char* gemv_batch_variant_name(int v) {
+  return v == 0 ? "LDS-shared: 8 waves, 16 queries x 8 tiles" : v == 1 ? "L2 query fragments: 4 waves, 16 queries x 4 tiles" : "";
+}
+
 int launch_gemv_mfma_batch(uint64_t* c, int64_t c_client_stride, const uint64_t* rows, int64_t rows_client_stride,
                            int nq, const uint16_t* db, const void* dbfrag, int db_rows, int64_t db_cols, int count,
-                           void* scratch, hipStream_t s) {
+                           void* scratch, hipStream_t s, int variant) {
   if (nq < 1 || nq > 2 || count < 1 || db_rows <= 0 || db_rows % 256 != 0 || db_rows > (1 << 17) || db_cols <= 0)
     return -1;
   int8_t* afrag = static_cast<int8_t*>(scratch);
@@ -529,7 +534,11 @@ int launch_gemv_mfma_batch(uint64_t* c, int64_t c_client_stride, const uint64_t*
   const int ngroups = (count + 4 * kCpw - 1) / (4 * kCpw);
   const int64_t col_blocks = (db_cols + 16 * kTw - 1) / (16 * kTw);
   const int64_t blocks = (col_blocks + 7) / 8 * 8 * ngroups;
-  if (dbfrag && db_cols % 16 == 0) {
+  if (dbfrag && db_cols % 16 == 0 && variant == 1) {
+    hipLaunchKernelGGL((k_gemv_wide<kCpw, kTw, 2, true>), dim3((unsigned)blocks), dim3(256), 0, s, c, c_client_stride,
+                       afrag, rsum, db, static_cast<const u4*>(dbfrag), nq, db_rows, db_cols, count, ngroups,
+                       col_blocks);
+  } else if (dbfrag && db_cols % 16 == 0) {
     // 16 queries x 8 column tiles per block: 8 waves of 4 queries x 4 tiles,
     // a 3-step LDS ring of 32 KiB per step, one block per CU.
     constexpr int kQW = 4, kTW = 4, kWQ = 4, kWT = 2, kD = 3;

@@ -794,12 +794,12 @@ size_t fold_lds_bytes() { return (size_t)(12 + 4) * kNPad * sizeof(uint32_t); }
 int launch_fold(uint8_t* out, const uint64_t* a_hat, const uint64_t* cvec, const uint64_t* P,
                 const uint32_t* bodies, const uint32_t* neg_a, const uint32_t* tw, int64_t instances, int d,
                 int nq, hipStream_t s) {
-  return launch_fold_batch(out, 0, a_hat, cvec, P, bodies, neg_a, tw, instances, d, nq, 1, s);
+  return launch_fold_batch(out, 0, a_hat, cvec, P, bodies, neg_a, tw, instances, d, nq, 1, s, 0);
 }
 
 int launch_fold_batch(uint8_t* out, int64_t out_stride, const uint64_t* a_hat, const uint64_t* cvec,
                       const uint64_t* P, const uint32_t* bodies, const uint32_t* neg_a, const uint32_t* tw,
-                      int64_t instances, int d, int nq, int clients, hipStream_t s) {
+                      int64_t instances, int d, int nq, int clients, hipStream_t s, int variant) {
   if (d <= 0 || instances % d != 0 || nq <= 0 || clients <= 0) return -1;
   FoldArgs a{out, a_hat, cvec, P, bodies, neg_a, tw, instances, out_stride, d, (int)(instances / d), nq};
   static bool attr_set = false;
@@ -812,7 +812,8 @@ int launch_fold_batch(uint8_t* out, int64_t out_stride, const uint64_t* a_hat, c
   const int blocks = clients * nq * a.c;
   // Below two workgroups per CU the 1024-thread form (both primes at once)
   // has the shorter chain latency; above, three k_fold_seq per CU win.
-  if (blocks >= kFoldSeqMinBlocks)
+  // variant 1 / 2 force k_fold / k_fold_seq.
+  if (variant == 2 || (variant != 1 && blocks >= kFoldSeqMinBlocks))
     hipLaunchKernelGGL(k_fold_seq, dim3(blocks), dim3(kFold2Threads), 0, s, a);
   else
     hipLaunchKernelGGL(k_fold, dim3(blocks), dim3(kFoldThreads), fold_lds_bytes(), s, a);

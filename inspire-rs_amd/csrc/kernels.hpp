@@ -24,7 +24,7 @@ void launch_unpack_query_batch(const uint8_t* q, int64_t q_stride, int nq, int d
 // Fold for `clients` keyword queries; client b's responses at out + b*out_stride.
 int launch_fold_batch(uint8_t* out, int64_t out_stride, const uint64_t* a_hat, const uint64_t* cvec,
                       const uint64_t* P, const uint32_t* bodies, const uint32_t* neg_a, const uint32_t* tw,
-                      int64_t instances, int d, int nq, int clients, hipStream_t s);
+                      int64_t instances, int d, int nq, int clients, hipStream_t s, int variant = 0);
 
 // DB encode (interpolation of each row's column groups, p = 65535): in and out
 // column-major [db_cols][db_rows] (out may equal in); work holds
@@ -51,7 +51,9 @@ int launch_gemv_mfma(uint64_t* c, const uint64_t* rows, int nq, const uint16_t* 
 size_t gemv_mfma_batch_scratch_bytes(int db_rows, int count);
 int launch_gemv_mfma_batch(uint64_t* c, int64_t c_client_stride, const uint64_t* rows, int64_t rows_client_stride,
                            int nq, const uint16_t* db, const void* dbfrag, int db_rows, int64_t db_cols, int count,
-                           void* scratch, hipStream_t s);
+                           void* scratch, hipStream_t s, int variant = 0);
+int gemv_batch_variants();
+const char* gemv_batch_variant_name(int v);
 
 // Tiled int8-limb key-switch material for the MFMA packing (packing_mfma.hip),
 // rotation terms in key-slot order per NTT slot; dlog: DeviceConstants::slot_dlog.
@@ -66,7 +68,10 @@ size_t packing_afrag_bytes(int count);
 // packing_afrag_bytes(count) bytes.  k_post_zmajor adds the z_body terms.
 int launch_packing_mfma(uint32_t* S, int64_t s_client_stride, const uint8_t* tiled, uint8_t* afrag,
                         const uint64_t* y_body, int64_t key_client_stride, int count, int64_t instances,
-                        hipStream_t s);
+                        hipStream_t s, int variant = 0);
+// Tile configurations of the 64-client-group packing kernel (variant index).
+int packing_wide_variants();
+const char* packing_wide_variant_name(int v);
 
 // c: [nq][db_cols] canonical mod q; rows: [nq][db_rows] condensed; db: [db_cols][db_rows].
 int launch_gemv(uint64_t* c, const uint64_t* rows, int nq, const uint16_t* db, int db_rows, int64_t db_cols,

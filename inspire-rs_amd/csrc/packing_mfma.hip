@@ -431,9 +431,20 @@ static void launch_wide(PackArgs a, hipStream_t s) {
   hipLaunchKernelGGL((k_packing_wide<MT, NTW, W, D, WPE, WM>), dim3((unsigned)blocks), dim3(64 * W), 0, s, a);
 }
 
+// Tile configurations for 64-client groups (MT, NTW, W, D, WPE, WM); the
+// server's autotuner times them on the device and keeps the fastest.
+const char* const kWideVariantNames[] = {"16x2 tiles, 11 waves, ring 2, 3 waves/SIMD",
+                                         "16x2 tiles, 4 waves, ring 3, 2 blocks/CU",
+                                         "16x1 tiles, 8 waves, ring 3, 2 blocks/CU",
+                                         "16x2 tiles, 8 waves, ring 6, 1 block/CU"};
+int packing_wide_variants() { return 4; }
+const char* packing_wide_variant_name(int v) {
+  return v >= 0 && v < packing_wide_variants() ? kWideVariantNames[v] : "";
+}
+
 int launch_packing_mfma(uint32_t* S, int64_t s_client_stride, const uint8_t* tiled, uint8_t* afrag,
                         const uint64_t* y_body, int64_t key_client_stride, int count, int64_t instances,
-                        hipStream_t s) {
+                        hipStream_t s, int variant) {
   if (count < 1) return -1;
   const int bpad = batch_key_pad(count);
   const int64_t total = 2 * (int64_t)(bpad / 4) * kKSteps * 64;
@@ -446,10 +457,16 @@ int launch_packing_mfma(uint32_t* S, int64_t s_client_stride, const uint8_t* til
     launch_small<4, 44>(a, s);    // 4 M-tiles x 44 N-tiles
   else if (bpad == 32)
     launch_wide<8, 4, 4, 6, 1, 1>(a, s);     // one 32-client group: 4 waves of 8 x 4 tiles
-  else
-    // 64-client groups: 11 waves of 16 x 2 tiles sharing one client-key ring,
-    // 2-step B ring, 3 waves per SIMD; 88 N-tiles = 4 groups of 22.
-    launch_wide<16, 2, 11, 2, 3, 1>(a, s);
+  else {
+    // 64-client groups.  Variant 0: 11 waves of 16 x 2 tiles sharing one
+    // client-key ring, 2-step B ring, 3 waves per SIMD; 88 N-tiles = 4 groups.
+    switch (variant) {
+      case 1: launch_wide<16, 2, 4, 3, 2, 1>(a, s); break;
+      case 2: launch_wide<16, 1, 8, 3, 4, 1>(a, s); break;
+      case 3: launch_wide<16, 2, 8, 6, 2, 1>(a, s); break;
+      default: launch_wide<16, 2, 11, 2, 3, 1>(a, s); break;
+    }
+  }
   return 0;
 }
 

@@ -262,6 +262,8 @@ struct inspire_server {
   int gemv_kind = INSPIRE_GEMV_MFMA;
   int packing_kind = INSPIRE_PACKING_VALU;
   uint8_t* tiled = nullptr;      // fragment-ordered int8 limbs of bold_t (MFMA packing)
+  // batched kernel variants (set by the autotuner or inspire_server_set_batch_kernels)
+  int pack_variant = 0, gemv_variant = 0, fold_variant = 0;
   bool tiled_valid = false;
   Batch batch;
   std::vector<uint8_t> host_out;
@@ -469,6 +471,129 @@ extern "C" int inspire_server_set_max_batch(inspire_server_t* s, int max_batch) 
   return ensure_tiled(s);
 }
 
+extern "C" int inspire_server_set_batch_kernels(inspire_server_t* s, int packing_variant, int gemv_variant,
+                                                int fold_variant) {
+  if (!s || packing_variant < 0 || packing_variant >= packing_wide_variants() || gemv_variant < 0 ||
+      gemv_variant >= gemv_batch_variants() || fold_variant < 0 || fold_variant > 2)
+    return set_error(INSPIRE_ERR_INVALID, "unknown kernel variant");
+  s->pack_variant = packing_variant;
+  s->gemv_variant = gemv_variant;
+  s->fold_variant = fold_variant;
+  return INSPIRE_OK;
+}
+
+// Times each batched kernel variant on the device for `count` keyword queries
+// of nq hash queries (the batch buffers' contents: the kernels' work and
+// memory traffic do not depend on the values), keeps the fastest of each
+// stage, then measures the batch-stage throughput at batch sizes 1, 2, 4, ...
+// up to max_batch with those kernels and suggests the smallest batch within
+// 10% of the best throughput.
+extern "C" int inspire_server_autotune(inspire_server_t* s, int count, int nq, int reps, inspire_tune_t* out) {
+  if (!s || !out) return set_error(INSPIRE_ERR_INVALID, "null argument");
+  if (!s->have_db || !s->have_precomp) return set_error(INSPIRE_ERR_STATE, "database or precomputation not loaded");
+  if (count < 1 || count > s->batch.max) return set_error(INSPIRE_ERR_INVALID, "count must be in [1, max_batch]");
+  if (nq < 1 || nq > 2 || nq > s->shape.max_queries) return set_error(INSPIRE_ERR_INVALID, "hash queries per query");
+  if (reps < 1) reps = 1;
+  if (!s->tiled_valid) {
+    int rc = ensure_tiled(s);
+    if (rc) return rc;
+  }
+  *out = inspire_tune_t{};
+  Batch& b = s->batch;
+  hipStream_t st = s->stream;
+  const int64_t rows = s->shape.db_rows, inst = s->shape.instances;
+  const size_t blob = (size_t)nq * s->c * kRespBytes;
+  uint8_t* d_out = nullptr;
+  int rc = dev_alloc(&d_out, blob * s->batch.max, "autotune output");
+  if (rc) return rc;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    (void)hipFree(d_out);
+    return set_error(INSPIRE_ERR_HIP, "hipEventCreate failed");
+  }
+  auto pack = [&](int n, int v) {
+    launch_packing_mfma(b.S, inst * 2 * kN, s->tiled, b.keys, b.y_body, (int64_t)kTExp * kN, n, inst, st, v);
+  };
+  auto gemv = [&](int n, int v) {
+    (void)launch_gemv_mfma_batch(b.cvec, (int64_t)nq * s->db_cols, b.rows, (int64_t)nq * rows, nq, s->db, s->dbfrag,
+                                 (int)rows, s->db_cols, n, b.gemv_scratch, st, v);
+  };
+  auto fold = [&](int n, int v) {
+    (void)launch_fold_batch(d_out, (int64_t)blob, s->a_hat, b.cvec, b.P, b.bodies, s->consts->neg_a,
+                            s->consts->tw, inst, s->shape.interpolate_degree, nq, n, st, v);
+  };
+  // Average milliseconds of `reps` runs after one warm-up; -1 on a HIP error.
+  auto timed = [&](auto&& fn) -> double {
+    fn();
+    if (hipEventRecord(e0, st) != hipSuccess) return -1;
+    for (int r = 0; r < reps; ++r) fn();
+    if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return -1;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1;
+    return (double)ms / reps;
+  };
+  double best;
+  rc = INSPIRE_OK;
+  (void)hipStreamWaitEvent(st, b.done, 0);
+  // the packing variants differ only for groups of 64 clients
+  out->packing_variants = batch_key_pad(count) >= 64 ? packing_wide_variants() : 1;
+  best = 1e30;
+  for (int v = 0; v < out->packing_variants && v < 8; ++v) {
+    out->packing_ms[v] = timed([&] { pack(count, v); });
+    if (out->packing_ms[v] < 0) rc = INSPIRE_ERR_HIP;
+    if (out->packing_ms[v] >= 0 && out->packing_ms[v] < best) best = out->packing_ms[v], out->packing_variant = v;
+  }
+  out->gemv_variants = s->dbfrag ? gemv_batch_variants() : 1;
+  best = 1e30;
+  for (int v = 0; v < out->gemv_variants && v < 8; ++v) {
+    out->gemv_ms[v] = timed([&] { gemv(count, v); });
+    if (out->gemv_ms[v] < 0) rc = INSPIRE_ERR_HIP;
+    if (out->gemv_ms[v] >= 0 && out->gemv_ms[v] < best) best = out->gemv_ms[v], out->gemv_variant = v;
+  }
+  best = 1e30;
+  for (int v = 1; v <= 2; ++v) {
+    out->fold_ms[v] = timed([&] { fold(count, v); });
+    if (out->fold_ms[v] < 0) rc = INSPIRE_ERR_HIP;
+    if (out->fold_ms[v] >= 0 && out->fold_ms[v] < best) best = out->fold_ms[v], out->fold_variant = v;
+  }
+  if (rc == INSPIRE_OK) {
+    s->pack_variant = out->packing_variant;
+    s->gemv_variant = out->gemv_variant;
+    s->fold_variant = out->fold_variant;
+    // batch-size sweep with the chosen kernels (packing, post, GEMV, fold)
+    double top = 0;
+    for (int n = 1; n <= s->batch.max && out->nbatch < 16; n *= 2) {
+      const double ms = timed([&] {
+        pack(n, s->pack_variant);
+        launch_post_zmajor(b.P, b.S, b.z_body, (int64_t)kTExp * kN, s->bth, s->consts->tw, n, inst, st);
+        gemv(n, s->gemv_variant);
+        fold(n, s->fold_variant);
+      });
+      if (ms <= 0) {
+        rc = INSPIRE_ERR_HIP;
+        break;
+      }
+      out->batch_sizes[out->nbatch] = n;
+      out->batch_qps[out->nbatch] = n / (ms * 1e-3);
+      top = std::max(top, out->batch_qps[out->nbatch]);
+      ++out->nbatch;
+    }
+    for (int i = 0; i < out->nbatch; ++i)
+      if (out->batch_qps[i] >= 0.9 * top) {
+        out->suggested_batch = out->batch_sizes[i];
+        break;
+      }
+  }
+  (void)hipEventRecord(b.done, st);
+  (void)hipStreamSynchronize(st);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(d_out);
+  if (rc != INSPIRE_OK) return set_error(rc, "autotune: a timed launch failed");
+  HIP_TRY(hipGetLastError(), "autotune launch");
+  return INSPIRE_OK;
+}
+
 extern "C" int inspire_respond_batch_device(inspire_server_t* s, const uint8_t* d_queries, size_t q_stride,
                                             size_t len, int nq, int count, uint8_t* d_out, size_t out_stride,
                                             void* stream) {
@@ -495,16 +620,17 @@ extern "C" int inspire_respond_batch_device(inspire_server_t* s, const uint8_t* 
   launch_unpack_query_batch(d_queries, (int64_t)q_stride, nq, (int)rows, count, b.y_body, b.z_body, b.rows, b.bodies,
                             st);
   mark(1);
-  launch_packing_mfma(b.S, inst * 2 * kN, s->tiled, b.keys, b.y_body, (int64_t)kTExp * kN, count, inst, st);
+  launch_packing_mfma(b.S, inst * 2 * kN, s->tiled, b.keys, b.y_body, (int64_t)kTExp * kN, count, inst, st,
+                      s->pack_variant);
   mark(2);
   launch_post_zmajor(b.P, b.S, b.z_body, (int64_t)kTExp * kN, s->bth, s->consts->tw, count, inst, st);
   mark(3);
   if (launch_gemv_mfma_batch(b.cvec, (int64_t)nq * s->db_cols, b.rows, (int64_t)nq * rows, nq, s->db, s->dbfrag,
-                             (int)rows, s->db_cols, count, b.gemv_scratch, st))
+                             (int)rows, s->db_cols, count, b.gemv_scratch, st, s->gemv_variant))
     return set_error(INSPIRE_ERR_INVALID, "unsupported GEMV shape");
   mark(4);
   if (launch_fold_batch(d_out, (int64_t)out_stride, s->a_hat, b.cvec, b.P, b.bodies, s->consts->neg_a, s->consts->tw,
-                        inst, s->shape.interpolate_degree, nq, count, st))
+                        inst, s->shape.interpolate_degree, nq, count, st, s->fold_variant))
     return set_error(INSPIRE_ERR_INVALID, "unsupported fold shape");
   mark(5);
   HIP_TRY(hipGetLastError(), "batch launch");

@@ -28,6 +28,15 @@ STREAM_Y, STREAM_Z, STREAM_ROW, STREAM_BODY = 16, 17, 18, 19
 _lib = None
 
 
+class TuneResult(ctypes.Structure):
+    """inspire_tune_t (include/inspire_hip.h)."""
+    _fields_ = [("packing_variant", ctypes.c_int), ("gemv_variant", ctypes.c_int), ("fold_variant", ctypes.c_int),
+                ("packing_variants", ctypes.c_int), ("gemv_variants", ctypes.c_int),
+                ("packing_ms", ctypes.c_double * 8), ("gemv_ms", ctypes.c_double * 8), ("fold_ms", ctypes.c_double * 3),
+                ("nbatch", ctypes.c_int), ("batch_sizes", ctypes.c_int * 16), ("batch_qps", ctypes.c_double * 16),
+                ("suggested_batch", ctypes.c_int)]
+
+
 class InspireError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("inspire error %d: %s" % (code, msg))
@@ -85,6 +94,8 @@ def lib():
         L.inspire_server_set_slots.argtypes = [vp, i32]
         L.inspire_server_set_packing_kernel.argtypes = [vp, i32]
         L.inspire_server_set_max_batch.argtypes = [vp, i32]
+        L.inspire_server_set_batch_kernels.argtypes = [vp, i32, i32, i32]
+        L.inspire_server_autotune.argtypes = [vp, i32, i32, i32, ctypes.POINTER(TuneResult)]
         L.inspire_respond_batch_device.argtypes = [vp, vp, sz, sz, i32, i32, vp, sz, vp]
         L.inspire_packing_device.argtypes = [vp, vp, vp, vp, vp, vp, i64, vp]
         L.inspire_post_device.argtypes = [vp, vp, i64, vp]
@@ -154,6 +165,20 @@ class Server:
 
     def set_db(self, db, host=True):
         check(lib().inspire_server_set_db(self.h, _ptr(db), 1 if host else 0))
+
+    def set_batch_kernels(self, packing_variant, gemv_variant, fold_variant):
+        check(lib().inspire_server_set_batch_kernels(self.h, packing_variant, gemv_variant, fold_variant))
+
+    def autotune(self, count, nq=2, reps=3):
+        """Times the batched kernel variants on this device, keeps the fastest
+        and sweeps the batch size; returns the measurements as a dict."""
+        r = TuneResult()
+        check(lib().inspire_server_autotune(self.h, count, nq, reps, ctypes.byref(r)))
+        return {"packing_variant": r.packing_variant, "gemv_variant": r.gemv_variant,
+                "fold_variant": r.fold_variant, "packing_ms": list(r.packing_ms[:r.packing_variants]),
+                "gemv_ms": list(r.gemv_ms[:r.gemv_variants]), "fold_ms": list(r.fold_ms[1:3]),
+                "batch_qps": {int(r.batch_sizes[i]): r.batch_qps[i] for i in range(r.nbatch)},
+                "suggested_batch": r.suggested_batch}
 
     def encode_db(self, raw, host=True):
         """Raw column-major DB in, encoded on the device into the server's DB

@@ -375,6 +375,46 @@ def test_batch_results_equal_solo_results(torch, count, inst, d):
     srv.close()
 
 
+@pytest.mark.parametrize("count,inst,d", [(70, 8, 4), (64, 96, 8)])
+def test_batch_kernel_variants_agree_and_autotune(torch, count, inst, d):
+    # Every packing / GEMV / fold variant gives the same bytes, and so does the
+    # autotuner's choice.
+    say("batch variants", count, inst, d)
+    db_rows = 2048
+    srv = IA.Server(db_rows, inst, d)
+    srv.fill_synthetic(43)
+    srv.set_max_batch(count)
+    qs = _queries(db_rows, [300 + i for i in range(count)], small=False)
+    qlen = len(qs[0][0])
+    stride = (qlen + 16 + 255) // 256 * 256
+    hbuf = np.zeros(stride * count, dtype=np.uint8)
+    for i, (q, _) in enumerate(qs):
+        hbuf[i * stride:i * stride + qlen] = np.frombuffer(q, dtype=np.uint8)
+    dq = dev_from(torch, hbuf)
+    blob = 2 * (inst // d) * O.RESP_BYTES
+
+    def run():
+        dout = empty(torch, count * blob, np.uint8)
+        srv.respond_batch_device(dq, stride, qlen, 2, count, dout, blob)
+        torch.cuda.synchronize()
+        return dout.cpu().numpy().tobytes()
+
+    ref = run()
+    for pv in range(4):
+        for gv in range(2):
+            for fv in range(3):
+                srv.set_batch_kernels(pv, gv, fv)
+                assert run() == ref, (pv, gv, fv)
+    t = srv.autotune(count, nq=2, reps=2)
+    say("autotune", t)
+    assert len(t["packing_ms"]) == 4 and min(t["packing_ms"]) > 0 and min(t["fold_ms"]) > 0
+    assert t["suggested_batch"] in t["batch_qps"]
+    assert run() == ref
+    with pytest.raises(IA.InspireError):
+        srv.set_batch_kernels(4, 0, 0)
+    srv.close()
+
+
 # ---------------------------------------------------------------------------
 # Continuous-batching engine
 # ---------------------------------------------------------------------------
