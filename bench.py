@@ -310,6 +310,16 @@ def main():
         t0 = time.time()
         srv.set_max_batch(max(sizes))
         log("batch buffers + fragment-ordered bold_t built in %.1fs" % (time.time() - t0))
+        # Autotuner: kernel variants for 64-query batches, then the batch-size sweep.
+        tb = min(64, max(sizes))
+        t0 = time.time()
+        tune = srv.autotune(tb, nq=2, reps=3)
+        tune["count"] = tb
+        result["autotune"] = tune
+        log("autotune (%.1fs): packing %s ms -> %d, gemv %s ms -> %d, fold %s ms -> %d, suggested batch %d" % (
+            time.time() - t0, [round(x, 2) for x in tune["packing_ms"]], tune["packing_variant"],
+            [round(x, 2) for x in tune["gemv_ms"]], tune["gemv_variant"], [round(x, 2) for x in tune["fold_ms"]],
+            tune["fold_variant"], tune["suggested_batch"]))
         single_ms = total_ms / args.steps
         for B in sizes:
             bq, bqlen, bstride = make_queries(IA, torch, db_rows, min(B, args.queries), 17)

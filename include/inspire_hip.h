@@ -223,6 +223,11 @@ int inspire_server_set_packing_kernel(inspire_server_t* srv, int kind);
  * Asynchronous on `stream` (NULL = the server's stream); a second batch waits
  * for the first one's buffers. */
 int inspire_server_set_max_batch(inspire_server_t* srv, int max_batch);
+/* Batched responds run as HIP graphs (default on): the launch sequence of a
+ * (count, nq, query and output buffers, stream, kernel variants) key is
+ * captured on its second use and replayed afterwards; up to 256 keys are kept.
+ * Off: every call launches its kernels directly.  Results are identical. */
+int inspire_server_set_graphs(inspire_server_t* srv, int on);
 int inspire_respond_batch_device(inspire_server_t* srv, const uint8_t* d_queries, size_t q_stride, size_t len,
                                  int nq, int count, uint8_t* d_out, size_t out_stride, void* stream);
 

@@ -232,6 +232,20 @@ struct Slot {
 };
 
 // Buffers for batched responds (inspire_respond_batch_device).
+struct BatchGraph {
+  int count = 0, nq = 0, variants = 0;
+  const void* queries = nullptr;
+  size_t q_stride = 0;
+  const void* out = nullptr;
+  size_t out_stride = 0;
+  hipStream_t stream = nullptr;
+  hipGraphExec_t exec = nullptr;  // nullptr: seen once, not captured yet
+  bool same(const BatchGraph& o) const {
+    return count == o.count && nq == o.nq && variants == o.variants && queries == o.queries &&
+           q_stride == o.q_stride && out == o.out && out_stride == o.out_stride && stream == o.stream;
+  }
+};
+
 struct Batch {
   int max = 0;
   uint64_t *y_body = nullptr, *z_body = nullptr, *rows = nullptr, *P = nullptr, *cvec = nullptr;
@@ -264,6 +278,10 @@ struct inspire_server {
   uint8_t* tiled = nullptr;      // fragment-ordered int8 limbs of bold_t (MFMA packing)
   // batched kernel variants (set by the autotuner or inspire_server_set_batch_kernels)
   int pack_variant = 0, gemv_variant = 0, fold_variant = 0;
+  // Batched responds replayed as HIP graphs, one per launch key; a key is
+  // captured on its second use (the first runs the launches directly).
+  bool use_graphs = true;
+  std::vector<BatchGraph> graphs;
   bool tiled_valid = false;
   Batch batch;
   std::vector<uint8_t> host_out;
@@ -294,9 +312,16 @@ int dev_alloc(T** p, size_t count, const char* what) {
 
 constexpr int kDefaultSlots = 2;
 
+void free_graphs(inspire_server_t* s) {
+  for (auto& g : s->graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  s->graphs.clear();
+}
+
 void free_batch(inspire_server_t* s) {
   Batch& b = s->batch;
   if (b.done) (void)hipEventSynchronize(b.done);
+  free_graphs(s);
   void* ptrs[] = {b.y_body, b.z_body, b.rows, b.P, b.cvec, b.bodies, b.S, b.keys, b.gemv_scratch};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -594,6 +619,17 @@ extern "C" int inspire_server_autotune(inspire_server_t* s, int count, int nq, i
   return INSPIRE_OK;
 }
 
+static int launch_batch(inspire_server_t* s, const uint8_t* d_queries, size_t q_stride, int nq, int count,
+                        uint8_t* d_out, size_t out_stride, hipStream_t st, int mark_events);
+
+extern "C" int inspire_server_set_graphs(inspire_server_t* s, int on) {
+  if (!s) return set_error(INSPIRE_ERR_INVALID, "null server");
+  HIP_TRY(hipDeviceSynchronize(), "drain before changing graph use");
+  free_graphs(s);
+  s->use_graphs = on != 0;
+  return INSPIRE_OK;
+}
+
 extern "C" int inspire_respond_batch_device(inspire_server_t* s, const uint8_t* d_queries, size_t q_stride,
                                             size_t len, int nq, int count, uint8_t* d_out, size_t out_stride,
                                             void* stream) {
@@ -611,11 +647,69 @@ extern "C" int inspire_respond_batch_device(inspire_server_t* s, const uint8_t* 
   }
   hipStream_t st = stream ? (hipStream_t)stream : s->stream;
   Batch& b = s->batch;
+  HIP_TRY(hipStreamWaitEvent(st, b.done, 0), "batch wait");
+  // Graph replay: the same launch key (count, nq, buffers, stream, kernel
+  // variants) replays the sequence captured on its second use.
+  BatchGraph key;
+  key.count = count, key.nq = nq, key.queries = d_queries, key.q_stride = q_stride, key.out = d_out;
+  key.out_stride = out_stride, key.stream = st;
+  key.variants = s->pack_variant | s->gemv_variant << 4 | s->fold_variant << 8;
+  BatchGraph* g = nullptr;
+  if (s->use_graphs && !s->profile) {
+    for (auto& e : s->graphs)
+      if (e.same(key)) g = &e;
+    if (g && g->exec) {
+      HIP_TRY(hipGraphLaunch(g->exec, st), "batch graph launch");
+      HIP_TRY(hipEventRecord(b.done, st), "batch done");
+      return INSPIRE_OK;
+    }
+    if (!g) {
+      if (s->graphs.size() >= 256) free_graphs(s);
+      s->graphs.push_back(key);  // first use: run directly below
+    } else {
+      hipGraph_t graph = nullptr;
+      HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed), "batch capture");
+      int rc = launch_batch(s, d_queries, q_stride, nq, count, d_out, out_stride, st, -1);
+      hipError_t e = hipStreamEndCapture(st, &graph);
+      if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+      if (graph) (void)hipGraphDestroy(graph);
+      if (e != hipSuccess) {
+        g->exec = nullptr;
+        return set_error(INSPIRE_ERR_HIP, std::string("batch graph: ") + hipGetErrorString(e));
+      }
+      HIP_TRY(hipGraphLaunch(g->exec, st), "batch graph launch");
+      HIP_TRY(hipEventRecord(b.done, st), "batch done");
+      return INSPIRE_OK;
+    }
+  }
+  int rc = launch_batch(s, d_queries, q_stride, nq, count, d_out, out_stride, st, s->profile ? 0 : -1);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(b.done, st), "batch done");
+  if (s->profile) {
+    HIP_TRY(hipEventSynchronize(s->ev[5]), "profile sync");
+    for (int i = 0; i < 5; ++i) {
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, s->ev[i], s->ev[i + 1]), "profile elapsed");
+      s->stage_ms[i] += ms;
+    }
+    ++s->profiled;
+  }
+  return INSPIRE_OK;
+}
+
+// The launches of one batched respond on `st`; with mark_events >= 0 an event
+// is recorded before and after each stage for the stage timer.
+static int launch_batch(inspire_server_t* s, const uint8_t* d_queries, size_t q_stride, int nq, int count,
+                        uint8_t* d_out, size_t out_stride, hipStream_t st, int mark_events) {
+  Batch& b = s->batch;
   const int64_t rows = s->shape.db_rows, inst = s->shape.instances;
   auto mark = [&](int i) {
-    if (s->profile) (void)hipEventRecord(s->ev[i], st);
+    if (mark_events >= 0) (void)hipEventRecord(s->ev[i], st);
   };
-  HIP_TRY(hipStreamWaitEvent(st, b.done, 0), "batch wait");
   mark(0);
   launch_unpack_query_batch(d_queries, (int64_t)q_stride, nq, (int)rows, count, b.y_body, b.z_body, b.rows, b.bodies,
                             st);
@@ -634,16 +728,6 @@ extern "C" int inspire_respond_batch_device(inspire_server_t* s, const uint8_t* 
     return set_error(INSPIRE_ERR_INVALID, "unsupported fold shape");
   mark(5);
   HIP_TRY(hipGetLastError(), "batch launch");
-  HIP_TRY(hipEventRecord(b.done, st), "batch done");
-  if (s->profile) {
-    HIP_TRY(hipEventSynchronize(s->ev[5]), "profile sync");
-    for (int i = 0; i < 5; ++i) {
-      float ms = 0.f;
-      HIP_TRY(hipEventElapsedTime(&ms, s->ev[i], s->ev[i + 1]), "profile elapsed");
-      s->stage_ms[i] += ms;
-    }
-    ++s->profiled;
-  }
   return INSPIRE_OK;
 }
 

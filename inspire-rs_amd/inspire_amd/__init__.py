@@ -95,6 +95,7 @@ def lib():
         L.inspire_server_set_packing_kernel.argtypes = [vp, i32]
         L.inspire_server_set_max_batch.argtypes = [vp, i32]
         L.inspire_server_set_batch_kernels.argtypes = [vp, i32, i32, i32]
+        L.inspire_server_set_graphs.argtypes = [vp, i32]
         L.inspire_server_autotune.argtypes = [vp, i32, i32, i32, ctypes.POINTER(TuneResult)]
         L.inspire_respond_batch_device.argtypes = [vp, vp, sz, sz, i32, i32, vp, sz, vp]
         L.inspire_packing_device.argtypes = [vp, vp, vp, vp, vp, vp, i64, vp]
@@ -165,6 +166,9 @@ class Server:
 
     def set_db(self, db, host=True):
         check(lib().inspire_server_set_db(self.h, _ptr(db), 1 if host else 0))
+
+    def set_graphs(self, on):
+        check(lib().inspire_server_set_graphs(self.h, 1 if on else 0))
 
     def set_batch_kernels(self, packing_variant, gemv_variant, fold_variant):
         check(lib().inspire_server_set_batch_kernels(self.h, packing_variant, gemv_variant, fold_variant))

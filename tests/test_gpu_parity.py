@@ -415,6 +415,45 @@ def test_batch_kernel_variants_agree_and_autotune(torch, count, inst, d):
     srv.close()
 
 
+def test_batch_graph_replays_equal_direct_launches(torch):
+    # A batch key is launched directly on first use, captured on the second and
+    # replayed after; each replay (with other work and synchronizations in
+    # between, and the output cleared first) must equal the direct result.
+    say("batch graphs")
+    db_rows, inst, d, count = 2048, 96, 8, 24
+    srv = IA.Server(db_rows, inst, d)
+    srv.fill_synthetic(47)
+    srv.set_max_batch(40)
+    qs = _queries(db_rows, [400 + i for i in range(count + 5)], small=False)
+    qlen = len(qs[0][0])
+    stride = (qlen + 16 + 255) // 256 * 256
+    hbuf = np.zeros(stride * (count + 5), dtype=np.uint8)
+    for i, (q, _) in enumerate(qs):
+        hbuf[i * stride:i * stride + qlen] = np.frombuffer(q, dtype=np.uint8)
+    dq = dev_from(torch, hbuf)
+    blob = 2 * (inst // d) * O.RESP_BYTES
+    dout = empty(torch, (count + 5) * blob, np.uint8)
+
+    def run(n, q_off=0):
+        dout.zero_()
+        srv.respond_batch_device(dq[q_off * stride:], stride, qlen, 2, n, dout, blob)
+        torch.cuda.synchronize()
+        return dout[:n * blob].cpu().numpy().tobytes()
+
+    srv.set_graphs(False)
+    ref = run(count)
+    ref5 = run(5, count)
+    srv.set_graphs(True)
+    for rep in range(5):
+        assert run(count) == ref, rep
+        assert srv.respond_serialized(qs[rep][0])  # other work on the server between replays
+        assert run(5, count) == ref5, rep
+    srv.set_batch_kernels(1, 1, 2)  # a new key: direct, captured, replayed
+    for rep in range(3):
+        assert run(count) == ref, ("variants", rep)
+    srv.close()
+
+
 # ---------------------------------------------------------------------------
 # Continuous-batching engine
 # ---------------------------------------------------------------------------
