@@ -189,6 +189,7 @@ def main():
                     help="Poisson arrival rates (q/s) for the serving run ('' to skip)")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrivals per rate span about this long")
     ap.add_argument("--no-encode", action="store_true", help="skip the DB encode measurement")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default batched kernels")
     ap.add_argument("--packing", choices=("mfma", "valu"), default="mfma",
                     help="single-query packing kernel: int8 MFMA over the fragment-ordered bold_t, or the VALU "
                          "kernel over the reference layout")
@@ -311,15 +312,16 @@ def main():
         srv.set_max_batch(max(sizes))
         log("batch buffers + fragment-ordered bold_t built in %.1fs" % (time.time() - t0))
         # Autotuner: kernel variants for 64-query batches, then the batch-size sweep.
-        tb = min(64, max(sizes))
-        t0 = time.time()
-        tune = srv.autotune(tb, nq=2, reps=3)
-        tune["count"] = tb
-        result["autotune"] = tune
-        log("autotune (%.1fs): packing %s ms -> %d, gemv %s ms -> %d, fold %s ms -> %d, suggested batch %d" % (
-            time.time() - t0, [round(x, 2) for x in tune["packing_ms"]], tune["packing_variant"],
-            [round(x, 2) for x in tune["gemv_ms"]], tune["gemv_variant"], [round(x, 2) for x in tune["fold_ms"]],
-            tune["fold_variant"], tune["suggested_batch"]))
+        if not args.no_autotune:
+            tb = min(64, max(sizes))
+            t0 = time.time()
+            tune = srv.autotune(tb, nq=2, reps=3)
+            tune["count"] = tb
+            result["autotune"] = tune
+            log("autotune (%.1fs): packing %s ms -> %d, gemv %s ms -> %d, fold %s ms -> %d, suggested batch %d" % (
+                time.time() - t0, [round(x, 2) for x in tune["packing_ms"]], tune["packing_variant"],
+                [round(x, 2) for x in tune["gemv_ms"]], tune["gemv_variant"], [round(x, 2) for x in tune["fold_ms"]],
+                tune["fold_variant"], tune["suggested_batch"]))
         single_ms = total_ms / args.steps
         for B in sizes:
             bq, bqlen, bstride = make_queries(IA, torch, db_rows, min(B, args.queries), 17)

@@ -240,11 +240,16 @@ int inspire_respond_batch_device(inspire_server_t* srv, const uint8_t* d_queries
 int inspire_server_set_batch_kernels(inspire_server_t* srv, int packing_variant, int gemv_variant, int fold_variant);
 
 /* Autotuner.  Times every batched kernel variant on this device for `count`
- * keyword queries of nq hash queries (reps runs each, after a warm-up), keeps
- * the fastest of each stage for the server's batched respond, then measures
+ * keyword queries of nq hash queries, each inside the whole stage sequence
+ * (packing, post, GEMV, fold; reps runs after a warm-up), stage by stage:
+ * the packing variants first, then the GEMV variants with the chosen packing,
+ * then the fold kernels.  Keeps the fastest for the server's batched respond
+ * (the *_ms fields are those sequence times), then measures
  * the batch stages' throughput at batch sizes 1, 2, 4, ... up to max_batch and
  * suggests the smallest batch within 10% of the best.  Needs set_max_batch
- * >= count; uses the batch buffers (no other call on the server meanwhile). */
+ * >= count; fills the batch buffers with random residues first (the device
+ * runs faster on zero operands than on real queries) and uses them, so no
+ * other call on the server may run meanwhile. */
 typedef struct {
   int packing_variant, gemv_variant, fold_variant; /* chosen */
   int packing_variants, gemv_variants;             /* variants timed */

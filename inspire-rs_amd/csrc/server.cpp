@@ -508,9 +508,10 @@ extern "C" int inspire_server_set_batch_kernels(inspire_server_t* s, int packing
 }
 
 // Times each batched kernel variant on the device for `count` keyword queries
-// of nq hash queries (the batch buffers' contents: the kernels' work and
-// memory traffic do not depend on the values), keeps the fastest of each
-// stage, then measures the batch-stage throughput at batch sizes 1, 2, 4, ...
+// of nq hash queries (on the batch buffers' contents: the kernels' work and
+// memory traffic do not depend on the values) inside the whole stage
+// sequence, keeps the fastest of each stage, then measures the batch-stage
+// throughput at batch sizes 1, 2, 4, ...
 // up to max_batch with those kernels and suggests the smallest batch within
 // 10% of the best throughput.
 extern "C" int inspire_server_autotune(inspire_server_t* s, int count, int nq, int reps, inspire_tune_t* out) {
@@ -560,24 +561,44 @@ extern "C" int inspire_server_autotune(inspire_server_t* s, int count, int nq, i
   double best;
   rc = INSPIRE_OK;
   (void)hipStreamWaitEvent(st, b.done, 0);
+  // Uniform residues in the batch inputs: the device draws less power on
+  // zero operands and runs those kernels faster than on real queries.
+  {
+    const uint64_t keys = (uint64_t)s->batch.max * kTExp * kN;
+    launch_synth_fill(b.y_body, 0x5eed, 16, 0, keys, kSynthCondensed, st);
+    launch_synth_fill(b.z_body, 0x5eed, 17, 0, keys, kSynthCondensed, st);
+    launch_synth_fill(b.rows, 0x5eed, 18, 0, (uint64_t)s->batch.max * s->shape.max_queries * rows, kSynthCondensed,
+                      st);
+    launch_synth_fill(b.bodies, 0x5eed, 19, 0, (uint64_t)s->batch.max * s->shape.max_queries * 6 * kN,
+                      kSynthCondensed, st);
+  }
+  // Each candidate is timed inside the whole stage sequence (packing, post,
+  // GEMV, fold), stage by stage: a kernel timed back to back on its own ranks
+  // the variants differently from the same kernel between the other stages.
+  auto stages = [&](int n, int pv, int gv, int fv) {
+    pack(n, pv);
+    launch_post_zmajor(b.P, b.S, b.z_body, (int64_t)kTExp * kN, s->bth, s->consts->tw, n, inst, st);
+    gemv(n, gv);
+    fold(n, fv);
+  };
   // the packing variants differ only for groups of 64 clients
   out->packing_variants = batch_key_pad(count) >= 64 ? packing_wide_variants() : 1;
   best = 1e30;
   for (int v = 0; v < out->packing_variants && v < 8; ++v) {
-    out->packing_ms[v] = timed([&] { pack(count, v); });
+    out->packing_ms[v] = timed([&] { stages(count, v, 0, 0); });
     if (out->packing_ms[v] < 0) rc = INSPIRE_ERR_HIP;
     if (out->packing_ms[v] >= 0 && out->packing_ms[v] < best) best = out->packing_ms[v], out->packing_variant = v;
   }
   out->gemv_variants = s->dbfrag ? gemv_batch_variants() : 1;
   best = 1e30;
   for (int v = 0; v < out->gemv_variants && v < 8; ++v) {
-    out->gemv_ms[v] = timed([&] { gemv(count, v); });
+    out->gemv_ms[v] = timed([&] { stages(count, out->packing_variant, v, 0); });
     if (out->gemv_ms[v] < 0) rc = INSPIRE_ERR_HIP;
     if (out->gemv_ms[v] >= 0 && out->gemv_ms[v] < best) best = out->gemv_ms[v], out->gemv_variant = v;
   }
   best = 1e30;
   for (int v = 1; v <= 2; ++v) {
-    out->fold_ms[v] = timed([&] { fold(count, v); });
+    out->fold_ms[v] = timed([&] { stages(count, out->packing_variant, out->gemv_variant, v); });
     if (out->fold_ms[v] < 0) rc = INSPIRE_ERR_HIP;
     if (out->fold_ms[v] >= 0 && out->fold_ms[v] < best) best = out->fold_ms[v], out->fold_variant = v;
   }
@@ -588,12 +609,7 @@ extern "C" int inspire_server_autotune(inspire_server_t* s, int count, int nq, i
     // batch-size sweep with the chosen kernels (packing, post, GEMV, fold)
     double top = 0;
     for (int n = 1; n <= s->batch.max && out->nbatch < 16; n *= 2) {
-      const double ms = timed([&] {
-        pack(n, s->pack_variant);
-        launch_post_zmajor(b.P, b.S, b.z_body, (int64_t)kTExp * kN, s->bth, s->consts->tw, n, inst, st);
-        gemv(n, s->gemv_variant);
-        fold(n, s->fold_variant);
-      });
+      const double ms = timed([&] { stages(n, s->pack_variant, s->gemv_variant, s->fold_variant); });
       if (ms <= 0) {
         rc = INSPIRE_ERR_HIP;
         break;
