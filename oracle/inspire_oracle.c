@@ -908,3 +908,454 @@ int orc_num_threads(void) {
     return 1;
 #endif
 }
+
+/* ================================================================== */
+/* Client and server setup (the producers of the respond path's state) */
+/* ================================================================== */
+/* These restate the reference's client (client/keyword.rs, client/base.rs,
+ * client/online.rs), its packing keys (packing.rs:1838-1945, 3272-3294) and
+ * the server's offline precomputation (server/engine.rs:678-1000 CPU path:
+ * hint, prep_pack_many_lwes, full_packing_with_preprocessing_offline).  They
+ * exist so that tests can run real ciphertexts through the respond path and
+ * check that the client decrypts the stored values: that end-to-end decrypt
+ * is what pins the packing, post-process, fold and modulus switch semantics.
+ *
+ * Randomness.  Public randomness (SEED_0 query masks, W_SEED/V_SEED packing
+ * masks, the RGSW seeds) comes from the ChaCha20 restatement above, exactly
+ * as the reference consumes it (one u64 per coefficient, `% modulus`).  Secret
+ * keys and noise come from the reference's OS randomness (rand::random), so no
+ * vector of them exists; here they come from the counter-based generator with
+ * the reference's distributions (ternary key of Hamming weight h, discrete
+ * Gaussian of width 16.042421 by CDT, discrete_gaussian.rs:82-133). */
+
+#define NOISE_WIDTH 16.042421          /* pir/params.rs:69-96 */
+#define DG_MAX 65                      /* ceil(16.042421 * NUM_WIDTHS=4) */
+static uint64_t DG_CDF[2 * DG_MAX + 1];
+static int dg_init = 0;
+
+/* discrete_gaussian.rs:82-109: CDF of p(i) ~ exp(-pi i^2 / w^2), i in [-max, max]. */
+static void dg_setup(void) {
+    if (dg_init) return;
+    double table[2 * DG_MAX + 1], total = 0.0;
+    for (int i = -DG_MAX; i <= DG_MAX; i++) {
+        double p = __builtin_exp(-3.14159265358979323846 * (double)i * (double)i / (NOISE_WIDTH * NOISE_WIDTH));
+        table[i + DG_MAX] = p;
+        total += p;
+    }
+    double cum = 0.0;
+    for (int i = 0; i < 2 * DG_MAX + 1; i++) {
+        cum += table[i] / total;
+        double v = cum * 18446744073709551615.0;
+        DG_CDF[i] = v >= 18446744073709551615.0 ? UINT64_MAX : (uint64_t)(v + 0.5);
+    }
+    dg_init = 1;
+}
+
+/* discrete_gaussian.rs:111-133: the smallest i with u <= cdf[i]; value i - max mod m. */
+static uint64_t dg_sample(uint64_t u, uint64_t m) {
+    int out = 2 * DG_MAX;
+    for (int i = 2 * DG_MAX; i >= 0; i--)
+        if (u <= DG_CDF[i]) out = i;
+    int64_t v = (int64_t)out - DG_MAX;
+    return v < 0 ? (uint64_t)(v + (int64_t)m) : (uint64_t)v;
+}
+
+/* PolyMatrixRaw::noise (poly.rs:238-262): N coefficients mod q. */
+static void noise_poly(uint64_t* e, uint64_t seed, uint32_t stream, uint64_t start) {
+    dg_setup();
+    for (int z = 0; z < N; z++) e[z] = dg_sample(orc_synth_word(seed, stream, start + z), MODQ);
+}
+
+/* raw poly mod q -> NTT residues [2][N] (poly.rs:857-876 to_ntt). */
+static void raw_to_ntt(const uint64_t* raw, uint64_t* out) {
+    for (int c = 0; c < 2; c++) {
+        for (int z = 0; z < N; z++) out[c * N + z] = raw[z] % MODULI[c];
+        ntt_forward_one(out + c * N, c);
+    }
+}
+
+/* NTT residues [2][N] -> raw poly mod q (poly.rs:900-918 from_ntt: INTT + CRT). */
+static void ntt_to_raw(const uint64_t* in, uint64_t* raw) {
+    uint64_t t[2 * N];
+    memcpy(t, in, sizeof(t));
+    ntt_inverse_one(t, 0);
+    ntt_inverse_one(t + N, 1);
+    for (int z = 0; z < N; z++) raw[z] = crt_compose_2(t[z], t[N + z]);
+}
+
+/* out = a * b (+ out if acc), per CRT prime, canonical. */
+static void ntt_mul(uint64_t* out, const uint64_t* a, const uint64_t* b, int acc) {
+    for (int c = 0; c < 2; c++)
+        for (int z = 0; z < N; z++) {
+            uint64_t v = (a[c * N + z] * b[c * N + z]) % MODULI[c];
+            out[c * N + z] = acc ? (out[c * N + z] + v) % MODULI[c] : v;
+        }
+}
+
+static void ntt_add(uint64_t* out, const uint64_t* a) {
+    for (int c = 0; c < 2; c++)
+        for (int z = 0; z < N; z++) out[c * N + z] = (out[c * N + z] + a[c * N + z]) % MODULI[c];
+}
+
+static void condense(const uint64_t* ntt, uint64_t* out) { /* packing.rs:867-884 */
+    for (int z = 0; z < N; z++) out[z] = ntt[z] | (ntt[N + z] << 32);
+}
+
+/* gen_ternary_mat (client/base.rs:126-139): h coefficients 1, h coefficients
+ * q - 1, shuffled.  h = 0 is the keyword client's key: KeywordClient builds
+ * its YClient from Client::init (client/keyword.rs:88), whose sk_reg is the
+ * zero polynomial (client/base.rs:370-389). */
+void orc_client_secret(uint64_t seed, int hamming, uint64_t* sk) {
+    memset(sk, 0, sizeof(uint64_t) * N);
+    for (int i = 0; i < hamming; i++) sk[i] = 1;
+    for (int i = hamming; i < 2 * hamming; i++) sk[i] = MODQ - 1;
+    for (int i = N - 1; i > 0; i--) { /* Fisher-Yates */
+        uint64_t j = orc_synth_word(seed, 0x51, (uint64_t)i) % (uint64_t)(i + 1);
+        uint64_t t = sk[i]; sk[i] = sk[j]; sk[j] = t;
+    }
+}
+
+/* PolyMatrixNTT::random_rng (poly.rs:418-442): per poly, crt 0 then crt 1, one
+ * u64 per residue, `% q_c`.  out: [npolys][2][N]. */
+void orc_random_ntt(const uint8_t seed[32], uint64_t* out, int npolys) {
+    size_t n = (size_t)npolys * 2 * N;
+    orc_chacha20_u64(seed, out, n);
+    for (size_t i = 0; i < n; i++) out[i] %= MODULI[(i / N) & 1];
+}
+
+static void packing_masks(uint64_t* w_mask /*[3][2][N]*/, uint64_t* v_mask /*[3][2][N]*/) {
+    uint8_t ws[32], vs[32];
+    memset(ws, 7, 32);             /* W_SEED, pir/scheme.rs:46-48 */
+    memset(vs, 7, 32);
+    vs[0] = 8; vs[1] = 8;          /* V_SEED, pir/scheme.rs:49-51 */
+    orc_random_ntt(ws, w_mask, TEXP);
+    orc_random_ntt(vs, v_mask, TEXP);
+}
+
+/* generate_ksk_body (packing.rs:3272-3294): for k < 3,
+ *   body_k = NTT(tau_gen(s)) * 2^(19k) - NTT(s) * mask_k + NTT(e_k).
+ * mask: [3][2][N]; out: [3][N] condensed (condense_matrix). */
+static void ksk_body(uint64_t* out, const uint64_t* sk, uint32_t gen, const uint64_t* mask,
+                     uint64_t seed, uint32_t stream) {
+    uint64_t tau[N], s_ntt[2 * N], tau_ntt[2 * N], e[N], e_ntt[2 * N], acc[2 * N];
+    orc_automorph_raw(sk, tau, gen, MODQ);
+    raw_to_ntt(sk, s_ntt);
+    raw_to_ntt(tau, tau_ntt);
+    for (int k = 0; k < TEXP; k++) {
+        noise_poly(e, seed, stream, (uint64_t)k * N);
+        raw_to_ntt(e, e_ntt);
+        for (int c = 0; c < 2; c++) {
+            uint64_t m = MODULI[c], g = (1ULL << (19 * k)) % m;   /* build_gadget, gadget.rs:11-33 */
+            for (int z = 0; z < N; z++) {
+                uint64_t neg_mask = (m - mask[((size_t)k * 2 + c) * N + z]) % m;
+                uint64_t v = (tau_ntt[c * N + z] * g) % m;
+                v = (v + s_ntt[c * N + z] * neg_mask) % m;
+                acc[c * N + z] = (v + e_ntt[c * N + z]) % m;
+            }
+        }
+        condense(acc, out + (size_t)k * N);
+    }
+}
+
+/* PackingKeys::init_full (packing.rs:1895-1944): y_body with generator
+ * gen_pows[1] = 5 and mask W, z_body with 2n - 1 and mask V.  [3][N] condensed each. */
+void orc_client_packing_keys(uint64_t* y_body, uint64_t* z_body, const uint64_t* sk, uint64_t seed) {
+    init_all();
+    uint64_t w_mask[TEXP * 2 * N], v_mask[TEXP * 2 * N];
+    packing_masks(w_mask, v_mask);
+    ksk_body(y_body, sk, 5, w_mask, seed, 0x61);
+    ksk_body(z_body, sk, 2 * N - 1, v_mask, seed, 0x62);
+}
+
+/* Query row (client/keyword.rs:180-187 -> generate_query_b_values,
+ * client/online.rs:226-287, 340-357, pack_query 26-52): db_rows/N RLWE
+ * encryptions under the SEED_0 public masks (ChaCha20Rng(get_seed(0)) = the
+ * all-zero seed, one mask polynomial per RLWE, consumed in order); the RLWE
+ * holding the target row encrypts scale_k = floor(q/p) at coefficient
+ * target_row % N.  Each b = s*a + e + m (get_scaled_regev_sample with factor 1,
+ * client/base.rs:438-457).  out: [db_rows] condensed (b mod q0 | b mod q1 << 32). */
+void orc_client_query_row(uint64_t* out, int64_t db_rows, int64_t target_row, const uint64_t* sk,
+                          uint64_t seed, uint32_t stream) {
+    init_all();
+    const uint64_t scale_k = MODQ / 65535ULL;
+    int64_t nrlwe = db_rows / N;
+    uint8_t zero_seed[32];
+    memset(zero_seed, 0, 32);
+    uint64_t* a_all = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)nrlwe * N);
+    orc_chacha20_u64(zero_seed, a_all, (size_t)nrlwe * N);
+    uint64_t s_ntt[2 * N];
+    raw_to_ntt(sk, s_ntt);
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < nrlwe; k++) {
+        uint64_t a[N], a_ntt[2 * N], e[N], b_ntt[2 * N], b[N];
+        for (int z = 0; z < N; z++) a[z] = a_all[(size_t)k * N + z] % MODQ;
+        raw_to_ntt(a, a_ntt);
+        ntt_mul(b_ntt, s_ntt, a_ntt, 0);
+        ntt_to_raw(b_ntt, b);
+        noise_poly(e, seed, stream, (uint64_t)k * N);
+        for (int z = 0; z < N; z++) {
+            uint64_t v = (b[z] + e[z]) % MODQ;
+            if (k * N + z == target_row) v = (v + scale_k) % MODQ;
+            out[(size_t)k * N + z] = (v % MODULI[0]) | ((v % MODULI[1]) << 32);
+        }
+    }
+    free(a_all);
+}
+
+/* ct_gsw_body (client/keyword.rs:150-178): for j < t_gsw = 3, with
+ * sigma = +-2^(19j) X^(exponent mod n) (negative for exponent >= n),
+ * column 2j+1 = body of Enc(sigma) and column 2j = body of Enc(s * sigma), both
+ * under the RGSW public mask (all six RGSW seeds are [7;32], so the mask is the
+ * one orc_rgsw_neg_a_ntt negates).  body = NTT(s)*NTT(a) + NTT(e) + msg
+ * (encrypt_matrix_reg, client/base.rs:502-511).  out: [6][2][N] NTT residues. */
+void orc_client_rgsw_body(uint64_t* out, const uint64_t* sk, uint32_t exponent, uint64_t seed,
+                          uint32_t stream) {
+    init_all();
+    uint64_t neg_a[2 * N], a[N], a_ntt[2 * N], s_ntt[2 * N], sa[2 * N];
+    orc_rgsw_neg_a_ntt(neg_a, a);
+    raw_to_ntt(a, a_ntt);
+    raw_to_ntt(sk, s_ntt);
+    ntt_mul(sa, s_ntt, a_ntt, 0);
+    exponent %= 2 * N;
+    for (int j = 0; j < TEXP; j++) {
+        uint64_t sig[N], sig_ntt[2 * N], prod[2 * N], e[N], e_ntt[2 * N];
+        memset(sig, 0, sizeof(sig));
+        uint64_t g = 1ULL << (19 * j);
+        sig[exponent % N] = exponent < N ? g : MODQ - g;
+        raw_to_ntt(sig, sig_ntt);
+        ntt_mul(prod, s_ntt, sig_ntt, 0);
+        for (int half = 0; half < 2; half++) {
+            int col = 2 * j + (half ? 0 : 1);
+            const uint64_t* msg = half ? prod : sig_ntt;
+            noise_poly(e, seed, stream, (uint64_t)col * N);
+            raw_to_ntt(e, e_ntt);
+            uint64_t* o = out + (size_t)col * 2 * N;
+            for (int c = 0; c < 2; c++)
+                for (int z = 0; z < N; z++)
+                    o[c * N + z] = (sa[c * N + z] + e_ntt[c * N + z] + msg[c * N + z]) % MODULI[c];
+        }
+    }
+}
+
+/* Hint + prep_pack (server/engine.rs:733-800 -> answer_hint_ring_gpu over the
+ * SEED_0 masks, then prep_pack_many_lwes, packing.rs:1032-1104).
+ *
+ * LWE row r = kN + i of the query is coefficient i of RLWE k, whose row 0 is
+ * abar_k = -a_k; its mask vector is negacyclic_perm(abar_k, i) (utils.rs:5-35),
+ * i.e. <mask, s> = coef_i(abar_k * s).  The hint column of DB column j is
+ * A_j = sum_r DB[j][r] * mask_r, and prep_pack turns it into the polynomial
+ * at_j = negacyclic_perm(A_j, 0), the unique polynomial with
+ * const(at_j * s) = <A_j, s> for every s.  Since coef_i(P) = const(X^-i P),
+ *   at_j = sum_k Dt_jk * abar_k,  Dt_jk(X) = sum_i DB[j][kN+i] X^-i,
+ * a negacyclic product computed here in the NTT domain.  (Any exact method
+ * gives the same at_j mod q; the test decrypts through it.)
+ * db: encoded, column-major [db_cols][db_rows] u16.  out: NTT residues
+ * [ncols][2][N] for columns c0 .. c0+ncols-1. */
+void orc_setup_atilde(uint64_t* out, const uint16_t* db, int64_t db_rows, int64_t c0, int64_t ncols) {
+    init_all();
+    int64_t nrlwe = db_rows / N;
+    uint8_t zero_seed[32];
+    memset(zero_seed, 0, 32);
+    uint64_t* a_all = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)nrlwe * N);
+    uint64_t* abar = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)nrlwe * 2 * N);
+    orc_chacha20_u64(zero_seed, a_all, (size_t)nrlwe * N);
+    for (int64_t k = 0; k < nrlwe; k++) {
+        uint64_t t[N];
+        for (int z = 0; z < N; z++) {
+            uint64_t a = a_all[(size_t)k * N + z] % MODQ;
+            t[z] = a ? MODQ - a : 0;
+        }
+        raw_to_ntt(t, abar + (size_t)k * 2 * N);
+    }
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t jj = 0; jj < ncols; jj++) {
+        const uint16_t* col = db + (size_t)(c0 + jj) * db_rows;
+        uint64_t* o = out + (size_t)jj * 2 * N;
+        uint64_t dt[N], dt_ntt[2 * N];
+        memset(o, 0, sizeof(uint64_t) * 2 * N);
+        for (int64_t k = 0; k < nrlwe; k++) {
+            const uint16_t* dk = col + k * N;
+            dt[0] = dk[0];
+            for (int m = 1; m < N; m++) dt[m] = dk[N - m] ? MODQ - dk[N - m] : 0;
+            raw_to_ntt(dt, dt_ntt);
+            ntt_mul(o, dt_ntt, abar + (size_t)k * 2 * N, 1);
+        }
+    }
+    free(a_all);
+    free(abar);
+}
+
+/* acc[c][m] += (X^e * a)[m] for a given as residues a[c][N] and their
+ * negations na[c][N] (values < 2^28, so 2048 lazy additions fit in u64). */
+static void add_monomial_times(uint64_t* acc, const uint32_t* a, const uint32_t* na, uint32_t e) {
+    int neg = e >= N;
+    uint32_t s = e % N;
+    for (int c = 0; c < 2; c++) {
+        const uint32_t* hi = (neg ? na : a) + c * N;   /* X^s a: coefficients s.. */
+        const uint32_t* lo = (neg ? a : na) + c * N;   /* wrapped ones change sign */
+        uint64_t* o = acc + c * N;
+        for (uint32_t m = s; m < N; m++) o[m] += hi[m - s];
+        for (uint32_t m = 0; m < s; m++) o[m] += lo[m + N - s];
+    }
+}
+
+/* full_packing_with_preprocessing_offline_inner, CPU path
+ * (packing.rs:2477-2750), for `instances` outputs of N columns each.
+ * at: NTT residues [instances][N][2][N] from orc_setup_atilde.
+ *   r_i    = tau_{g_i}     (n^-1 * sum_j mono(j*g_-i)        * at_j),
+ *   rbar_i = tau_{2n-g_i}  (n^-1 * sum_j mono(2n - j*g_-i)   * at_j),
+ * for i < n/2 (g_i = 5^i mod 2n, g_-i = gen_pows[(n-i) mod n]); the monomial
+ * products are done as signed shifts in the coefficient domain, which equals
+ * the reference's NTT-domain products with monomial_ntts.  Then the collapse
+ * (2694-2749): for i = n/2-2 .. 0, bold_t[i] = NTT(G^-1(raw(r_{i+1}))),
+ * r_i += sum_k w_all[i][k] * bold_t[i][k], likewise for the bars with w_bar_all;
+ * bold_t_hat = NTT(G^-1(raw(rbar_0))), r_0 += sum_k v_mask[k] * bold_t_hat[k],
+ * a_hat = raw(r_0).  w_all/w_bar_all are the W mask rotated by g_i and 2n - g_i
+ * (OfflinePackingKeys::init_full, packing.rs:1838-1872 + generate_rotations_double).
+ * Outputs: bold_t, bold_t_bar [instances][3069][N] and bold_t_hat [instances][3][N]
+ * condensed; a_hat [instances][N] raw mod q. */
+void orc_setup_precomp(const uint64_t* at, int64_t instances, uint64_t* bold_t, uint64_t* bold_t_bar,
+                       uint64_t* bold_t_hat, uint64_t* a_hat) {
+    init_all();
+    const int HALF = N / 2;
+    uint64_t* w_mask = (uint64_t*)malloc(sizeof(uint64_t) * TEXP * 2 * N);
+    uint64_t* v_mask = (uint64_t*)malloc(sizeof(uint64_t) * TEXP * 2 * N);
+    packing_masks(w_mask, v_mask);
+    /* w_all[i][k][c][z] = w_mask[k][c][table_{g_i}[z]] (apply_automorph_ntt_double) */
+    uint64_t* w_all = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)NROT * TEXP * 2 * N);
+    uint64_t* w_bar = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)NROT * TEXP * 2 * N);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < NROT; i++) {
+        uint32_t g = gen_pow(i), gb = 2 * N - g;
+        for (int k = 0; k < TEXP; k++)
+            for (int c = 0; c < 2; c++)
+                for (int z = 0; z < N; z++) {
+                    size_t o = (((size_t)i * TEXP + k) * 2 + c) * N + z;
+                    w_all[o] = w_mask[((size_t)k * 2 + c) * N + auto_index(g, z)];
+                    w_bar[o] = w_mask[((size_t)k * 2 + c) * N + auto_index(gb, z)];
+                }
+    }
+    const uint64_t ninv[2] = {inv_mod(N, MODULI[0]), inv_mod(N, MODULI[1])};
+    uint32_t* a_res = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)N * 2 * N);
+    uint32_t* na_res = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)N * 2 * N);
+    uint64_t* r_all = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)HALF * 2 * N);
+    uint64_t* r_bar = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)HALF * 2 * N);
+    for (int64_t t = 0; t < instances; t++) {
+        /* coefficient-domain residues of every at_j and of its negation */
+#pragma omp parallel for schedule(static)
+        for (int j = 0; j < N; j++) {
+            uint64_t tmp[2 * N];
+            memcpy(tmp, at + ((size_t)t * N + j) * 2 * N, sizeof(tmp));
+            ntt_inverse_one(tmp, 0);
+            ntt_inverse_one(tmp + N, 1);
+            for (int c = 0; c < 2; c++)
+                for (int z = 0; z < N; z++) {
+                    uint32_t v = (uint32_t)tmp[c * N + z];
+                    a_res[((size_t)j * 2 + c) * N + z] = v;
+                    na_res[((size_t)j * 2 + c) * N + z] = v ? (uint32_t)MODULI[c] - v : 0;
+                }
+        }
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int i = 0; i < HALF; i++) {
+            uint64_t acc[2 * N], accb[2 * N], raw_c[2][N], rot[2 * N];
+            memset(acc, 0, sizeof(acc));
+            memset(accb, 0, sizeof(accb));
+            uint32_t gmi = gen_pow((N - i) % N);
+            for (uint32_t j = 0; j < N; j++) {
+                uint32_t e = (uint32_t)(((uint64_t)j * gmi) % (2 * N));
+                add_monomial_times(acc, a_res + (size_t)j * 2 * N, na_res + (size_t)j * 2 * N, e);
+                add_monomial_times(accb, a_res + (size_t)j * 2 * N, na_res + (size_t)j * 2 * N,
+                                   (2 * N - e) % (2 * N));
+            }
+            for (int which = 0; which < 2; which++) {
+                uint64_t* src = which ? accb : acc;
+                uint32_t tt = which ? 2 * N - gen_pow(i) : gen_pow(i);
+                for (int c = 0; c < 2; c++) {
+                    for (int z = 0; z < N; z++) raw_c[c][z] = ((src[c * N + z] % MODULI[c]) * ninv[c]) % MODULI[c];
+                    /* automorph_poly per residue (poly.rs:543-556) */
+                    for (uint64_t z = 0; z < N; z++) {
+                        uint64_t num = (z * tt) / N, rem = (z * tt) % N;
+                        uint64_t v = raw_c[c][z];
+                        rot[c * N + rem] = (num % 2 == 0) ? v : (v ? MODULI[c] - v : 0);
+                    }
+                    ntt_forward_one(rot + c * N, c);
+                }
+                memcpy((which ? r_bar : r_all) + (size_t)i * 2 * N, rot, sizeof(rot));
+            }
+        }
+        /* collapse: two independent chains (r and rbar) */
+        uint64_t* bt = bold_t + (size_t)t * NINNER * N;
+        uint64_t* btb = bold_t_bar + (size_t)t * NINNER * N;
+#pragma omp parallel for schedule(static, 1) num_threads(2)
+        for (int which = 0; which < 2; which++) {
+            uint64_t* r = which ? r_bar : r_all;
+            const uint64_t* w = which ? w_bar : w_all;
+            uint64_t* outb = which ? btb : bt;
+            for (int i = HALF - 2; i >= 0; i--) {
+                uint64_t raw[N], dig[2 * N], res[2 * N];
+                ntt_to_raw(r + (size_t)(i + 1) * 2 * N, raw);
+                memset(res, 0, sizeof(res));
+                for (int k = 0; k < TEXP; k++) {
+                    for (int c = 0; c < 2; c++) {
+                        for (int z = 0; z < N; z++) dig[c * N + z] = (raw[z] >> (19 * k)) & ((1ULL << 19) - 1);
+                        ntt_forward_one(dig + c * N, c);
+                    }
+                    condense(dig, outb + ((size_t)i * TEXP + k) * N);
+                    ntt_mul(res, w + (((size_t)i * TEXP + k) * 2) * N, dig, 1);
+                }
+                ntt_add(r + (size_t)i * 2 * N, res);
+            }
+        }
+        {
+            uint64_t raw[N], dig[2 * N], res[2 * N];
+            ntt_to_raw(r_bar, raw);
+            memset(res, 0, sizeof(res));
+            for (int k = 0; k < TEXP; k++) {
+                for (int c = 0; c < 2; c++) {
+                    for (int z = 0; z < N; z++) dig[c * N + z] = (raw[z] >> (19 * k)) & ((1ULL << 19) - 1);
+                    ntt_forward_one(dig + c * N, c);
+                }
+                condense(dig, bold_t_hat + ((size_t)t * TEXP + k) * N);
+                ntt_mul(res, v_mask + (size_t)k * 2 * N, dig, 1);
+            }
+            ntt_add(r_all, res);
+            ntt_to_raw(r_all, a_hat + (size_t)t * N);
+        }
+    }
+    free(w_mask); free(v_mask); free(w_all); free(w_bar);
+    free(a_res); free(na_res); free(r_all); free(r_bar);
+}
+
+/* recover_how_many (modulus_switch.rs:102-134) with how_many = gamma = N, then
+ * decrypt_ct_reg_measured (client/online.rs:118-136): s * row0 + row1 in the
+ * NTT domain, raw, rescaled q -> p = 65535.  blob: 12288 bytes; out: [N]. */
+void orc_client_decrypt(const uint8_t* blob, const uint64_t* sk, uint64_t* out) {
+    init_all();
+    const uint64_t q1p = 1ULL << 20, q2p = 268369921ULL;
+    uint64_t r0[N], r1[N], r0n[2 * N], r1n[2 * N], sn[2 * N], dec[2 * N], raw[N];
+    size_t bo = 0;
+    for (int z = 0; z < N; z++) { r0[z] = rescale(read_bits(blob, bo, 28), q2p, MODQ); bo += 28; }
+    for (int z = 0; z < N; z++) { r1[z] = rescale(read_bits(blob, bo, 20), q1p, MODQ); bo += 20; }
+    raw_to_ntt(r0, r0n);
+    raw_to_ntt(r1, r1n);
+    raw_to_ntt(sk, sn);
+    ntt_mul(dec, sn, r0n, 0);
+    ntt_add(dec, r1n);
+    ntt_to_raw(dec, raw);
+    for (int z = 0; z < N; z++) out[z] = rescale(raw[z], MODQ, 65535ULL);
+}
+
+/* Decrypt of an unswitched RLWE (a, b) [2][N] raw mod q (no modulus switch):
+ * rescale(s*a + b, q, p).  Used to check the packing stage on its own. */
+void orc_client_decrypt_raw(const uint64_t* ct, const uint64_t* sk, uint64_t* out) {
+    init_all();
+    uint64_t an[2 * N], bn[2 * N], sn[2 * N], dec[2 * N], raw[N], b[N];
+    for (int z = 0; z < N; z++) b[z] = ct[N + z] % MODQ;
+    raw_to_ntt(ct, an);
+    raw_to_ntt(b, bn);
+    raw_to_ntt(sk, sn);
+    ntt_mul(dec, sn, an, 0);
+    ntt_add(dec, bn);
+    ntt_to_raw(dec, raw);
+    for (int z = 0; z < N; z++) out[z] = rescale(raw[z], MODQ, 65535ULL);
+}

@@ -323,3 +323,124 @@ def encode(raw, db_rows, db_cols, d, gamma=N, modulus=65535):
     lib().orc_encode(_p(raw), _p(out), ctypes.c_int64(db_rows), ctypes.c_int64(db_cols), ctypes.c_int(d),
                      ctypes.c_int(gamma), ctypes.c_uint32(modulus), ctypes.c_uint32(num_inv))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Client and server setup (producers of real respond-path state)
+# ---------------------------------------------------------------------------
+
+HAMMING_WEIGHT = 256  # client/base.rs:14
+P_MOD = 65535
+
+
+def client_secret(seed, hamming=HAMMING_WEIGHT):
+    """gen_ternary_mat (client/base.rs:126-139); hamming=0 is the keyword
+    client's zero key (Client::init, client/base.rs:370-389)."""
+    sk = np.zeros(N, dtype=np.uint64)
+    lib().orc_client_secret(ctypes.c_uint64(seed), ctypes.c_int(hamming), _p(sk))
+    return sk
+
+
+def client_packing_keys(sk, seed):
+    """PackingKeys::init_full (packing.rs:1895-1944) -> y_body, z_body [3][N] condensed."""
+    y = np.zeros((TEXP, N), dtype=np.uint64)
+    z = np.zeros((TEXP, N), dtype=np.uint64)
+    lib().orc_client_packing_keys(_p(y), _p(z), _p(np.ascontiguousarray(sk, dtype=np.uint64)),
+                                  ctypes.c_uint64(seed))
+    return y, z
+
+
+def client_query_row(db_rows, target_row, sk, seed, stream=0x70):
+    """generate_query_b_values + pack_query (client/online.rs:226-357, 26-52)."""
+    out = np.zeros(db_rows, dtype=np.uint64)
+    lib().orc_client_query_row(_p(out), ctypes.c_int64(db_rows), ctypes.c_int64(target_row),
+                               _p(np.ascontiguousarray(sk, dtype=np.uint64)), ctypes.c_uint64(seed),
+                               ctypes.c_uint32(stream))
+    return out
+
+
+def client_rgsw_body(sk, exponent, seed, stream=0x80):
+    """ct_gsw_body (client/keyword.rs:150-178) -> [6][2][N] NTT residues."""
+    out = np.zeros((6, 2, N), dtype=np.uint64)
+    lib().orc_client_rgsw_body(_p(out), _p(np.ascontiguousarray(sk, dtype=np.uint64)),
+                               ctypes.c_uint32(exponent), ctypes.c_uint64(seed), ctypes.c_uint32(stream))
+    return out
+
+
+def rgsw_exponent(sub_col, d):
+    """client/keyword.rs:156-158: (2n * target_sub_col / d) mod 2n."""
+    return (2 * N * sub_col // d) % (2 * N)
+
+
+def client_query(sk, db_rows, d, targets, seed):
+    """KeywordClient::query for explicit (target_row, target_sub_col) pairs, one
+    per hash query (client/keyword.rs:147-206) -> (y, z, rows, bodies)."""
+    y, z = client_packing_keys(sk, seed)
+    rows = np.zeros((len(targets), db_rows), dtype=np.uint64)
+    bodies = np.zeros((len(targets), 6, 2, N), dtype=np.uint64)
+    for i, (row, sub) in enumerate(targets):
+        rows[i] = client_query_row(db_rows, row, sk, seed, 0x70 + i)
+        bodies[i] = client_rgsw_body(sk, rgsw_exponent(sub, d), seed, 0x80 + i)
+    return y, z, rows, bodies
+
+
+def setup_atilde(db, db_rows, c0, ncols):
+    """Hint + prep_pack (server/engine.rs:733-800, packing.rs:1032-1104) ->
+    NTT residues [ncols][2][N] of the packing inputs of columns c0 .. c0+ncols-1."""
+    out = np.zeros((ncols, 2, N), dtype=np.uint64)
+    lib().orc_setup_atilde(_p(out), _p(np.ascontiguousarray(db, dtype=np.uint16)), ctypes.c_int64(db_rows),
+                           ctypes.c_int64(c0), ctypes.c_int64(ncols))
+    return out
+
+
+def setup_precomp(at, instances):
+    """full_packing_with_preprocessing_offline (packing.rs:2477-2750) for
+    `instances` outputs -> bold_t, bold_t_bar, bold_t_hat (condensed), a_hat."""
+    bt = np.zeros(instances * NINNER * N, dtype=np.uint64)
+    btb = np.zeros(instances * NINNER * N, dtype=np.uint64)
+    bth = np.zeros(instances * TEXP * N, dtype=np.uint64)
+    ahat = np.zeros(instances * N, dtype=np.uint64)
+    lib().orc_setup_precomp(_p(np.ascontiguousarray(at, dtype=np.uint64)), ctypes.c_int64(instances),
+                            _p(bt), _p(btb), _p(bth), _p(ahat))
+    return bt, btb, bth, ahat
+
+
+def setup_state(db_enc, db_rows, instances):
+    """KeywordServer::setup's offline precomputation (keyword.rs:427-458) over an
+    encoded DB, one output polynomial at a time -> (bold_t, bold_t_bar, bold_t_hat, a_hat)."""
+    bt = np.zeros(instances * NINNER * N, dtype=np.uint64)
+    btb = np.zeros(instances * NINNER * N, dtype=np.uint64)
+    bth = np.zeros(instances * TEXP * N, dtype=np.uint64)
+    ahat = np.zeros(instances * N, dtype=np.uint64)
+    for t in range(instances):
+        at = setup_atilde(db_enc, db_rows, t * N, N)
+        o = setup_precomp(at, 1)
+        bt[t * NINNER * N:(t + 1) * NINNER * N] = o[0]
+        btb[t * NINNER * N:(t + 1) * NINNER * N] = o[1]
+        bth[t * TEXP * N:(t + 1) * TEXP * N] = o[2]
+        ahat[t * N:(t + 1) * N] = o[3]
+    return bt, btb, bth, ahat
+
+
+def client_decrypt(blob, sk):
+    """recover_how_many + decrypt_ct_reg_measured (modulus_switch.rs:102-134,
+    client/online.rs:118-136) of one 12288-byte chain blob -> [N] values mod p."""
+    b = np.frombuffer(bytes(blob), dtype=np.uint8).copy()
+    assert b.size == RESP_BYTES
+    out = np.zeros(N, dtype=np.uint64)
+    lib().orc_client_decrypt(_p(b), _p(np.ascontiguousarray(sk, dtype=np.uint64)), _p(out))
+    return out
+
+
+def client_decrypt_raw(ct, sk):
+    """Decrypt of an unswitched RLWE [2][N] (a, b) mod q, rescaled to p."""
+    out = np.zeros(N, dtype=np.uint64)
+    lib().orc_client_decrypt_raw(_p(np.ascontiguousarray(ct, dtype=np.uint64)),
+                                 _p(np.ascontiguousarray(sk, dtype=np.uint64)), _p(out))
+    return out
+
+
+def decrypt_response(blob, sk):
+    """All chains of one hash query's response -> [c][N] values mod p."""
+    c = len(blob) // RESP_BYTES
+    return np.stack([client_decrypt(blob[w * RESP_BYTES:(w + 1) * RESP_BYTES], sk) for w in range(c)])
