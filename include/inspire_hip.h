@@ -179,6 +179,16 @@ int inspire_server_encode_db(inspire_server_t* srv, const uint16_t* raw, int hos
  * condensed, a_hat [instances][2048] raw mod q. */
 int inspire_server_set_precomp(inspire_server_t* srv, const uint64_t* bold_t, const uint64_t* bold_t_bar,
                                const uint64_t* bold_t_hat, const uint64_t* a_hat, int host);
+/* KeywordServer::setup's offline precomputation on the device
+ * (keyword.rs:427-458 -> perform_offline_precomputation_simplepir,
+ * server/engine.rs:678-1000 with the GPU prep-pack and collapse of
+ * hint_kernel.cu, prep_pack_kernel.cu, packing_kernel.cu and
+ * collapse_kernel.cu): from the server's encoded DB, computes the hint over the
+ * SEED_0 query masks, prep-pack, and full_packing_with_preprocessing_offline
+ * (packing.rs:2477-2750) with the W_SEED / V_SEED packing masks, and installs
+ * bold_t, bold_t_bar, bold_t_hat and a_hat as inspire_server_set_precomp
+ * would.  Needs the DB (set_db or encode_db) first.  Synchronous. */
+int inspire_server_setup_offline(inspire_server_t* srv);
 /* Cuckoo stash entries returned with every response (keyword.rs:624-627). */
 int inspire_server_set_stash(inspire_server_t* srv, const uint8_t* entries, uint32_t entry_size,
                              uint32_t count);
@@ -293,6 +303,13 @@ int inspire_unpack_query_device(const uint8_t* d_query, int nq, int64_t db_rows,
  * column-major [db_cols][db_rows] u16 (d_out may equal d_in); interp_degree a
  * power of two <= 32; db_cols a multiple of interp_degree * 2048, db_rows of
  * 64.  Allocates its work buffer (up to the DB's size) and returns when done. */
+/* The offline precomputation of inspire_server_setup_offline on caller-owned
+ * device buffers: d_db encoded [instances*2048][db_rows] u16; outputs in the
+ * layouts of inspire_server_set_precomp.  Work memory: 64 MiB per instance of
+ * a pass, as many instances per pass as fit in free memory less 1 GiB.
+ * Synchronous on `stream` (NULL: the null stream). */
+int inspire_setup_offline_device(const uint16_t* d_db, int64_t db_rows, int64_t instances, uint64_t* d_bold_t,
+                                 uint64_t* d_bold_t_bar, uint64_t* d_bold_t_hat, uint64_t* d_a_hat, void* stream);
 int inspire_encode_db_device(uint16_t* d_out, const uint16_t* d_in, int64_t db_rows, int64_t db_cols,
                              int interp_degree, void* stream);
 /* gemv_kernel.cu:46-103 for nq query rows at once: d_c [nq][db_cols]. */

@@ -96,6 +96,14 @@ void launch_post_zmajor(uint64_t* P, const uint32_t* S, const uint64_t* z_body, 
                         const uint64_t* bth, const uint32_t* tw, int clients, int64_t instances, hipStream_t s);
 
 
+// Offline precomputation (setup.hip): bold_t, bold_t_bar, bold_t_hat (condensed)
+// and a_hat of every instance from the encoded DB, `group` instances per pass
+// with setup_offline_bytes_per_instance() bytes of work memory each.  Returns
+// an INSPIRE_* status (the message set on failure); synchronous on s.
+size_t setup_offline_bytes_per_instance();
+int launch_setup_offline(const uint16_t* db, int64_t db_rows, int64_t instances, uint64_t* bt, uint64_t* btb,
+                         uint64_t* bth, uint64_t* a_hat, const uint32_t* tw, int64_t group, hipStream_t s);
+
 size_t fold_lds_bytes();
 // out: [nq][instances/d][12288] response blobs.
 int launch_fold(uint8_t* out, const uint64_t* a_hat, const uint64_t* cvec, const uint64_t* P,

@@ -801,6 +801,46 @@ extern "C" int inspire_server_set_precomp(inspire_server_t* s, const uint64_t* b
   return s->tiled ? ensure_tiled(s) : INSPIRE_OK;
 }
 
+// Work group of instances for the offline precomputation: as many as fit in
+// free memory less 1 GiB, at most 256.
+static int setup_group(int64_t instances, int64_t* group) {
+  size_t free_b = 0, total_b = 0;
+  HIP_TRY(hipMemGetInfo(&free_b, &total_b), "memory info");
+  const size_t spare = free_b > (1ull << 30) ? free_b - (1ull << 30) : 0;
+  *group = std::min<int64_t>({instances, 256, (int64_t)(spare / setup_offline_bytes_per_instance())});
+  if (*group < 1) return set_error(INSPIRE_ERR_NOMEM, "no room for the offline precomputation of one instance");
+  return INSPIRE_OK;
+}
+
+extern "C" int inspire_setup_offline_device(const uint16_t* d_db, int64_t db_rows, int64_t instances,
+                                            uint64_t* d_bt, uint64_t* d_btb, uint64_t* d_bth, uint64_t* d_a_hat,
+                                            void* stream) {
+  if (!d_db || !d_bt || !d_btb || !d_bth || !d_a_hat) return set_error(INSPIRE_ERR_INVALID, "null argument");
+  if (db_rows <= 0 || db_rows % kN || instances <= 0 || instances > (1 << 20))
+    return set_error(INSPIRE_ERR_INVALID, "setup shape: db_rows a positive multiple of 2048, 0 < instances <= 2^20");
+  const DeviceConstants* dc = device_constants();
+  if (!dc) return set_error(INSPIRE_ERR_HIP, "device constants");
+  int64_t group = 0;
+  int rc = setup_group(instances, &group);
+  if (rc) return rc;
+  return launch_setup_offline(d_db, db_rows, instances, d_bt, d_btb, d_bth, d_a_hat, dc->tw, group,
+                              (hipStream_t)stream);
+}
+
+extern "C" int inspire_server_setup_offline(inspire_server_t* s) {
+  if (!s) return set_error(INSPIRE_ERR_INVALID, "null server");
+  if (!s->have_db) return set_error(INSPIRE_ERR_STATE, "setup_offline needs the database (set_db or encode_db)");
+  int64_t group = 0;
+  int rc = setup_group(s->shape.instances, &group);
+  if (rc) return rc;
+  rc = launch_setup_offline(s->db, s->shape.db_rows, s->shape.instances, s->bt, s->btb, s->bth, s->a_hat,
+                            s->consts->tw, group, s->stream);
+  if (rc) return rc;
+  s->have_precomp = true;
+  s->tiled_valid = false;
+  return s->tiled ? ensure_tiled(s) : INSPIRE_OK;
+}
+
 extern "C" int inspire_server_set_stash(inspire_server_t* s, const uint8_t* entries, uint32_t entry_size,
                                         uint32_t count) {
   if (!s || (count && !entries)) return set_error(INSPIRE_ERR_INVALID, "null argument");

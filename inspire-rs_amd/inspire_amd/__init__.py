@@ -75,6 +75,8 @@ def lib():
         L.inspire_encode_db_device.argtypes = [vp, vp, i64, i64, i32, vp]
         L.inspire_server_set_precomp.argtypes = [vp, vp, vp, vp, vp, i32]
         L.inspire_server_set_stash.argtypes = [vp, vp, u32, u32]
+        L.inspire_server_setup_offline.argtypes = [vp]
+        L.inspire_setup_offline_device.argtypes = [vp, i64, i64, vp, vp, vp, vp, vp]
         L.inspire_server_fill_synthetic.argtypes = [vp, u64]
         L.inspire_server_response_bytes.argtypes = [vp, i32]
         L.inspire_server_response_bytes.restype = sz
@@ -193,6 +195,11 @@ class Server:
         check(lib().inspire_server_set_precomp(self.h, _ptr(bt), _ptr(btb), _ptr(bth), _ptr(a_hat),
                                                1 if host else 0))
 
+    def setup_offline(self):
+        """KeywordServer::setup's offline precomputation on the device from the
+        server's DB (keyword.rs:427-458)."""
+        check(lib().inspire_server_setup_offline(self.h))
+
     def set_stash(self, entries):
         entries = list(entries)
         size = len(entries[0]) if entries else 0
@@ -290,6 +297,12 @@ class Engine:
 
 def synth_fill(d_out, seed, stream_id, count, kind, start=0, stream=None):
     check(lib().inspire_synth_fill(_ptr(d_out), seed, stream_id, start, count, kind, stream))
+
+
+def setup_offline_device(d_db, db_rows, instances, d_bt, d_btb, d_bth, d_a_hat, stream=None):
+    """Offline precomputation on caller-owned device buffers (keyword.rs:427-458)."""
+    check(lib().inspire_setup_offline_device(_ptr(d_db), db_rows, instances, _ptr(d_bt), _ptr(d_btb), _ptr(d_bth),
+                                             _ptr(d_a_hat), stream))
 
 
 def unpack_query_device(d_query, nq, db_rows, d_y, d_z, d_rows, d_bodies, stream=None):

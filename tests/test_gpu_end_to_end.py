@@ -116,3 +116,40 @@ def test_gpu_batched_clients_decrypt_their_own_values(torch, real, count):
     # the first client's bytes also against the oracle
     exp = O.respond_raw(enc, DB_ROWS, INST, D, *st, *clients[0][2])
     assert out[:blob] == b"".join(exp)
+
+
+@pytest.mark.parametrize("db_rows,inst", [(DB_ROWS, INST), (4096, 2)])
+def test_gpu_offline_setup_matches_oracle(torch, real, db_rows, inst):
+    # hint + prep-pack + InspiRING precompute + collapse on the device equal the
+    # oracle's restatement of the reference's CPU path, word for word.
+    say("setup_offline_device", db_rows, inst)
+    if db_rows == DB_ROWS and inst == INST:
+        _, enc, st, _ = real
+    else:
+        rng = np.random.default_rng(db_rows + inst)
+        enc = rng.integers(0, 65535, size=inst * N * db_rows, dtype=np.uint16)
+        st = O.setup_state(enc, db_rows, inst)
+    d_db = torch.from_numpy(enc.view(np.int16)).cuda()
+    outs = [torch.zeros(a.size, dtype=torch.int64, device="cuda") for a in st]
+    IA.setup_offline_device(d_db, db_rows, inst, *outs)
+    torch.cuda.synchronize()
+    for name, got, exp in zip(("bold_t", "bold_t_bar", "bold_t_hat", "a_hat"), outs, st):
+        g = got.cpu().numpy().view(np.uint64)
+        assert np.array_equal(g, exp), (name, int(np.argmax(g != exp)))
+
+
+def test_gpu_encode_setup_respond_decrypts(torch, real):
+    # The whole server side on the device: encode, offline precomputation,
+    # respond; the client decrypts the stored values.
+    rawm, enc, st, _ = real
+    say("encode + setup_offline + respond")
+    srv = IA.Server(DB_ROWS, INST, D)
+    srv.encode_db(np.ascontiguousarray(rawm).reshape(-1))
+    srv.setup_offline()
+    targets = [(5, 7), (1500, 2)]
+    sk, parts, q = client(901, O.HAMMING_WEIGHT, targets)
+    resp = srv.respond_serialized(q)
+    assert resp == O.serialize_response(O.respond_raw(enc, DB_ROWS, INST, D, *st, *parts), [])
+    for (row, sub), blob in zip(targets, split_response(resp)):
+        assert np.array_equal(O.decrypt_response(blob, sk), expected(rawm, row, sub)), (row, sub)
+    srv.close()
