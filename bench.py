@@ -146,6 +146,10 @@ def cpu_baseline(plan, out_samples=32, fold_chains=2):
     t0 = time.perf_counter()
     O.encode(raw, er, d * N, d)
     t_enc = (time.perf_counter() - t0) * (rows / er) * plan["c"]
+    # Offline precomputation: one output polynomial (2048 columns), scaled.
+    t0 = time.perf_counter()
+    O.setup_precomp(O.setup_atilde(db, rows, 0, N), 1)
+    t_setup = (time.perf_counter() - t0) * inst
     sample = ("packing+INTT for %d of %d outputs, GEMV of %d of %d columns x 2 hash queries, "
               "fold+modswitch of %d of %d chains; linearly scaled to one keyword query"
               % (J, inst, J * N, plan["db_cols"], fold_chains, 2 * plan["c"]))
@@ -153,7 +157,8 @@ def cpu_baseline(plan, out_samples=32, fold_chains=2):
             "sample": sample, "seconds_per_query": per_query,
             "stage_s": {"packing": t_pack * scale, "post": t_post * scale, "gemv": t_gemv * scale,
                         "fold": t_fold * 2 * plan["c"]},
-            "encode_s": t_enc, "encode_sample": "64 of %d rows of 1 of %d column groups, scaled" % (rows, plan["c"])}
+            "encode_s": t_enc, "encode_sample": "64 of %d rows of 1 of %d column groups, scaled" % (rows, plan["c"]),
+            "offline_setup_s": t_setup, "offline_setup_sample": "1 of %d outputs (hint, prep-pack, precompute, collapse), scaled" % inst}
 
 
 def profiled_traffic(kernel_prefix):
@@ -188,7 +193,8 @@ def main():
     ap.add_argument("--rates", default="100,500,1000,2000,3000",
                     help="Poisson arrival rates (q/s) for the serving run ('' to skip)")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrivals per rate span about this long")
-    ap.add_argument("--no-encode", action="store_true", help="skip the DB encode measurement")
+    ap.add_argument("--no-encode", action="store_true", help="skip the DB encode and offline setup measurements")
+    ap.add_argument("--no-setup", action="store_true", help="skip the offline setup measurement")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default batched kernels")
     ap.add_argument("--packing", choices=("mfma", "valu"), default="mfma",
                     help="single-query packing kernel: int8 MFMA over the fragment-ordered bold_t, or the VALU "
@@ -388,12 +394,26 @@ def main():
         t0 = time.perf_counter()
         IA.encode_db_device(raw, raw, db_rows, plan["db_cols"], d)
         enc_s = time.perf_counter() - t0
-        del raw
-        torch.cuda.empty_cache()
         result["setup"] = {"encode_ms": enc_s * 1e3, "encode_db_bytes": 2 * ncell,
                            "encode_gbs": 2 * ncell * 2 / enc_s / 1e9,
                            "note": "raw DB read once and encoded DB written once (2 x 2.95 GB) over the call's wall time"}
         log("encode %.1f ms (%.0f GB/s of read+write DB bytes)" % (enc_s * 1e3, 4 * ncell / enc_s / 1e9))
+        # Offline precomputation of the encoded DB (hint, prep-pack, InspiRING
+        # precompute and collapse): bold_t, bold_t_bar, bold_t_hat, a_hat.
+        if not args.no_setup:
+            outs_pre = [torch.empty(n, dtype=torch.int64, device="cuda")
+                        for n in (inst * 3069 * IA.N, inst * 3069 * IA.N, inst * 3 * IA.N, inst * IA.N)]
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            IA.setup_offline_device(raw, db_rows, inst, *outs_pre)
+            off_s = time.perf_counter() - t0
+            result["setup"].update({"offline_ms": off_s * 1e3, "offline_instances": inst,
+                                    "offline_note": "one synchronous call: 352 outputs x (2048 column NTTs, "
+                                                    "4096 slot NTTs, two 1023-step collapse chains), work memory included"})
+            log("offline precomputation %.1f ms" % (off_s * 1e3))
+            del outs_pre
+        del raw
+        torch.cuda.empty_cache()
 
     # Serving: continuous batching under open-loop Poisson arrivals.
     serving = []

@@ -347,6 +347,46 @@ int inspire_fold_device(uint8_t* d_out, const uint64_t* d_a_hat, const uint64_t*
  * response (stash included) and the submit-to-ready latency.  The server must
  * not be used by other callers while an engine runs on it.
  * ---------------------------------------------------------------------- */
+/* ---------------------------------------------------------------------- */
+/* Keyword layer: PQKEYV2 datasets, cuckoo table, server setup             */
+/* ---------------------------------------------------------------------- */
+
+/* PublicParams with its KeywordPirHandshake (protocol/wire.rs, returned by
+ * KeywordServer::public_params, server/keyword.rs:505-531). */
+typedef struct {
+  int64_t num_items;       /* cuckoo buckets */
+  int64_t item_size_bits;  /* 8 * entry_size */
+  int64_t dim0;
+  uint8_t cuckoo_seed[16]; /* DETERMINISTIC_SEED (cuckoo.rs:27-29) */
+  int32_t num_hashes;      /* 2 */
+  int64_t key_size, value_size, entry_size;
+  int64_t num_records;     /* occupied buckets + stash entries */
+} inspire_public_params_t;
+
+/* setup_keyword_server (dataset/keyword.rs:29-44 -> KeywordServer::setup,
+ * server/keyword.rs:366-468): loads a PQKEYV2 dataset (dataset.rs:95-142:
+ * magic "PQKEYV2\0", u8 kem-name length + name, u32 LE count, u16 LE key
+ * length, then count x (32-byte id, key)), builds the cuckoo table (key = id,
+ * value = public key, SipHash-2-4 positions, cuckoo.rs:98-244), lays it out as
+ * the column-major raw DB (cuckoo.rs:314-353), and creates a server with the
+ * DB encoded and the offline precomputation done on the device and the stash
+ * installed.  buckets / dim0 <= 0 select the reference defaults.  pp (may be
+ * NULL) receives public_params.  The cuckoo eviction order is seeded here
+ * (the reference uses OS randomness), so tables are repeatable. */
+int inspire_setup_keyword_server(const char* pqkeyv2_path, int64_t buckets, int64_t dim0, inspire_server_t** out,
+                                 inspire_public_params_t* pp);
+/* KeywordServer::setup from records in memory: count keys of key_size bytes
+ * and count values of value_size bytes, back to back. */
+int inspire_setup_keyword_records(const uint8_t* keys, uint32_t key_size, const uint8_t* values,
+                                  uint32_t value_size, int64_t count, int64_t buckets, int64_t dim0,
+                                  inspire_server_t** out, inspire_public_params_t* pp);
+/* CuckooHash::all_positions with the deterministic seed (cuckoo.rs:98-132):
+ * the client's two bucket indices for a key.  Host only. */
+int inspire_cuckoo_positions(const uint8_t* key, size_t key_len, int64_t num_buckets, int64_t* out2);
+/* SipHash-2-4 of msg under (k0, k1), as siphasher's SipHasher::new_with_keys,
+ * write, finish.  Host only. */
+uint64_t inspire_siphash24(uint64_t k0, uint64_t k1, const uint8_t* msg, size_t len);
+
 typedef struct inspire_engine inspire_engine_t;
 int inspire_engine_create(inspire_server_t* srv, int nq, int max_batch, double window_ms, int capacity,
                           inspire_engine_t** out);

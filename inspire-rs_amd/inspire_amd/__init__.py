@@ -28,6 +28,12 @@ STREAM_Y, STREAM_Z, STREAM_ROW, STREAM_BODY = 16, 17, 18, 19
 _lib = None
 
 
+class PublicParamsT(ctypes.Structure):
+    _fields_ = [("num_items", ctypes.c_int64), ("item_size_bits", ctypes.c_int64), ("dim0", ctypes.c_int64),
+                ("cuckoo_seed", ctypes.c_uint8 * 16), ("num_hashes", ctypes.c_int32), ("key_size", ctypes.c_int64),
+                ("value_size", ctypes.c_int64), ("entry_size", ctypes.c_int64), ("num_records", ctypes.c_int64)]
+
+
 class TuneResult(ctypes.Structure):
     """inspire_tune_t (include/inspire_hip.h)."""
     _fields_ = [("packing_variant", ctypes.c_int), ("gemv_variant", ctypes.c_int), ("fold_variant", ctypes.c_int),
@@ -76,6 +82,13 @@ def lib():
         L.inspire_server_set_precomp.argtypes = [vp, vp, vp, vp, vp, i32]
         L.inspire_server_set_stash.argtypes = [vp, vp, u32, u32]
         L.inspire_server_setup_offline.argtypes = [vp]
+        L.inspire_setup_keyword_server.argtypes = [ctypes.c_char_p, i64, i64, ctypes.POINTER(vp),
+                                                   ctypes.POINTER(PublicParamsT)]
+        L.inspire_setup_keyword_records.argtypes = [vp, u32, vp, u32, i64, i64, i64, ctypes.POINTER(vp),
+                                                    ctypes.POINTER(PublicParamsT)]
+        L.inspire_cuckoo_positions.argtypes = [vp, sz, i64, ctypes.POINTER(i64)]
+        L.inspire_siphash24.argtypes = [u64, u64, vp, sz]
+        L.inspire_siphash24.restype = u64
         L.inspire_setup_offline_device.argtypes = [vp, i64, i64, vp, vp, vp, vp, vp]
         L.inspire_server_fill_synthetic.argtypes = [vp, u64]
         L.inspire_server_response_bytes.argtypes = [vp, i32]
@@ -151,6 +164,14 @@ class Server:
         h = ctypes.c_void_p()
         check(lib().inspire_server_create(ctypes.byref(shape), ctypes.byref(h)))
         self.h = h
+
+    @classmethod
+    def _adopt(cls, h, plan):
+        self = cls.__new__(cls)
+        self.db_rows, self.instances, self.d = plan["db_rows"], plan["instances"], plan["interpolate_degree"]
+        self.c = self.instances // self.d
+        self.h = h
+        return self
 
     def close(self):
         if self.h:
@@ -297,6 +318,57 @@ class Engine:
 
 def synth_fill(d_out, seed, stream_id, count, kind, start=0, stream=None):
     check(lib().inspire_synth_fill(_ptr(d_out), seed, stream_id, start, count, kind, stream))
+
+
+def _public_params(pp):
+    return {"num_items": pp.num_items, "item_size_bits": pp.item_size_bits, "dim0": pp.dim0,
+            "cuckoo_seed": bytes(pp.cuckoo_seed), "num_hashes": pp.num_hashes, "key_size": pp.key_size,
+            "value_size": pp.value_size, "entry_size": pp.entry_size, "num_records": pp.num_records}
+
+
+def setup_keyword_server(path, buckets=None, dim0=None):
+    """setup_keyword_server (dataset/keyword.rs:29-44) from a PQKEYV2 file:
+    returns (Server, public_params dict)."""
+    h = ctypes.c_void_p()
+    pp = PublicParamsT()
+    check(lib().inspire_setup_keyword_server(os.fsencode(path), buckets or 0, dim0 or 0, ctypes.byref(h),
+                                             ctypes.byref(pp)))
+    par = _public_params(pp)
+    return Server._adopt(h, _plan_of(par)), par
+
+
+def setup_keyword_records(keys, values, buckets=None, dim0=None):
+    """KeywordServer::setup from in-memory records (lists of equal-length bytes)."""
+    ks, vs = len(keys[0]), len(values[0])
+    kb = ctypes.create_string_buffer(b"".join(keys), ks * len(keys))
+    vb = ctypes.create_string_buffer(b"".join(values), vs * len(values))
+    h = ctypes.c_void_p()
+    pp = PublicParamsT()
+    check(lib().inspire_setup_keyword_records(ctypes.cast(kb, ctypes.c_void_p), ks, ctypes.cast(vb, ctypes.c_void_p),
+                                              vs, len(keys), buckets or 0, dim0 or 0, ctypes.byref(h),
+                                              ctypes.byref(pp)))
+    par = _public_params(pp)
+    return Server._adopt(h, _plan_of(par)), par
+
+
+def _plan_of(par):
+    """Server shape from public params (the client derives it the same way,
+    client/keyword.rs:60-118)."""
+    kv = par["entry_size"]
+    return plan_shape(par["num_items"], kv - 1, 1, buckets=par["num_items"], dim0=par["dim0"])
+
+
+def cuckoo_positions(key, num_buckets):
+    """CuckooHash::all_positions with the deterministic seed (cuckoo.rs:98-132)."""
+    out = (ctypes.c_int64 * 2)()
+    kb = ctypes.create_string_buffer(bytes(key), len(key) or 1)
+    check(lib().inspire_cuckoo_positions(ctypes.cast(kb, ctypes.c_void_p), len(key), num_buckets, out))
+    return [out[0], out[1]]
+
+
+def siphash24(k0, k1, msg):
+    mb = ctypes.create_string_buffer(bytes(msg), len(msg) or 1)
+    return lib().inspire_siphash24(k0, k1, ctypes.cast(mb, ctypes.c_void_p), len(msg))
 
 
 def setup_offline_device(d_db, db_rows, instances, d_bt, d_btb, d_bth, d_a_hat, stream=None):

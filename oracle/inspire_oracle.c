@@ -1359,3 +1359,30 @@ void orc_client_decrypt_raw(const uint64_t* ct, const uint64_t* sk, uint64_t* ou
     ntt_to_raw(dec, raw);
     for (int z = 0; z < N; z++) out[z] = rescale(raw[z], MODQ, 65535ULL);
 }
+
+/* SipHash-2-4 (Aumasson and Bernstein 2012; siphasher::sip::SipHasher with
+ * new_with_keys, write, finish), as CuckooHash::hash uses it
+ * (pir/keyword/cuckoo.rs:98-124). */
+uint64_t orc_siphash24(uint64_t k0, uint64_t k1, const uint8_t* m, size_t len) {
+#define SROTL(x, b) (((x) << (b)) | ((x) >> (64 - (b))))
+#define SROUND() do { v0 += v1; v1 = SROTL(v1, 13); v1 ^= v0; v0 = SROTL(v0, 32); \
+                      v2 += v3; v3 = SROTL(v3, 16); v3 ^= v2; \
+                      v0 += v3; v3 = SROTL(v3, 21); v3 ^= v0; \
+                      v2 += v1; v1 = SROTL(v1, 17); v1 ^= v2; v2 = SROTL(v2, 32); } while (0)
+    uint64_t v0 = k0 ^ 0x736f6d6570736575ULL, v1 = k1 ^ 0x646f72616e646f6dULL;
+    uint64_t v2 = k0 ^ 0x6c7967656e657261ULL, v3 = k1 ^ 0x7465646279746573ULL;
+    size_t i = 0;
+    for (; i + 8 <= len; i += 8) {
+        uint64_t w = 0;
+        for (int b = 0; b < 8; b++) w |= (uint64_t)m[i + b] << (8 * b);
+        v3 ^= w; SROUND(); SROUND(); v0 ^= w;
+    }
+    uint64_t w = (uint64_t)(len & 0xff) << 56;
+    for (int b = 0; i + b < len; b++) w |= (uint64_t)m[i + b] << (8 * b);
+    v3 ^= w; SROUND(); SROUND(); v0 ^= w;
+    v2 ^= 0xff;
+    SROUND(); SROUND(); SROUND(); SROUND();
+    return v0 ^ v1 ^ v2 ^ v3;
+#undef SROUND
+#undef SROTL
+}

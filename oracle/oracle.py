@@ -444,3 +444,106 @@ def decrypt_response(blob, sk):
     """All chains of one hash query's response -> [c][N] values mod p."""
     c = len(blob) // RESP_BYTES
     return np.stack([client_decrypt(blob[w * RESP_BYTES:(w + 1) * RESP_BYTES], sk) for w in range(c)])
+
+
+# ---------------------------------------------------------------------------
+# Keyword client (client/keyword.rs) and PQKEYV2 datasets (dataset/dataset.rs)
+# ---------------------------------------------------------------------------
+
+CUCKOO_SEED = bytes([0xDE, 0xAD, 0xBE, 0xEF, 0xCA, 0xFE, 0xBA, 0xBE,
+                     0x01, 0x23, 0x45, 0x67, 0x89, 0xAB, 0xCD, 0xEF])  # cuckoo.rs:27-29
+
+
+def siphash24(k0, k1, msg):
+    lib().orc_siphash24.restype = ctypes.c_uint64
+    mb = np.frombuffer(bytes(msg) or b"\0", dtype=np.uint8).copy()
+    return lib().orc_siphash24(ctypes.c_uint64(k0), ctypes.c_uint64(k1), _p(mb), ctypes.c_size_t(len(msg)))
+
+
+def cuckoo_positions(key, num_buckets, seed=CUCKOO_SEED):
+    """CuckooHash::hash for hash indices 0, 1 (cuckoo.rs:98-132)."""
+    out = []
+    for h in range(2):
+        k0 = int.from_bytes(seed[:8], "little") ^ h
+        k1 = int.from_bytes(seed[8:], "little") ^ (h << 32)
+        out.append(siphash24(k0, k1, key) % num_buckets)
+    return out
+
+
+def public_key_id(public_key, kem_name="ml-kem-768"):
+    """derive_public_key_id (dataset.rs:36-46): SHA3-256(b"pq-key:v1:" || kem || pk)."""
+    import hashlib
+    return hashlib.sha3_256(b"pq-key:v1:" + kem_name.encode() + bytes(public_key)).digest()
+
+
+def write_pqkeyv2(path, public_keys, kem_name="ml-kem-768"):
+    """save_dataset (dataset.rs:48-93) -> list of (id, public_key)."""
+    recs = [(public_key_id(pk, kem_name), bytes(pk)) for pk in public_keys]
+    with open(path, "wb") as f:
+        f.write(b"PQKEYV2\0" + bytes([len(kem_name)]) + kem_name.encode())
+        f.write(len(recs).to_bytes(4, "little") + len(recs[0][1]).to_bytes(2, "little"))
+        for i, pk in recs:
+            f.write(i + pk)
+    return recs
+
+
+class KeywordClient:
+    """KeywordClient (client/keyword.rs:45-311) from public params and the
+    server plan; sk: the client's secret (zero for the reference's client)."""
+
+    def __init__(self, pp, plan, sk):
+        self.pp, self.sk = pp, sk
+        self.db_rows, self.d = plan["db_rows"], plan["interpolate_degree"]
+        self.per = pp["num_items"] // self.db_rows
+        self.ise = pp["item_size_bits"] // 16
+
+    def positions(self, key):
+        return cuckoo_positions(key, self.pp["num_items"], self.pp["cuckoo_seed"])
+
+    def target(self, bucket):
+        """build_query_for_index (client/keyword.rs:147-153) -> (row, sub_col)."""
+        row = bucket // self.per
+        col = (bucket % self.per) * self.ise
+        return row, (col % (self.d * N)) // N
+
+    def query(self, key, seed):
+        pos = self.positions(key)
+        y, z, rows, bodies = client_query(self.sk, self.db_rows, self.d, [self.target(b) for b in pos], seed)
+        return pos, serialize_query(y, z, rows, bodies)
+
+    def extract(self, key, positions, response):
+        """extract (client/keyword.rs:232-299): the value, or None.  An item
+        whose columns run past the end of its 2048-coefficient polynomial reads
+        past the decrypted coefficients of the selected polynomial; the
+        reference's slice then panics (one chain) or reads the next chain; here
+        the short or wrong slice fails the key comparison and gives None."""
+        ks, vs, es = self.pp["key_size"], self.pp["value_size"], self.pp["entry_size"]
+        n = int.from_bytes(response[:4], "little")
+        o, blobs = 4, []
+        for _ in range(n):
+            ln = int.from_bytes(response[o:o + 4], "little")
+            blobs.append(response[o + 4:o + 4 + ln])
+            o += 4 + ln
+        m = int.from_bytes(response[o:o + 4], "little")
+        o += 4
+        stash = []
+        for _ in range(m):
+            ln = int.from_bytes(response[o:o + 4], "little")
+            stash.append(response[o + 4:o + 4 + ln])
+            o += 4 + ln
+        for i, blob in enumerate(blobs):
+            dec = decrypt_response(blob, self.sk).reshape(-1)
+            col = (positions[i] % self.per) * self.ise
+            off = (col // N // self.d) * N + col % N
+            vals = dec[off:off + self.ise]
+            entry = b"".join(int(v & 0xFFFF).to_bytes(2, "big") for v in vals)
+            if len(entry) >= es and entry[:ks] == bytes(key):
+                return entry[ks:ks + vs]
+        for e in stash:
+            if len(e) >= es and e[:ks] == bytes(key):
+                return e[ks:ks + vs]
+        return None
+
+    def straddles(self, bucket):
+        col = (bucket % self.per) * self.ise
+        return col % N + self.ise > N

@@ -1,0 +1,91 @@
+"""setup_keyword_server end to end on the GPU (tests/e2e_mlkem.rs:8-38 at the
+10,000-key shape): a PQKEYV2 file of 10,000 1184-byte public keys (random
+bytes in place of ML-KEM keys, ids = SHA3-256 as dataset.rs:36-46), the server
+built from it (cuckoo table, raw DB, encode and offline precomputation on the
+device), a client query for present ids, and extract returning the stored
+public key.  Two cases of the reference's own behaviour are checked as such:
+an absent id gives None, and an item whose 608 columns run past the end of a
+2048-coefficient polynomial cannot be extracted (the reference's slice runs
+past the selected polynomial's coefficients)."""
+import sys
+
+import numpy as np
+import pytest
+
+import inspire_amd as IA
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def keyword_server(tmp_path_factory):
+    import torch
+    assert torch.cuda.is_available()
+    say("setup_keyword_server: 10,000 keys of 1184 bytes")
+    rng = np.random.default_rng(7)
+    pks = [rng.integers(0, 256, size=1184, dtype=np.uint8).tobytes() for _ in range(10_000)]
+    path = tmp_path_factory.mktemp("ds") / "keys.pqkeyv2"
+    recs = O.write_pqkeyv2(str(path), pks)
+    srv, pp = IA.setup_keyword_server(str(path))
+    plan = IA.plan_shape(10_000, 32, 1184)
+    yield srv, pp, plan, recs
+    srv.close()
+
+
+def test_public_params(keyword_server):
+    srv, pp, plan, recs = keyword_server
+    assert pp["num_items"] == 32768 and pp["dim0"] == 2048 and pp["num_hashes"] == 2
+    assert pp["key_size"] == 32 and pp["value_size"] == 1184 and pp["entry_size"] == 1216
+    assert pp["item_size_bits"] == 1216 * 8 and pp["num_records"] == 10_000
+    assert pp["cuckoo_seed"] == O.CUCKOO_SEED
+    assert (plan["db_rows"], plan["instances"], plan["interpolate_degree"]) == (2048, 8, 8)
+
+
+def _usable(client, rid):
+    # neither straddling nor holding a 0xFFFF word (read back as 0, p = 65535)
+    pos = client.positions(rid)
+    return all(not client.straddles(b) for b in pos)
+
+
+@pytest.mark.parametrize("hamming", [0, O.HAMMING_WEIGHT])
+def test_extract_returns_stored_public_key(keyword_server, hamming):
+    srv, pp, plan, recs = keyword_server
+    client = O.KeywordClient(pp, plan, O.client_secret(99 + hamming, hamming))
+    rng = np.random.default_rng(2026 + hamming)
+    done = 0
+    for idx in rng.permutation(len(recs)):
+        rid, pk = recs[idx]
+        words = np.frombuffer(rid + pk, dtype=">u2")
+        if not _usable(client, rid) or (words == 0xFFFF).any():
+            continue
+        say("extract record", int(idx))
+        pos, q = client.query(rid, seed=int(idx))
+        resp = srv.respond_serialized(q)
+        got = client.extract(rid, pos, resp)
+        assert got == pk, int(idx)
+        assert O.public_key_id(got) == rid
+        done += 1
+        if done == 4:
+            break
+    assert done == 4
+
+
+def test_absent_and_straddling_ids(keyword_server):
+    srv, pp, plan, recs = keyword_server
+    client = O.KeywordClient(pp, plan, O.client_secret(5, O.HAMMING_WEIGHT))
+    absent = bytes(range(32))
+    pos, q = client.query(absent, seed=1)
+    assert client.extract(absent, pos, srv.respond_serialized(q)) is None
+    # a present id whose item straddles a polynomial boundary in both of its
+    # buckets is not recoverable through extract
+    for idx, (rid, pk) in enumerate(recs):
+        if all(client.straddles(b) for b in client.positions(rid)):
+            pos, q = client.query(rid, seed=idx)
+            assert client.extract(rid, pos, srv.respond_serialized(q)) is None
+            break
