@@ -163,6 +163,16 @@ typedef struct inspire_server inspire_server_t;
  * that the MFMA GEMVs read; if that copy cannot be allocated they read the
  * column-major database. */
 int inspire_server_create(const inspire_shape_t* shape, inspire_server_t** out);
+/* A server that keeps only the fragment-ordered copy of bold_t / bold_t_bar
+ * (the MFMA packing's operand, the same size as the condensed pair) and not
+ * the condensed pair itself: half the key-switch memory, so that the 4M-key
+ * ML-KEM-512 state beside a 250k-key ML-KEM-1024 state, or a 32M-key state
+ * at dim0 65536, fit on one GPU.  Only the MFMA packing runs
+ * (INSPIRE_PACKING_VALU is refused).  The setters build the fragments from a
+ * transient condensed copy (set_precomp, setup_offline: bold_t's size again
+ * for the duration of the call) or straight from the generator
+ * (fill_synthetic). */
+int inspire_server_create_lean(const inspire_shape_t* shape, inspire_server_t** out);
 void inspire_server_destroy(inspire_server_t* srv);
 
 /* Encoded DB, column-major [db_cols][db_rows] u16 (cuckoo.rs:314-353 after

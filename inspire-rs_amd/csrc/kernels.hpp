@@ -60,6 +60,10 @@ const char* gemv_batch_variant_name(int v);
 size_t tiled_bold_t_bytes(int64_t instances);
 void launch_tile_bold_t(uint8_t* out, const uint64_t* bt, const uint64_t* btb, const uint16_t* dlog,
                         int64_t instances, hipStream_t s);
+// The same copy of the synthetic state of `seed` (what launch_synth_fill writes
+// to bold_t / bold_t_bar with these stream ids), without the condensed copy.
+void launch_tile_bold_t_synth(uint8_t* out, uint64_t seed, uint32_t stream_bt, uint32_t stream_btb,
+                              const uint16_t* dlog, int64_t instances, hipStream_t s);
 int batch_key_pad(int count);
 // Device bytes of the client-key fragments for up to `count` clients.
 size_t packing_afrag_bytes(int count);

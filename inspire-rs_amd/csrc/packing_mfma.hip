@@ -63,9 +63,26 @@ __device__ inline uint32_t odd_inverse_mod_2n(uint32_t e) {
   return x & (2 * kN - 1);
 }
 
+// Sources of the condensed key-switch terms the tiling reads: device memory,
+// or the counter-based generator (synthetic state, the values that
+// launch_synth_fill would have written, for servers without a condensed copy).
+struct CondensedFromMemory {
+  const uint64_t* bt;
+  const uint64_t* btb;
+  __device__ uint64_t operator()(uint32_t w, size_t idx) const { return (w ? btb : bt)[idx]; }
+};
+struct CondensedFromGenerator {
+  uint64_t seed;
+  uint32_t stream_bt, stream_btb;
+  __device__ uint64_t operator()(uint32_t w, size_t idx) const {
+    const uint64_t h = synth_word(seed, w ? stream_btb : stream_bt, idx);
+    return (((h & 0xffffffffull) * kQ0) >> 32) | ((((h >> 32) * kQ1) >> 32) << 32);
+  }
+};
+
 // One thread per 16-byte fragment row: out[p][z][nt][s][lane][16].
-__global__ void k_tile_bold_t(uint8_t* __restrict__ out, const uint64_t* __restrict__ bt,
-                              const uint64_t* __restrict__ btb, const uint16_t* __restrict__ dlog,
+template <class Src>
+__global__ void k_tile_bold_t(uint8_t* __restrict__ out, Src src, const uint16_t* __restrict__ dlog,
                               int64_t instances) {
   const int64_t ntiles = tiled_ntiles(instances);
   const int64_t total = 2 * (int64_t)kN * ntiles * kKSteps * 64;
@@ -90,7 +107,7 @@ __global__ void k_tile_bold_t(uint8_t* __restrict__ out, const uint64_t* __restr
         const uint32_t term = slot_term(ez_inv, sigma, dlog);
         const int i = (int)(term & 0x3ffu);
         if (i >= kNumRot) continue;
-        const uint64_t v = ((term >> 15) ? btb : bt)[((size_t)j * kNInner + 3 * i + k) * kN + z];
+        const uint64_t v = src(term >> 15, ((size_t)j * kNInner + 3 * i + k) * kN + z);
         const uint32_t res = (uint32_t)(p ? (v >> 32) : v);
         w[t >> 2] |= ((res >> (7 * m)) & 127u) << (8 * (t & 3));
       }
@@ -402,7 +419,14 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE, WPE
 
 void launch_tile_bold_t(uint8_t* out, const uint64_t* bt, const uint64_t* btb, const uint16_t* dlog,
                         int64_t instances, hipStream_t s) {
-  hipLaunchKernelGGL(k_tile_bold_t, dim3(65536), dim3(256), 0, s, out, bt, btb, dlog, instances);
+  hipLaunchKernelGGL(k_tile_bold_t<CondensedFromMemory>, dim3(65536), dim3(256), 0, s, out,
+                     CondensedFromMemory{bt, btb}, dlog, instances);
+}
+
+void launch_tile_bold_t_synth(uint8_t* out, uint64_t seed, uint32_t stream_bt, uint32_t stream_btb,
+                              const uint16_t* dlog, int64_t instances, hipStream_t s) {
+  hipLaunchKernelGGL(k_tile_bold_t<CondensedFromGenerator>, dim3(65536), dim3(256), 0, s, out,
+                     CondensedFromGenerator{seed, stream_bt, stream_btb}, dlog, instances);
 }
 
 // Client padding: 4 and 16 clients for k_packing_mfma, then one group of 32 or

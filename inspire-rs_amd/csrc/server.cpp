@@ -267,6 +267,9 @@ struct inspire_server {
   void* dbfrag = nullptr;  // fragment-ordered DB for the MFMA GEMVs (nullptr: column-major kernels)
   uint64_t *bt = nullptr, *btb = nullptr, *bth = nullptr, *a_hat = nullptr;
   bool have_db = false, have_precomp = false;
+  // Lean servers keep only the fragment-ordered copy of bold_t (MFMA packing
+  // only); the condensed copy exists only while a setter fills the tiles.
+  bool lean = false;
   std::vector<uint8_t> stash;  // entries back to back
   uint32_t stash_entry = 0, stash_count = 0;
   // request slots
@@ -356,7 +359,7 @@ int ensure_tiled(inspire_server_t* s) {
     if (rc) return rc;
     s->tiled_valid = false;
   }
-  if (!s->tiled_valid && s->have_precomp) {
+  if (!s->tiled_valid && s->have_precomp && s->bt && s->btb) {
     launch_tile_bold_t(s->tiled, s->bt, s->btb, s->consts->slot_dlog, s->shape.instances, s->stream);
     HIP_TRY(hipGetLastError(), "tile bold_t launch");
     HIP_TRY(hipStreamSynchronize(s->stream), "tile bold_t");
@@ -401,7 +404,7 @@ int alloc_slots(inspire_server_t* s, int count) {
   return INSPIRE_OK;
 }
 
-extern "C" int inspire_server_create(const inspire_shape_t* shape, inspire_server_t** out) {
+static int server_create(const inspire_shape_t* shape, bool lean, inspire_server_t** out) {
   if (!shape || !out) return set_error(INSPIRE_ERR_INVALID, "null argument");
   *out = nullptr;
   if (shape->db_rows < 2048 || shape->db_rows % 2048 != 0 || shape->db_rows > (1 << 20))
@@ -427,18 +430,23 @@ extern "C" int inspire_server_create(const inspire_shape_t* shape, inspire_serve
   }
   const int64_t inst = shape->instances, rows = shape->db_rows;
   s->max_query_bytes = query_bytes(rows, shape->max_queries);
+  s->lean = lean;
+  if (lean) s->packing_kind = INSPIRE_PACKING_MFMA;
   int rc = INSPIRE_OK;
   if (!rc) rc = dev_alloc(&s->db, (size_t)rows * s->db_cols, "database");
-  if (!rc && rows % 64 == 0 && s->db_cols % 16 == 0 &&
-      hipMalloc(&s->dbfrag, tiled_db_bytes((int)rows, s->db_cols)) != hipSuccess) {
-    s->dbfrag = nullptr;  // the GEMVs then read the column-major DB
-    (void)hipGetLastError();
-  }
-  if (!rc) rc = dev_alloc(&s->bt, (size_t)inst * kNInner * kN, "bold_t");
-  if (!rc) rc = dev_alloc(&s->btb, (size_t)inst * kNInner * kN, "bold_t_bar");
+  if (!rc && lean) rc = dev_alloc(&s->tiled, tiled_bold_t_bytes(inst), "tiled bold_t");
+  if (!rc && !lean) rc = dev_alloc(&s->bt, (size_t)inst * kNInner * kN, "bold_t");
+  if (!rc && !lean) rc = dev_alloc(&s->btb, (size_t)inst * kNInner * kN, "bold_t_bar");
   if (!rc) rc = dev_alloc(&s->bth, (size_t)inst * kTExp * kN, "bold_t_hat");
   if (!rc) rc = dev_alloc(&s->a_hat, (size_t)inst * kN, "a_hat");
   if (!rc) rc = alloc_slots(s, kDefaultSlots);
+  // Optional, last: the fragment-ordered DB copy (the GEMVs read the
+  // column-major DB when it does not fit).
+  if (!rc && rows % 64 == 0 && s->db_cols % 16 == 0 &&
+      hipMalloc(&s->dbfrag, tiled_db_bytes((int)rows, s->db_cols)) != hipSuccess) {
+    s->dbfrag = nullptr;
+    (void)hipGetLastError();
+  }
   if (rc) {
     std::string msg = g_last_error;
     inspire_server_destroy(s);
@@ -447,6 +455,43 @@ extern "C" int inspire_server_create(const inspire_shape_t* shape, inspire_serve
   s->host_out.resize((size_t)shape->max_queries * s->c * kRespBytes);
   *out = s;
   return INSPIRE_OK;
+}
+
+extern "C" int inspire_server_create(const inspire_shape_t* shape, inspire_server_t** out) {
+  return server_create(shape, false, out);
+}
+
+extern "C" int inspire_server_create_lean(const inspire_shape_t* shape, inspire_server_t** out) {
+  return server_create(shape, true, out);
+}
+
+// Lean servers: a condensed bold_t / bold_t_bar pair for the duration of a
+// setter (acquire), then tiled and released (release).
+static int acquire_condensed(inspire_server_t* s) {
+  if (!s->lean || s->bt) return INSPIRE_OK;
+  const size_t n = (size_t)s->shape.instances * kNInner * kN;
+  int rc = dev_alloc(&s->bt, n, "bold_t (transient)");
+  if (!rc) rc = dev_alloc(&s->btb, n, "bold_t_bar (transient)");
+  if (rc) {
+    std::string msg = g_last_error;
+    if (s->bt) (void)hipFree(s->bt);
+    s->bt = nullptr;
+    return set_error(rc, msg);
+  }
+  return INSPIRE_OK;
+}
+
+static int release_condensed(inspire_server_t* s, int rc) {
+  if (!s->lean) return rc;
+  if (!rc) {
+    s->tiled_valid = false;
+    rc = ensure_tiled(s);
+  }
+  (void)hipStreamSynchronize(s->stream);
+  if (s->bt) (void)hipFree(s->bt);
+  if (s->btb) (void)hipFree(s->btb);
+  s->bt = s->btb = nullptr;
+  return rc;
 }
 
 extern "C" void inspire_server_destroy(inspire_server_t* s) {
@@ -473,6 +518,8 @@ extern "C" int inspire_server_set_slots(inspire_server_t* s, int count) {
 extern "C" void* inspire_server_stream(inspire_server_t* s) { return s ? (void*)s->stream : nullptr; }
 
 extern "C" int inspire_server_set_packing_kernel(inspire_server_t* s, int kind) {
+  if (s && s->lean && kind == INSPIRE_PACKING_VALU)
+    return set_error(INSPIRE_ERR_STATE, "lean server: no condensed bold_t, MFMA packing only");
   if (!s || (kind != INSPIRE_PACKING_VALU && kind != INSPIRE_PACKING_MFMA))
     return set_error(INSPIRE_ERR_INVALID, "unknown packing kernel");
   if (kind == INSPIRE_PACKING_MFMA) {
@@ -791,6 +838,17 @@ extern "C" int inspire_server_set_precomp(inspire_server_t* s, const uint64_t* b
   if (!s || !bt || !btb || !bth || !a_hat) return set_error(INSPIRE_ERR_INVALID, "null argument");
   const size_t inst = (size_t)s->shape.instances;
   const hipMemcpyKind k = host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+  if (s->lean) {
+    int rc = acquire_condensed(s);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(s->bt, bt, inst * kNInner * kN * 8, k, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->btb, btb, inst * kNInner * kN * 8, k, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->bth, bth, inst * kTExp * kN * 8, k, s->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->a_hat, a_hat, inst * kN * 8, k, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    s->have_precomp = e == hipSuccess;
+    return release_condensed(s, e == hipSuccess ? INSPIRE_OK : hip_error(e, "copy precomp"));
+  }
   HIP_TRY(hipMemcpyAsync(s->bt, bt, inst * kNInner * kN * 8, k, s->stream), "copy bold_t");
   HIP_TRY(hipMemcpyAsync(s->btb, btb, inst * kNInner * kN * 8, k, s->stream), "copy bold_t_bar");
   HIP_TRY(hipMemcpyAsync(s->bth, bth, inst * kTExp * kN * 8, k, s->stream), "copy bold_t_hat");
@@ -833,8 +891,14 @@ extern "C" int inspire_server_setup_offline(inspire_server_t* s) {
   int64_t group = 0;
   int rc = setup_group(s->shape.instances, &group);
   if (rc) return rc;
+  rc = acquire_condensed(s);
+  if (rc) return rc;
   rc = launch_setup_offline(s->db, s->shape.db_rows, s->shape.instances, s->bt, s->btb, s->bth, s->a_hat,
                             s->consts->tw, group, s->stream);
+  if (s->lean) {
+    s->have_precomp = rc == INSPIRE_OK;
+    return release_condensed(s, rc);
+  }
   if (rc) return rc;
   s->have_precomp = true;
   s->tiled_valid = false;
@@ -913,15 +977,20 @@ extern "C" int inspire_server_fill_synthetic(inspire_server_t* s, uint64_t seed)
   if (!s) return set_error(INSPIRE_ERR_INVALID, "null server");
   const uint64_t inst = (uint64_t)s->shape.instances;
   launch_synth_fill(s->db, seed, kStreamDb, 0, (uint64_t)s->shape.db_rows * s->db_cols, kSynthU16, s->stream);
-  launch_synth_fill(s->bt, seed, kStreamBt, 0, inst * kNInner * kN, kSynthCondensed, s->stream);
-  launch_synth_fill(s->btb, seed, kStreamBtb, 0, inst * kNInner * kN, kSynthCondensed, s->stream);
+  if (s->lean) {
+    launch_tile_bold_t_synth(s->tiled, seed, kStreamBt, kStreamBtb, s->consts->slot_dlog, s->shape.instances,
+                             s->stream);
+  } else {
+    launch_synth_fill(s->bt, seed, kStreamBt, 0, inst * kNInner * kN, kSynthCondensed, s->stream);
+    launch_synth_fill(s->btb, seed, kStreamBtb, 0, inst * kNInner * kN, kSynthCondensed, s->stream);
+  }
   launch_synth_fill(s->bth, seed, kStreamBth, 0, inst * kTExp * kN, kSynthCondensed, s->stream);
   launch_synth_fill(s->a_hat, seed, kStreamAhat, 0, inst * kN, kSynthModQ, s->stream);
   if (s->dbfrag) launch_tile_db(s->dbfrag, s->db, (int)s->shape.db_rows, s->db_cols, s->stream);
   HIP_TRY(hipGetLastError(), "synthetic fill launch");
   HIP_TRY(hipStreamSynchronize(s->stream), "synthetic fill");
   s->have_db = s->have_precomp = true;
-  s->tiled_valid = false;
+  s->tiled_valid = s->lean;
   return s->tiled ? ensure_tiled(s) : INSPIRE_OK;
 }
 

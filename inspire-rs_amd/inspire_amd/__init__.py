@@ -74,6 +74,7 @@ def lib():
         L.inspire_last_error.restype = ctypes.c_char_p
         L.inspire_plan_shape.argtypes = [i64, i64, i64, i64, i64, ctypes.POINTER(PlanT)]
         L.inspire_server_create.argtypes = [ctypes.POINTER(ShapeT), ctypes.POINTER(vp)]
+        L.inspire_server_create_lean.argtypes = [ctypes.POINTER(ShapeT), ctypes.POINTER(vp)]
         L.inspire_server_destroy.argtypes = [vp]
         L.inspire_server_destroy.restype = None
         L.inspire_server_set_db.argtypes = [vp, vp, i32]
@@ -157,12 +158,13 @@ def _ptr(t):
 class Server:
     """KeywordServer answer path resident on one GPU (keyword.rs:49-63, 534-640)."""
 
-    def __init__(self, db_rows, instances, interpolate_degree, max_queries=2):
+    def __init__(self, db_rows, instances, interpolate_degree, max_queries=2, lean=False):
         self.db_rows, self.instances, self.d = db_rows, instances, interpolate_degree
         self.c = instances // interpolate_degree
         shape = ShapeT(db_rows, instances, interpolate_degree, max_queries)
         h = ctypes.c_void_p()
-        check(lib().inspire_server_create(ctypes.byref(shape), ctypes.byref(h)))
+        create = lib().inspire_server_create_lean if lean else lib().inspire_server_create
+        check(create(ctypes.byref(shape), ctypes.byref(h)))
         self.h = h
 
     @classmethod

@@ -153,3 +153,46 @@ def test_gpu_encode_setup_respond_decrypts(torch, real):
     for (row, sub), blob in zip(targets, split_response(resp)):
         assert np.array_equal(O.decrypt_response(blob, sk), expected(rawm, row, sub)), (row, sub)
     srv.close()
+
+
+def test_lean_server_matches_full_server(torch, real):
+    # A lean server (fragment-ordered bold_t only) answers with the same bytes
+    # as the full server: from synthetic state, from set_precomp and from the
+    # device setup; it refuses the VALU packing.
+    rawm, enc, st, srv = real
+    say("lean server")
+    lean = IA.Server(DB_ROWS, INST, D, lean=True)
+    full = IA.Server(DB_ROWS, INST, D)
+    with pytest.raises(IA.InspireError):
+        lean.set_packing_kernel(IA.PACKING_VALU)
+    sk, parts, q = client(77, O.HAMMING_WEIGHT, [(9, 3), (700, 6)])
+    # synthetic state
+    lean.fill_synthetic(5)
+    full.fill_synthetic(5)
+    full.set_packing_kernel(IA.PACKING_MFMA)
+    assert lean.respond_serialized(q) == full.respond_serialized(q)
+    # real state through set_precomp, then through the device setup
+    lean.set_db(enc)
+    lean.set_precomp(*st)
+    exp = srv.respond_serialized(q)
+    assert lean.respond_serialized(q) == exp
+    lean.encode_db(np.ascontiguousarray(rawm).reshape(-1))
+    lean.setup_offline()
+    assert lean.respond_serialized(q) == exp
+    # batched on the lean server
+    lean.set_max_batch(8)
+    qlen = len(q)
+    stride = (qlen + 16 + 255) // 256 * 256
+    host = np.zeros(stride * 3, dtype=np.uint8)
+    for i in range(3):
+        host[i * stride:i * stride + qlen] = np.frombuffer(q, dtype=np.uint8)
+    blob = 2 * (INST // D) * O.RESP_BYTES
+    dout = torch.zeros(3 * blob, dtype=torch.uint8, device="cuda")
+    lean.respond_batch_device(torch.from_numpy(host).cuda(), stride, qlen, 2, 3, dout, blob)
+    torch.cuda.synchronize()
+    out = dout.cpu().numpy().tobytes()
+    body = split_response(exp)
+    for i in range(3):
+        assert out[i * blob:(i + 1) * blob] == body[0] + body[1]
+    lean.close()
+    full.close()
