@@ -107,6 +107,121 @@ def serve_poisson(IA, eng, blobs, rate, n, seed):
             "mean_batch": (b1["queries"] - b0["queries"]) / max(nb, 1)}
 
 
+def bench_mixed(IA, torch, rounds=12, seed=37):
+    """Two databases resident side by side on lean servers (fragment-ordered
+    bold_t only): ML-KEM-512 x 4M (800-byte keys) and ML-KEM-1024 x 250k
+    (1568-byte keys), synthetic state.  Each round draws a batch of 1-256
+    keyword queries, splits it at random between the two databases and runs
+    both shares in the same pass, each on its own stream.  Per database: the
+    algorithmic bytes of its share (its DB once, its bold_t once) over the
+    pass time."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    dbs = []
+    for name, recs, vs in (("ML-KEM-512 x 4M", 4_000_000, 800), ("ML-KEM-1024 x 250k", 250_000, 1568)):
+        plan = IA.plan_shape(recs, KEY, vs)
+        t0 = time.time()
+        srv = IA.Server(plan["db_rows"], plan["instances"], plan["interpolate_degree"], lean=True)
+        srv.fill_synthetic(17)
+        srv.set_max_batch(256)
+        srv.set_graphs(False)
+        dq, qlen, stride = make_queries(IA, torch, plan["db_rows"], 8, 29)
+        dq = dq.repeat(32)[:256 * stride].contiguous()
+        blob = 2 * plan["c"] * IA.RESP_BYTES
+        out = torch.empty(256 * blob, dtype=torch.uint8, device="cuda")
+        dbs.append({"name": name, "plan": plan, "srv": srv, "dq": dq, "qlen": qlen, "stride": stride, "blob": blob,
+                    "out": out, "stream": torch.cuda.Stream(), "queries": 0, "bytes": 0.0})
+        log("mixed: %s resident in %.1fs (instances %d)" % (name, time.time() - t0, plan["instances"]))
+
+    def run(counts):
+        for db, n in zip(dbs, counts):
+            if n:
+                db["srv"].respond_batch_device(db["dq"], db["stride"], db["qlen"], 2, n, db["out"], db["blob"],
+                                               db["stream"].cuda_stream)
+        torch.cuda.synchronize()
+
+    run([64, 64])  # warm-up
+    passes = []
+    for r in range(rounds):
+        total = int(rng.integers(1, 257))
+        n0 = int(rng.binomial(total, 0.5))
+        counts = [n0, total - n0]
+        t0 = time.perf_counter()
+        run(counts)
+        dt = time.perf_counter() - t0
+        passes.append({"queries": counts, "ms": dt * 1e3})
+        for db, n in zip(dbs, counts):
+            if n:
+                db["queries"] += n
+                db["bytes"] += db["plan"]["db_bytes"] + db["plan"]["bold_t_bytes"]
+    wall = sum(x["ms"] for x in passes) / 1e3
+    res = {"workload": "two lean servers on one GPU; batches of 1-256 keyword queries split at random between them, "
+                       "both shares in one pass on two streams",
+           "rounds": passes, "qps": sum(sum(x["queries"]) for x in passes) / wall, "dbs": []}
+    for db in dbs:
+        res["dbs"].append({"db": db["name"], "instances": db["plan"]["instances"], "queries": db["queries"],
+                           "algorithmic_GBps": db["bytes"] / wall / 1e9,
+                           "db_bytes": db["plan"]["db_bytes"], "bold_t_bytes": db["plan"]["bold_t_bytes"]})
+        db["srv"].close()
+    log("mixed: %.0f q/s over %d passes; %s" % (res["qps"], rounds, ", ".join(
+        "%s %.0f GB/s" % (d["db"], d["algorithmic_GBps"]) for d in res["dbs"])))
+    return res
+
+
+def bench_32m(IA, torch, rates=(100, 300, 600), serve_seconds=2.0):
+    """32M ML-KEM-768 keys at dim0 65536 (DB 94.5 GB, bold_t 35.4 GB) on a lean
+    server, synthetic state: single-query respond, batched respond and
+    continuous-batching service under Poisson arrivals."""
+    plan = IA.plan_shape(32_000_000, KEY, VALUE, dim0=65536)
+    db_rows, inst, d = plan["db_rows"], plan["instances"], plan["interpolate_degree"]
+    t0 = time.time()
+    srv = IA.Server(db_rows, inst, d, lean=True)
+    srv.fill_synthetic(19)
+    log("32m: state resident in %.1fs (db_rows %d, instances %d)" % (time.time() - t0, db_rows, inst))
+    dq, qlen, stride, blobs = make_queries(IA, torch, db_rows, 4, 31, return_host=True)
+    blob = 2 * plan["c"] * IA.RESP_BYTES
+    out = torch.empty(64 * blob, dtype=torch.uint8, device="cuda")
+    st = torch.cuda.Stream()
+    for k in range(2):
+        srv.respond_device(dq.data_ptr() + (k % 4) * stride, qlen, 2, out, st.cuda_stream)
+    torch.cuda.synchronize()
+    steps = 10
+    t0 = time.perf_counter()
+    for k in range(steps):
+        srv.respond_device(dq.data_ptr() + (k % 4) * stride, qlen, 2, out, st.cuda_stream)
+    torch.cuda.synchronize()
+    single_ms = (time.perf_counter() - t0) * 1e3 / steps
+    res = {"workload": "32M ML-KEM-768 keys, dim0 65536, lean server",
+           "db_rows": db_rows, "db_cols": plan["db_cols"], "instances": inst, "db_bytes": plan["db_bytes"],
+           "bold_t_bytes": plan["bold_t_bytes"], "single_query": {"ms": single_ms, "qps": 1e3 / single_ms},
+           "batched": {}, "serving": []}
+    log("32m: single query %.2f ms (%.1f q/s)" % (single_ms, 1e3 / single_ms))
+    srv.set_max_batch(64)
+    for B in (8, 32, 64):
+        bq = dq.repeat(B // 4 + 1)[:B * stride].contiguous()
+        srv.respond_batch_device(bq, stride, qlen, 2, B, out, blob, st.cuda_stream)
+        torch.cuda.synchronize()
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            srv.respond_batch_device(bq, stride, qlen, 2, B, out, blob, st.cuda_stream)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        res["batched"][str(B)] = {"ms_per_batch": ms, "qps": B * 1e3 / ms, "speedup_vs_single_query": B * single_ms / ms}
+        log("32m: batch %d: %.1f ms, %.0f q/s" % (B, ms, B * 1e3 / ms))
+        del bq
+    eng = IA.Engine(srv, nq=2, max_batch=64, window_ms=0.0, capacity=256)
+    for rate in rates:
+        n = int(min(1200, max(50, rate * serve_seconds)))
+        r = serve_poisson(IA, eng, blobs, float(rate), n, 41)
+        res["serving"].append(r)
+        log("32m: serve %.0f q/s offered: %.1f q/s done, p50 %.1f ms, p99 %.1f ms, mean batch %.1f, dropped %d" %
+            (rate, r["achieved_qps"], r["p50_ms"], r["p99_ms"], r["mean_batch"], r["dropped"]))
+    eng.close()
+    srv.close()
+    return res
+
+
 def cpu_baseline(plan, out_samples=32, fold_chains=2):
     """Times the CPU oracle (restatement of the reference's respond path) on a
     bounded sample and scales it to one keyword query (2 hash queries)."""
@@ -195,6 +310,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrivals per rate span about this long")
     ap.add_argument("--no-encode", action="store_true", help="skip the DB encode and offline setup measurements")
     ap.add_argument("--no-setup", action="store_true", help="skip the offline setup measurement")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the two-database (4M + 250k) configuration")
+    ap.add_argument("--no-32m", action="store_true", help="skip the 32M-key dim0-65536 configuration")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default batched kernels")
     ap.add_argument("--packing", choices=("mfma", "valu"), default="mfma",
                     help="single-query packing kernel: int8 MFMA over the fragment-ordered bold_t, or the VALU "
@@ -434,6 +551,18 @@ def main():
     srv.close()
     del outs, dq
     torch.cuda.empty_cache()
+    # The other configurations, each on its own (lean) servers after the 1M one
+    # is gone: two databases side by side, and the 32M-key serving shape.
+    for key, flag, fn in (("mixed_dbs", args.no_mixed, bench_mixed), ("serving_32m", args.no_32m, bench_32m)):
+        if flag:
+            continue
+        try:
+            result[key] = fn(IA, torch)
+        except Exception as e:  # a configuration that does not fit is reported, not fatal
+            result[key] = {"error": str(e)}
+            log("%s failed: %s" % (key, e))
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(plan)

@@ -196,3 +196,29 @@ def test_lean_server_matches_full_server(torch, real):
         assert out[i * blob:(i + 1) * blob] == body[0] + body[1]
     lean.close()
     full.close()
+
+
+def test_dim0_65536_shape_decrypts(torch):
+    # The 32M-key serving shape has db_rows = dim0 = 65536 (32 query RLWEs per
+    # hash query, a 65536-term GEMV): device encode + setup + respond at that
+    # row count with 2 instances, against the oracle and through decryption.
+    rows, inst, d = 65536, 2, 2
+    say("dim0 65536: 65536 x 2 instances, d 2")
+    rng = np.random.default_rng(65536)
+    raw = rng.integers(0, 65535, size=inst * N * rows, dtype=np.uint16)
+    enc = O.encode(raw, rows, inst * N, d)
+    st = O.setup_state(enc, rows, inst)
+    srv = IA.Server(rows, inst, d, lean=True)
+    srv.encode_db(raw)
+    srv.setup_offline()
+    sk = O.client_secret(3, O.HAMMING_WEIGHT)
+    targets = [(65535, 1), (12345, 0)]
+    y, z, qrows, bodies = O.client_query(sk, rows, d, targets, seed=4)
+    q = O.serialize_query(y, z, qrows, bodies)
+    resp = srv.respond_serialized(q)
+    assert resp == O.serialize_response(O.respond_raw(enc, rows, inst, d, *st, y, z, qrows, bodies), [])
+    rawm = raw.reshape(inst * N, rows)
+    for (row, sub), blob in zip(targets, split_response(resp)):
+        exp = np.stack([rawm[(w * d + sub) * N:(w * d + sub + 1) * N, row] for w in range(inst // d)])
+        assert np.array_equal(O.decrypt_response(blob, sk), exp.astype(np.uint64)), (row, sub)
+    srv.close()
