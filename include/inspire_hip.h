@@ -199,6 +199,11 @@ int inspire_server_set_precomp(inspire_server_t* srv, const uint64_t* bold_t, co
  * bold_t, bold_t_bar, bold_t_hat and a_hat as inspire_server_set_precomp
  * would.  Needs the DB (set_db or encode_db) first.  Synchronous. */
 int inspire_server_setup_offline(inspire_server_t* srv);
+/* Exchanges the resident state (DB, precomputation, fragment copies, stash)
+ * of two servers of the same shape on the same device, after waiting for
+ * their device work; the live-update step without an engine.  Calls on
+ * either handle must not overlap it. */
+int inspire_server_swap_state(inspire_server_t* live, inspire_server_t* staged);
 /* Cuckoo stash entries returned with every response (keyword.rs:624-627). */
 int inspire_server_set_stash(inspire_server_t* srv, const uint8_t* entries, uint32_t entry_size,
                              uint32_t count);
@@ -342,21 +347,6 @@ int inspire_post_device(uint64_t* d_P, const uint32_t* d_S, int64_t count, void*
 int inspire_fold_device(uint8_t* d_out, const uint64_t* d_a_hat, const uint64_t* d_c, const uint64_t* d_P,
                         const uint32_t* d_bodies, int64_t instances, int d, int nq, void* stream);
 
-/* ------------------------------------------------------------------------
- * Continuous-batching engine (no counterpart in the reference, whose server
- * loop answers one query per round trip, server.rs:116-158).  Callers submit
- * serialized keyword queries (nq hash queries each) from any thread; a
- * dispatcher packs the pending ones, up to max_batch, into one batched respond,
- * waiting up to window_ms for more arrivals when fewer than max_batch are
- * pending (0: dispatch at once).  A second batch goes in flight (its upload
- * overlapping the running one) only when a full batch is pending.
- * max_batch is capped at creation so the batch buffers fit in the free device
- * memory (less 1 GiB); inspire_engine_stats reports the cap.  capacity bounds
- * the submitted-but-not-waited queries (submit returns INSPIRE_ERR_CAPACITY
- * past it).  Every ticket is waited exactly once; wait writes the serialized
- * response (stash included) and the submit-to-ready latency.  The server must
- * not be used by other callers while an engine runs on it.
- * ---------------------------------------------------------------------- */
 /* ---------------------------------------------------------------------- */
 /* Keyword layer: PQKEYV2 datasets, cuckoo table, server setup             */
 /* ---------------------------------------------------------------------- */
@@ -397,6 +387,24 @@ int inspire_cuckoo_positions(const uint8_t* key, size_t key_len, int64_t num_buc
  * write, finish.  Host only. */
 uint64_t inspire_siphash24(uint64_t k0, uint64_t k1, const uint8_t* msg, size_t len);
 
+/* ------------------------------------------------------------------------
+ * Continuous-batching engine (no counterpart in the reference, whose server
+ * loop answers one query per round trip, server.rs:116-158).  Callers submit
+ * serialized keyword queries (nq hash queries each) from any thread; a
+ * dispatcher packs the pending ones, up to max_batch, into one batched respond,
+ * waiting up to window_ms for more arrivals when fewer than max_batch are
+ * pending (0: dispatch at once).  A second batch goes in flight (its upload
+ * overlapping the running one) only when a full batch is pending.
+ * max_batch is capped at creation so the batch buffers fit in the free device
+ * memory (less 1 GiB); inspire_engine_stats reports the cap.  capacity bounds
+ * the submitted-but-not-waited queries (submit returns INSPIRE_ERR_CAPACITY
+ * past it).  Every ticket is waited exactly once; wait writes the serialized
+ * response (stash included) and the submit-to-ready latency.  When cap is too
+ * small, wait returns INSPIRE_ERR_CAPACITY with the size needed in *resp_len
+ * and the ticket stays valid (after a state swap an answer keeps the stash of
+ * the state it ran on, which may be larger than the current one).  The server
+ * must not be used by other callers while an engine runs on it.
+ * ---------------------------------------------------------------------- */
 typedef struct inspire_engine inspire_engine_t;
 int inspire_engine_create(inspire_server_t* srv, int nq, int max_batch, double window_ms, int capacity,
                           inspire_engine_t** out);
@@ -404,6 +412,15 @@ int inspire_engine_submit(inspire_engine_t* eng, const uint8_t* query, size_t le
 int inspire_engine_wait(inspire_engine_t* eng, uint64_t ticket, uint8_t* resp, size_t cap, size_t* resp_len,
                         double* latency_ms);
 int inspire_engine_stats(inspire_engine_t* eng, int64_t* batches, int64_t* queries, int* max_batch);
+/* Live DB update (the reference's fold-then-swap of the encoded DB and the
+ * precomputation: encode_kernel.cu:453-516, packing_online_kernel.cu:310-333).
+ * Build the new state on a second server of the same shape (set_db or
+ * encode_db, set_precomp or setup_offline, set_stash), then swap: new batches
+ * are held back, the running ones finish, the engine's server and `staged`
+ * exchange their resident state, and the held queries run on the new state.
+ * `staged` then holds the old state (destroy it, or reuse it for the next
+ * update).  Queries are never answered from a mix of the two states. */
+int inspire_engine_swap_state(inspire_engine_t* eng, inspire_server_t* staged);
 /* Finishes the queued and running batches, then frees the engine. */
 void inspire_engine_destroy(inspire_engine_t* eng);
 

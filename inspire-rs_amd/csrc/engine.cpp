@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <algorithm>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -37,6 +38,16 @@ struct Request {
   ReqState state = kFree;
   Clock::time_point t_submit, t_done;
   std::string error;
+  uint64_t gen = 0;  // state generation the request ran on
+};
+
+// The stash of a state generation that was swapped out while some of its
+// answers were still to be collected (responses carry the stash of the state
+// they were computed on).
+struct StashSnapshot {
+  uint32_t entry = 0, count = 0;
+  std::vector<uint8_t> bytes;
+  int64_t unwaited = 0;
 };
 
 struct InFlight {
@@ -74,6 +85,10 @@ struct inspire_engine {
   std::condition_variable cv_work, cv_done;
   bool stop = false;
   bool dispatching = false;  // a batch has left `pending` but is not yet in `inflight`
+  bool paused = false;       // a state swap holds new batches back
+  uint64_t gen = 0;          // state generation (incremented by each swap)
+  std::unordered_map<uint64_t, int64_t> gen_unwaited;      // dispatched, not yet waited, per generation
+  std::unordered_map<uint64_t, StashSnapshot> old_stash;  // generations swapped out with answers pending
   std::thread dispatcher, completer;
 
   void dispatch_loop();
@@ -113,7 +128,8 @@ void inspire_engine::dispatch_loop() {
         return n;
       };
       cv_work.wait(lk, [&] {
-        return stop || (!pending.empty() && free_set() >= 0 && (busy() == 0 || (int)pending.size() >= max_batch));
+        return stop || (!paused && !pending.empty() && free_set() >= 0 &&
+                        (busy() == 0 || (int)pending.size() >= max_batch));
       });
       if (stop) return;
       if (window_ms > 0 && (int)pending.size() < max_batch) {
@@ -129,6 +145,8 @@ void inspire_engine::dispatch_loop() {
         const int sl = pending.front();
         pending.pop_front();
         reqs[sl].state = kRunning;
+        reqs[sl].gen = gen;
+        ++gen_unwaited[gen];
         batch.push_back(sl);
       }
       set_busy[set] = true;
@@ -340,15 +358,39 @@ extern "C" int inspire_engine_wait(inspire_engine_t* g, uint64_t ticket, uint8_t
   if (it == g->by_ticket.end()) return set_error(INSPIRE_ERR_INVALID, "unknown ticket");
   const int sl = it->second;
   Request& r = g->reqs[sl];
-  *resp_len = inspire_server_response_bytes(g->srv, g->nq);
+  // the largest framing this request can get: its generation's stash, or the
+  // current one if it has not run yet
+  auto framing = [&](const StashSnapshot** old) {
+    *old = nullptr;
+    if (r.state != kQueued) {
+      auto o = g->old_stash.find(r.gen);
+      if (o != g->old_stash.end()) *old = &o->second;
+    }
+    return *old ? response_bytes_with_stash(g->nq, g->info.blob / (size_t)g->nq, (*old)->entry, (*old)->count)
+                : inspire_server_response_bytes(g->srv, g->nq);
+  };
+  const StashSnapshot* old = nullptr;
+  *resp_len = std::max(framing(&old), inspire_server_response_bytes(g->srv, g->nq));
   if (!resp || cap < *resp_len) return set_error(INSPIRE_ERR_CAPACITY, "response buffer too small");
   g->cv_done.wait(lk, [&] { return r.state == kDone || r.state == kFailed; });
   int rc = INSPIRE_OK;
+  (void)framing(&old);
   if (r.state == kFailed) {
     rc = set_error(INSPIRE_ERR_HIP, r.error);
   } else {
-    server_write_response(g->srv, g->nq, g->out_store.data() + (size_t)sl * g->info.blob, resp);
+    const uint8_t* blobs = g->out_store.data() + (size_t)sl * g->info.blob;
+    *resp_len = old ? write_response(g->nq, g->info.blob / (size_t)g->nq, blobs, old->entry, old->count,
+                                     old->bytes.data(), resp)
+                    : server_write_response(g->srv, g->nq, blobs, resp);
     if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r.t_done - r.t_submit).count();
+  }
+  if (r.state == kDone || r.state == kFailed) {
+    // this request's generation has one answer fewer to collect
+    auto u = g->gen_unwaited.find(r.gen);
+    if (u != g->gen_unwaited.end() && --u->second == 0) {
+      g->gen_unwaited.erase(u);
+      g->old_stash.erase(r.gen);
+    }
   }
   g->by_ticket.erase(it);
   r.state = kFree;
@@ -363,6 +405,29 @@ extern "C" int inspire_engine_stats(inspire_engine_t* g, int64_t* batches, int64
   if (queries) *queries = g->queries;
   if (max_batch) *max_batch = g->max_batch;
   return INSPIRE_OK;
+}
+
+// Live DB update under the engine: new batches are held back, the running
+// ones finish, the server exchanges its state with `staged`
+// (inspire_server_swap_state), and the held queries run on the new state.
+extern "C" int inspire_engine_swap_state(inspire_engine_t* g, inspire_server_t* staged) {
+  if (!g || !staged) return set_error(INSPIRE_ERR_INVALID, "null argument");
+  std::unique_lock<std::mutex> lk(g->mu);
+  if (g->paused) return set_error(INSPIRE_ERR_INVALID, "a state swap is already running on this engine");
+  g->paused = true;
+  g->cv_done.wait(lk, [&] { return g->inflight.empty() && !g->dispatching; });
+  StashSnapshot snap;
+  server_stash(g->srv, &snap.entry, &snap.count, &snap.bytes);
+  int rc = inspire_server_swap_state(g->srv, staged);
+  if (rc == INSPIRE_OK) rc = server_info(g->srv, g->nq, &g->info);
+  if (rc == INSPIRE_OK) {
+    // answers computed on the old state keep its stash
+    if (g->gen_unwaited.count(g->gen)) g->old_stash[g->gen] = std::move(snap);
+    ++g->gen;
+  }
+  g->paused = false;
+  g->cv_work.notify_all();
+  return rc;
 }
 
 extern "C" void inspire_engine_destroy(inspire_engine_t* g) {

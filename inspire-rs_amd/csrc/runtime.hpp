@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 namespace inspire {
 
@@ -42,5 +43,12 @@ size_t server_batch_bytes(const inspire_server* s, int max);
 // serialize_keyword_response (wire.rs:708-726) of nq device blobs (host copy)
 // plus the stash; returns the bytes written.
 size_t server_write_response(const inspire_server* s, int nq, const uint8_t* blobs, uint8_t* resp);
+// The same with an explicit stash (count entries of `entry` bytes) and
+// per-hash-query blob size; and the size it writes.
+size_t write_response(int nq, size_t blob, const uint8_t* blobs, uint32_t entry, uint32_t count,
+                      const uint8_t* stash, uint8_t* resp);
+size_t response_bytes_with_stash(int nq, size_t blob, uint32_t entry, uint32_t count);
+// A copy of the server's stash.
+void server_stash(const inspire_server* s, uint32_t* entry, uint32_t* count, std::vector<uint8_t>* bytes);
 
 }  // namespace inspire

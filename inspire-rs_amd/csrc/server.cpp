@@ -494,6 +494,50 @@ static int release_condensed(inspire_server_t* s, int rc) {
   return rc;
 }
 
+// Live DB update (encode_kernel.cu:453-516 fold + swap, packing_online_kernel.cu:
+// 310-333 swap_precomp): `staged` was built (set_db / encode_db, set_precomp /
+// setup_offline, stash) while `live` served; the two exchange their resident
+// state.  Waits for both handles' device work; drops captured batch graphs,
+// whose kernels hold the old buffers.
+extern "C" int inspire_server_swap_state(inspire_server_t* live, inspire_server_t* staged) {
+  if (!live || !staged || live == staged) return set_error(INSPIRE_ERR_INVALID, "two distinct servers needed");
+  const inspire_shape_t &a = live->shape, &b = staged->shape;
+  if (a.db_rows != b.db_rows || a.instances != b.instances || a.interpolate_degree != b.interpolate_degree ||
+      live->device != staged->device)
+    return set_error(INSPIRE_ERR_INVALID, "swap needs servers of the same shape on the same device");
+  if (!staged->have_db || !staged->have_precomp)
+    return set_error(INSPIRE_ERR_STATE, "staged server has no database or precomputation");
+  for (inspire_server_t* s : {live, staged}) {
+    HIP_TRY(hipStreamSynchronize(s->stream), "swap: drain server stream");
+    for (Slot& sl : s->slots)
+      if (sl.done) HIP_TRY(hipEventSynchronize(sl.done), "swap: drain request slot");
+    if (s->batch.done) HIP_TRY(hipEventSynchronize(s->batch.done), "swap: drain batch");
+    free_graphs(s);
+  }
+  std::swap(live->db, staged->db);
+  std::swap(live->dbfrag, staged->dbfrag);
+  std::swap(live->bt, staged->bt);
+  std::swap(live->btb, staged->btb);
+  std::swap(live->bth, staged->bth);
+  std::swap(live->a_hat, staged->a_hat);
+  std::swap(live->tiled, staged->tiled);
+  std::swap(live->tiled_valid, staged->tiled_valid);
+  std::swap(live->lean, staged->lean);
+  std::swap(live->have_db, staged->have_db);
+  std::swap(live->have_precomp, staged->have_precomp);
+  std::swap(live->stash, staged->stash);
+  std::swap(live->stash_entry, staged->stash_entry);
+  std::swap(live->stash_count, staged->stash_count);
+  for (inspire_server_t* s : {live, staged}) {
+    if (s->lean || !s->bt) s->packing_kind = INSPIRE_PACKING_MFMA;
+    if (s->packing_kind == INSPIRE_PACKING_MFMA && !s->tiled_valid) {
+      int rc = ensure_tiled(s);
+      if (rc) return rc;
+    }
+  }
+  return INSPIRE_OK;
+}
+
 extern "C" void inspire_server_destroy(inspire_server_t* s) {
   if (!s) return;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
@@ -1108,8 +1152,23 @@ extern "C" int inspire_respond_serialized(inspire_server_t* s, const uint8_t* qu
 namespace inspire {
 
 size_t server_write_response(const inspire_server* s, int nq, const uint8_t* blobs, uint8_t* resp) {
+  return write_response(nq, (size_t)s->c * kRespBytes, blobs, s->stash_entry, s->stash_count, s->stash.data(),
+                        resp);
+}
+
+size_t response_bytes_with_stash(int nq, size_t blob, uint32_t entry, uint32_t count) {
+  return 4 + (size_t)nq * (4 + blob) + 4 + (size_t)count * (4 + entry);
+}
+
+void server_stash(const inspire_server* s, uint32_t* entry, uint32_t* count, std::vector<uint8_t>* bytes) {
+  *entry = s->stash_entry;
+  *count = s->stash_count;
+  *bytes = s->stash;
+}
+
+size_t write_response(int nq, size_t blob, const uint8_t* blobs, uint32_t stash_entry, uint32_t stash_count,
+                      const uint8_t* stash, uint8_t* resp) {
   // serialize_keyword_response (wire.rs:708-726)
-  const size_t blob = (size_t)s->c * kRespBytes;
   size_t o = 0;
   auto put32 = [&](uint32_t v) {
     std::memcpy(resp + o, &v, 4);
@@ -1121,11 +1180,11 @@ size_t server_write_response(const inspire_server* s, int nq, const uint8_t* blo
     std::memcpy(resp + o, blobs + (size_t)h * blob, blob);
     o += blob;
   }
-  put32(s->stash_count);
-  for (uint32_t i = 0; i < s->stash_count; ++i) {
-    put32(s->stash_entry);
-    std::memcpy(resp + o, s->stash.data() + (size_t)i * s->stash_entry, s->stash_entry);
-    o += s->stash_entry;
+  put32(stash_count);
+  for (uint32_t i = 0; i < stash_count; ++i) {
+    put32(stash_entry);
+    std::memcpy(resp + o, stash + (size_t)i * stash_entry, stash_entry);
+    o += stash_entry;
   }
   return o;
 }

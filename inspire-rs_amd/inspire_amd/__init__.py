@@ -83,6 +83,8 @@ def lib():
         L.inspire_server_set_precomp.argtypes = [vp, vp, vp, vp, vp, i32]
         L.inspire_server_set_stash.argtypes = [vp, vp, u32, u32]
         L.inspire_server_setup_offline.argtypes = [vp]
+        L.inspire_server_swap_state.argtypes = [vp, vp]
+        L.inspire_engine_swap_state.argtypes = [vp, vp]
         L.inspire_setup_keyword_server.argtypes = [ctypes.c_char_p, i64, i64, ctypes.POINTER(vp),
                                                    ctypes.POINTER(PublicParamsT)]
         L.inspire_setup_keyword_records.argtypes = [vp, u32, vp, u32, i64, i64, i64, ctypes.POINTER(vp),
@@ -218,6 +220,10 @@ class Server:
         check(lib().inspire_server_set_precomp(self.h, _ptr(bt), _ptr(btb), _ptr(bth), _ptr(a_hat),
                                                1 if host else 0))
 
+    def swap_state(self, staged):
+        """Live DB update: exchange resident state with a server of the same shape."""
+        check(lib().inspire_server_swap_state(self.h, staged.h))
+
     def setup_offline(self):
         """KeywordServer::setup's offline precomputation on the device from the
         server's DB (keyword.rs:427-458)."""
@@ -301,11 +307,20 @@ class Engine:
 
     def wait(self, ticket):
         cap = self.server.response_bytes(self.nq)
-        buf = ctypes.create_string_buffer(cap)
         n = ctypes.c_size_t()
         lat = ctypes.c_double()
-        check(lib().inspire_engine_wait(self.h, ticket, buf, cap, ctypes.byref(n), ctypes.byref(lat)))
-        return buf.raw[:n.value], lat.value
+        while True:
+            buf = ctypes.create_string_buffer(cap)
+            rc = lib().inspire_engine_wait(self.h, ticket, buf, cap, ctypes.byref(n), ctypes.byref(lat))
+            if rc == ERR_CAPACITY and n.value > cap:
+                cap = n.value  # an answer framed with a larger (earlier) stash
+                continue
+            check(rc)
+            return buf.raw[:n.value], lat.value
+
+    def swap_state(self, staged):
+        """Live DB update under the engine (inspire_engine_swap_state)."""
+        check(lib().inspire_engine_swap_state(self.h, staged.h))
 
     def stats(self):
         b, q, m = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()

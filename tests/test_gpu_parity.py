@@ -664,3 +664,54 @@ def test_legacy_encode_then_gemv(torch):
     L.gpu_encode_free_device_db()
     assert not L.gpu_encode_get_device_db()
     L.gpu_trim_memory_pool()
+
+
+def test_engine_live_update_swaps_state(torch):
+    # Live DB update: a staged server of the same shape with another DB,
+    # precomputation and stash replaces the engine's state between batches.
+    # Every answer equals the same query answered alone on the state it ran
+    # on, stash included, and never mixes the two.
+    say("engine live update")
+    import time as _time
+    db_rows, inst, d = 2048, 8, 4
+    live = IA.Server(db_rows, inst, d)
+    live.fill_synthetic(61)
+    live.set_stash([b"A" * 16])
+    staged = IA.Server(db_rows, inst, d)
+    staged.fill_synthetic(62)
+    staged.set_stash([b"B" * 16, b"C" * 16])
+    qs = [q for q, _ in _queries(db_rows, [700 + i for i in range(6)])]
+    solo_a = [live.respond_serialized(q) for q in qs]
+    solo_b = [staged.respond_serialized(q) for q in qs]
+    assert all(a != b for a, b in zip(solo_a, solo_b))
+    eng = IA.Engine(live, nq=2, max_batch=4, window_ms=0.0, capacity=32)
+    # 1: answered on the old state before the swap
+    t = [eng.submit(q) for q in qs[:2]]
+    assert [eng.wait(x)[0] for x in t] == solo_a[:2]
+    # 2: dispatched before the swap, collected after it: old state, old stash
+    t = eng.submit(qs[2])
+    for _ in range(2000):
+        if eng.stats()["queries"] >= 3:
+            break
+        _time.sleep(0.001)
+    assert eng.stats()["queries"] >= 3
+    eng.swap_state(staged)
+    assert eng.wait(t)[0] == solo_a[2]
+    # 3: after the swap, the new state and its stash
+    t = [eng.submit(q) for q in qs[3:]]
+    assert [eng.wait(x)[0] for x in t] == solo_b[3:]
+    # the staged handle now holds the old state
+    assert staged.respond_serialized(qs[0]) == solo_a[0]
+    # and back again while queries are queued: each is wholly one state or the other
+    t = [eng.submit(q) for q in qs]
+    eng.swap_state(staged)
+    for i, x in enumerate(t):
+        r = eng.wait(x)[0]
+        assert r in (solo_a[i], solo_b[i]), i
+    t = [eng.submit(q) for q in qs]
+    assert [eng.wait(x)[0] for x in t] == solo_a
+    with pytest.raises(IA.InspireError):
+        live.swap_state(IA.Server(db_rows, 4, d))
+    eng.close()
+    live.close()
+    staged.close()
