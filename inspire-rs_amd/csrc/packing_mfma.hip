@@ -37,6 +37,25 @@ namespace inspire {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef unsigned int u4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) void lds_void;
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+
+// buffer_load_dword... lds (the LDS DMA path with a buffer resource): the
+// per-lane offset and the wave-uniform soffset address the source, so a copy
+// whose step offset lives in an SGPR needs no VGPR address temporaries.
+__device__ void raw_buffer_load_lds(i32x4 rsrc, __attribute__((address_space(3))) void* lds, int size, int voffset,
+                                    int soffset, int offset, int aux) __asm("llvm.amdgcn.raw.buffer.load.lds");
+
+// Raw buffer resource over `bytes` bytes at `base` (gfx9 word 3: 32-bit data
+// format, no swizzle).  Wave-uniform.
+__device__ inline i32x4 make_raw_rsrc(const void* base, uint32_t bytes) {
+  const uint64_t a = reinterpret_cast<uint64_t>(base);
+  i32x4 r;
+  r.x = __builtin_amdgcn_readfirstlane((int)(uint32_t)a);
+  r.y = __builtin_amdgcn_readfirstlane((int)(uint32_t)(a >> 32) & 0xffff);
+  r.z = __builtin_amdgcn_readfirstlane((int)bytes);
+  r.w = 0x00020000;
+  return r;
+}
 
 constexpr int kK = kTExp * kN;    // 6144 rotation terms in slot order
 constexpr int kKSteps = kK / 64;  // 96
@@ -321,7 +340,12 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE, WPE
     const int64_t nt = min((int64_t)(nt0 + wn * NTW + y), a.ntiles - 1);
     bbase[y] = tb + (size_t)nt * kKSteps * 1024 + lane * 16;
   }
-  const u4* abase = reinterpret_cast<const u4*>(a.afrag) + ((size_t)p * (a.bpad / 4) + mg * MT) * kKSteps * 64 + lane;
+  // The client group's A fragments, [MT][96 steps][64 lanes][16 B], as a
+  // buffer resource: a copy reads the lane's 16 bytes (VGPR offset) of
+  // fragment (x, step) (wave-uniform offset, an SGPR).
+  const i32x4 arsrc = make_raw_rsrc(a.afrag + ((size_t)p * (a.bpad / 4) + mg * MT) * kKSteps * 1024,
+                                    (uint32_t)(MT * kKSteps * 1024));
+  const int alane = lane * 16;
 
   auto load_b = [&](int s, u4 (&regs)[NTW]) {
 #pragma unroll
@@ -337,8 +361,8 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE, WPE
 #pragma unroll
     for (int f = 0; f < kAPerWave; ++f) {
       const int x = min(wave * kAPerWave + f, MT - 1);
-      __builtin_amdgcn_global_load_lds((const void*)(abase + ((size_t)x * kKSteps + ss) * 64),
-                                       (lds_void*)&aring[slot][x][0], 16, 0, 0);
+      raw_buffer_load_lds(arsrc, (lds_void*)&aring[slot][x][0], 16, alane,
+                          __builtin_amdgcn_readfirstlane((x * kKSteps + ss) * 1024), 0, 0);
     }
   };
 
