@@ -461,17 +461,33 @@ void launch_post(uint64_t* P, const uint32_t* S, const uint32_t* tw, int count, 
 // (group of kPostJ outputs, client); P[b][j][z].
 constexpr int kPostJ = 4;
 constexpr int kPostThreads = 512;
+constexpr int kPostTJ = 8;  // output groups per L2 tile: 32 outputs = one 128-byte line of S
+constexpr int kPostTC = 4;  // clients per L2 tile
 
 __global__ __launch_bounds__(kPostThreads) void k_post_zmajor(uint64_t* __restrict__ P,
                                                              const uint32_t* __restrict__ S,
                                                              const uint64_t* __restrict__ z_body,
                                                              int64_t key_client_stride,
                                                              const uint64_t* __restrict__ bth,
-                                                             const uint32_t* __restrict__ tw_tab, int64_t instances) {
+                                                             const uint32_t* __restrict__ tw_tab, int64_t instances,
+                                                             int clients, int tc) {
   __shared__ uint32_t buf[2 * kPostJ][kNPad];
   const NttTables tw{tw_tab};
-  const int64_t j0 = (int64_t)blockIdx.x * kPostJ;
-  const int64_t b = blockIdx.y;
+  // Block order: each XCD (block id mod 8) takes a contiguous range of tiles
+  // of kPostTJ output groups x tc clients.  The kPostTJ groups of a tile read
+  // the same 128-byte lines of S (kPostTJ * kPostJ = 32 outputs), and its tc
+  // clients read the same bold_t_hat rows, so both meet in that XCD's L2.
+  const int64_t groups = (instances + kPostJ - 1) / kPostJ;
+  const int64_t jt_n = (groups + kPostTJ - 1) / kPostTJ, ct_n = (clients + tc - 1) / tc;
+  const int64_t total = jt_n * ct_n * kPostTJ * tc;
+  const int64_t per_xcd = (total + 7) / 8;
+  const int64_t r = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (r >= total) return;
+  const int64_t tile = r / (kPostTJ * tc), o = r % (kPostTJ * tc);
+  const int64_t g = (tile % jt_n) * kPostTJ + o % kPostTJ;
+  const int64_t b = (tile / jt_n) * tc + o / kPostTJ;
+  if (g >= groups || b >= clients) return;
+  const int64_t j0 = g * kPostJ;
   const int nj = (int)(instances - j0 < kPostJ ? instances - j0 : kPostJ);
   const uint32_t* Sb = S + (size_t)b * 2 * kN * instances;
   const uint64_t* zb = z_body + b * key_client_stride;
@@ -515,9 +531,12 @@ __global__ __launch_bounds__(kPostThreads) void k_post_zmajor(uint64_t* __restri
 
 void launch_post_zmajor(uint64_t* P, const uint32_t* S, const uint64_t* z_body, int64_t key_client_stride,
                         const uint64_t* bth, const uint32_t* tw, int clients, int64_t instances, hipStream_t s) {
-  const unsigned groups = (unsigned)((instances + kPostJ - 1) / kPostJ);
-  hipLaunchKernelGGL(k_post_zmajor, dim3(groups, (unsigned)clients), dim3(kPostThreads), 0, s, P, S, z_body,
-                     key_client_stride, bth, tw, instances);
+  const int tc = clients < kPostTC ? clients : kPostTC;
+  const int64_t groups = (instances + kPostJ - 1) / kPostJ;
+  const int64_t total = (groups + kPostTJ - 1) / kPostTJ * kPostTJ * ((clients + tc - 1) / tc) * tc;
+  const int64_t blocks = (total + 7) / 8 * 8;
+  hipLaunchKernelGGL(k_post_zmajor, dim3((unsigned)blocks), dim3(kPostThreads), 0, s, P, S, z_body,
+                     key_client_stride, bth, tw, instances, clients, tc);
 }
 
 // ---------------------------------------------------------------------------
