@@ -89,3 +89,43 @@ def test_absent_and_straddling_ids(keyword_server):
             pos, q = client.query(rid, seed=idx)
             assert client.extract(rid, pos, srv.respond_serialized(q)) is None
             break
+
+
+def test_million_key_dataset_extracts(tmp_path):
+    # The benchmark's shape with real state: a PQKEYV2 file of 1,000,000
+    # 1184-byte keys, setup_keyword_server (cuckoo table, raw DB, device
+    # encode and offline precomputation at 352 outputs), then queries for
+    # present ids through the full 11-chain response and extract.
+    import hashlib
+    rng = np.random.default_rng(11)
+    n, kl = 1_000_000, 1184
+    say("1M-key dataset: generating")
+    keys = rng.integers(0, 256, size=(n, kl), dtype=np.uint8)
+    dom = b"pq-key:v1:ml-kem-768"
+    ids = np.frombuffer(b"".join(hashlib.sha3_256(dom + keys[i].tobytes()).digest() for i in range(n)),
+                        dtype=np.uint8).reshape(n, 32)
+    path = tmp_path / "million.pqkeyv2"
+    with open(path, "wb") as f:
+        f.write(b"PQKEYV2\0" + bytes([10]) + b"ml-kem-768" + n.to_bytes(4, "little") + kl.to_bytes(2, "little"))
+        f.write(np.concatenate([ids, keys], axis=1).tobytes())
+    say("1M-key dataset: setup_keyword_server")
+    srv, pp = IA.setup_keyword_server(str(path))
+    plan = IA.plan_shape(n, 32, kl)
+    assert pp["num_items"] == 2_097_152 and pp["num_records"] == n
+    assert (plan["instances"], plan["interpolate_degree"], plan["c"]) == (352, 32, 11)
+    srv.set_packing_kernel(IA.PACKING_MFMA)
+    client = O.KeywordClient(pp, plan, O.client_secret(1234, O.HAMMING_WEIGHT))
+    done = 0
+    for idx in rng.permutation(n)[:200]:
+        rid, pk = ids[idx].tobytes(), keys[idx].tobytes()
+        if any(client.straddles(b) for b in client.positions(rid)) or (
+                np.frombuffer(rid + pk, dtype=">u2") == 0xFFFF).any():
+            continue
+        say("1M-key dataset: extract record", int(idx))
+        pos, q = client.query(rid, seed=int(idx))
+        assert client.extract(rid, pos, srv.respond_serialized(q)) == pk, int(idx)
+        done += 1
+        if done == 3:
+            break
+    assert done == 3
+    srv.close()
