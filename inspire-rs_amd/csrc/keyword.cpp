@@ -182,6 +182,8 @@ extern "C" int inspire_setup_keyword_records(const uint8_t* keys, uint32_t key_s
   for (int64_t r = 0; r < count; ++r) table.insert(keys + (size_t)r * key_size, values + (size_t)r * value_size);
   const int64_t per = plan.num_items / plan.db_rows;
   const int64_t ise = plan.item_size_bits / 16;
+  if (per * ise > plan.db_cols)  // the parameter derivation leaves room for every bucket of a row
+    return set_error(INSPIRE_ERR_INVALID, "cuckoo buckets of a row do not fit the DB columns");
   std::vector<uint16_t> raw((size_t)plan.db_rows * plan.db_cols);
   table.raw_db(raw.data(), plan.db_rows, plan.db_cols, per, ise);
   inspire_shape_t shape{plan.db_rows, plan.instances, (int32_t)plan.interpolate_degree, 2};
