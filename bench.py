@@ -397,10 +397,14 @@ def main():
     srv.profile(False)
     names = ["unpack", "packing", "post", "gemv", "fold"]
     stage = {n: stage_ms[i] / max(cnt, 1) for i, n in enumerate(names)}
-    # Algorithmic bytes of the packing kernel: bold_t and bold_t_bar once; the
-    # VALU kernel also reads bold_t_hat (the MFMA path adds those terms in the
-    # post-process).
-    pack_bytes = plan["bold_t_bytes"] + (inst * 3 * IA.N * 8 if args.packing == "valu" else 0)
+    # Algorithmic bytes of the packing kernel: bold_t and bold_t_bar once.  The
+    # MFMA kernel reads them bit-packed, 28 bits per residue (56 of the 64 bits
+    # of a condensed element); the VALU kernel reads the condensed u64 words and
+    # also bold_t_hat (the MFMA path adds those terms in the post-process).
+    if args.packing == "mfma":
+        pack_bytes = plan["bold_t_bytes"] * 56 // 64
+    else:
+        pack_bytes = plan["bold_t_bytes"] + inst * 3 * IA.N * 8
     pack_kernel = "k_packing_mfma<1, 44>" if args.packing == "mfma" else "k_packing<8, true>"
     gemv_bytes = plan["db_bytes"]
     pack_gbs = pack_bytes / (stage["packing"] / 1e3) / 1e9

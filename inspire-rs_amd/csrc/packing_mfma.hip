@@ -27,6 +27,15 @@
 //
 // Fragment order: K step s covers kappa = 64s .. 64s+63; lane (row/col r,
 // group g) holds kappa = 64s + 16g + t in byte t (v_mfma_i32_16x16x64_i8).
+//
+// Packed B fragments.  A lane's 16 limb bytes (four dwords w0..w3, every byte
+// below 128) are stored in 14 bytes: dword i < 3 is w_i with bit i of each
+// byte of w3 in its four free top bits, and a 16-bit word h holds bits 3..6 of
+// w3's bytes, one nibble per byte.  Lanes 2m and 2m+1 share 28 bytes at 28m:
+// [d0 d1 d2 of 2m | h of 2m, h of 2m+1 | d0 d1 d2 of 2m+1], so each lane
+// reads its data with one 16-byte load (at 28m and 28m + 12) and a fragment
+// is 896 bytes.  The tiled copy of the key-switch material is thus 28 bits
+// per residue, as the residues are.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -60,11 +69,46 @@ __device__ inline i32x4 make_raw_rsrc(const void* base, uint32_t bytes) {
 constexpr int kK = kTExp * kN;    // 6144 rotation terms in slot order
 constexpr int kKSteps = kK / 64;  // 96
 
+constexpr int kFragBytes = 896;  // packed B fragment
+
 __host__ __device__ inline int64_t tiled_ntiles(int64_t instances) { return (instances + 3) / 4; }
 
 // Bytes of the tiled key-switch material for `instances` outputs.
 size_t tiled_bold_t_bytes(int64_t instances) {
-  return (size_t)2 * kN * tiled_ntiles(instances) * kKSteps * 1024;
+  return (size_t)2 * kN * tiled_ntiles(instances) * kKSteps * kFragBytes;
+}
+
+__device__ u4 raw_buffer_load_b128(i32x4 rsrc, int voffset, int soffset, int aux) __asm("llvm.amdgcn.raw.buffer.load.v4i32");
+constexpr int kNonTemporal = 2;  // buffer-load cache policy: nt
+
+// Packs a lane's limb dwords (every byte < 128) into three dwords and a
+// 16-bit word (layout above).
+__device__ inline void pack_b_lane(const uint32_t (&w)[4], uint32_t (&d)[3], uint16_t& h) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) d[i] = w[i] | (((w[3] >> i) & 0x01010101u) << 7);
+  const uint32_t n = (w[3] >> 3) & 0x0f0f0f0fu;  // nibble k at bits 8k..8k+3
+  const uint32_t x = (n | (n >> 4)) & 0x00ff00ffu;  // nibbles 0,1 in byte 0; 2,3 in byte 2
+  h = (uint16_t)((x | (x >> 8)) & 0xffffu);
+}
+
+// Byte offset of a lane's 16-byte window in a packed fragment.
+__device__ inline int b_lane_offset(int lane) { return 28 * (lane >> 1) + 12 * (lane & 1); }
+
+// The lane's 16-byte window of the fragment at byte `frag` (wave-uniform, an
+// SGPR) of the buffer `rsrc`; `voff` = b_lane_offset(lane).  Non-temporal:
+// each fragment is read once.
+__device__ inline u4 load_b_packed(i32x4 rsrc, int voff, int frag) {
+  return raw_buffer_load_b128(rsrc, voff, frag, kNonTemporal);
+}
+
+// The 16 limb bytes (as an MFMA operand) from a lane's window; `odd` = lane & 1.
+__device__ inline v4i unpack_b(const u4 r, bool odd) {
+  const uint32_t d0 = odd ? r.y : r.x, d1 = odd ? r.z : r.y, d2 = odd ? r.w : r.z;
+  const uint32_t h = odd ? r.x >> 16 : r.w & 0xffffu;
+  const uint32_t low = ((d0 >> 7) & 0x01010101u) | ((d1 >> 6) & 0x02020202u) | ((d2 >> 5) & 0x04040404u);
+  uint32_t x = (h | (h << 8)) & 0x00ff00ffu;
+  x = (x | (x << 4)) & 0x0f0f0f0fu;
+  return v4i{(int)(d0 & 0x7f7f7f7fu), (int)(d1 & 0x7f7f7f7fu), (int)(d2 & 0x7f7f7f7fu), (int)((x << 3) | low)};
 }
 
 // Rotation term reaching slot sigma at NTT slot z: i | (w << 15), where i =
@@ -99,7 +143,7 @@ struct CondensedFromGenerator {
   }
 };
 
-// One thread per 16-byte fragment row: out[p][z][nt][s][lane][16].
+// One thread per lane of a fragment: out[p][z][nt][s] packed fragments.
 template <class Src>
 __global__ void k_tile_bold_t(uint8_t* __restrict__ out, Src src, const uint16_t* __restrict__ dlog,
                               int64_t instances) {
@@ -131,7 +175,15 @@ __global__ void k_tile_bold_t(uint8_t* __restrict__ out, Src src, const uint16_t
         w[t >> 2] |= ((res >> (7 * m)) & 127u) << (8 * (t & 3));
       }
     }
-    reinterpret_cast<u4*>(out)[e] = u4{w[0], w[1], w[2], w[3]};
+    uint32_t d[3];
+    uint16_t h;
+    pack_b_lane(w, d, h);
+    uint8_t* pair = out + (size_t)(e / 64) * kFragBytes + 28 * (lane >> 1);
+    uint32_t* dd = reinterpret_cast<uint32_t*>(pair + ((lane & 1) ? 16 : 0));
+    dd[0] = d[0];
+    dd[1] = d[1];
+    dd[2] = d[2];
+    reinterpret_cast<uint16_t*>(pair + 12)[lane & 1] = h;
   }
 }
 
@@ -229,8 +281,12 @@ __global__ __launch_bounds__(256) void k_packing_mfma(PackArgs a) {
   const int mg = (int)(r3 / a.ngroups);
   const int nt0 = ng * NT, mt0 = mg * MT;
 
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const uint8_t* tbase = a.tiled + (((size_t)p * kN + z) * a.ntiles) * (size_t)kKSteps * 1024;
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  // This (p, z)'s fragments as a buffer resource; a fragment's offset is an SGPR.
+  const i32x4 brsrc = make_raw_rsrc(a.tiled + (((size_t)p * kN + z) * a.ntiles) * (size_t)kKSteps * kFragBytes,
+                                    (uint32_t)(a.ntiles * kKSteps * kFragBytes));
+  const int voff = b_lane_offset(lane);
+  const bool odd = lane & 1;
   const u4* abase = reinterpret_cast<const u4*>(a.afrag) + ((size_t)p * (a.bpad / 4) + mt0) * kKSteps * 64;
   const bool a_thread = tid < MT * 64;
   const int ax = a_thread ? tid >> 6 : 0;  // A row (M-tile) this thread stages
@@ -239,8 +295,7 @@ __global__ __launch_bounds__(256) void k_packing_mfma(PackArgs a) {
 #pragma unroll
     for (int y = 0; y < NTW; ++y) {
       const int64_t nt = min((int64_t)(nt0 + wave * NTW + y), a.ntiles - 1);
-      regs[y] = __builtin_nontemporal_load(reinterpret_cast<const u4*>(tbase + ((size_t)nt * kKSteps + s) * 1024) +
-                                           lane);
+      regs[y] = load_b_packed(brsrc, voff, __builtin_amdgcn_readfirstlane((int)((nt * kKSteps + s) * kFragBytes)));
     }
   };
   auto load_a = [&](int s) { return abase[((size_t)ax * kKSteps + s) * 64 + lane]; };
@@ -263,12 +318,14 @@ __global__ __launch_bounds__(256) void k_packing_mfma(PackArgs a) {
     const int sn = min(s + 1, kKSteps - 1);
     an = load_a(sn);
     load_b(sn, bnext);
+    v4i bf[NTW];
+#pragma unroll
+    for (int y = 0; y < NTW; ++y) bf[y] = unpack_b(bcur[y], odd);
 #pragma unroll
     for (int x = 0; x < MT; ++x) {
       const v4i af = __builtin_bit_cast(v4i, afrag[cur][x][lane]);
 #pragma unroll
-      for (int y = 0; y < NTW; ++y)
-        acc[x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, __builtin_bit_cast(v4i, bcur[y]), acc[x][y], 0, 0, 0);
+      for (int y = 0; y < NTW; ++y) acc[x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af, bf[y], acc[x][y], 0, 0, 0);
     }
     if (a_thread) afrag[cur ^ 1][ax][lane] = an;
     __syncthreads();
@@ -333,13 +390,18 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE, WPE
   const int wm = wave % WM, wn = wave / WM;
   // Tiles past the matrix repeat the last real tile's loads and are not stored.
   const int nt0 = ng * NTG;
-  const uint8_t* tb = a.tiled + ((size_t)p * kN + z) * a.ntiles * kKSteps * 1024;
-  const uint8_t* bbase[NTW];
+  // This (p, z)'s B fragments as a buffer resource; the wave's tiles start at
+  // the wave-uniform byte offsets btile (SGPRs).
+  const i32x4 brsrc = make_raw_rsrc(a.tiled + ((size_t)p * kN + z) * a.ntiles * kKSteps * kFragBytes,
+                                    (uint32_t)(a.ntiles * kKSteps * kFragBytes));
+  int btile[NTW];
 #pragma unroll
   for (int y = 0; y < NTW; ++y) {
     const int64_t nt = min((int64_t)(nt0 + wn * NTW + y), a.ntiles - 1);
-    bbase[y] = tb + (size_t)nt * kKSteps * 1024 + lane * 16;
+    btile[y] = __builtin_amdgcn_readfirstlane((int)(nt * kKSteps * kFragBytes));
   }
+  const int voff = b_lane_offset(lane);
+  const bool odd = lane & 1;
   // The client group's A fragments, [MT][96 steps][64 lanes][16 B], as a
   // buffer resource: a copy reads the lane's 16 bytes (VGPR offset) of
   // fragment (x, step) (wave-uniform offset, an SGPR).
@@ -349,8 +411,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE, WPE
 
   auto load_b = [&](int s, u4 (&regs)[NTW]) {
 #pragma unroll
-    for (int y = 0; y < NTW; ++y)
-      regs[y] = __builtin_nontemporal_load(reinterpret_cast<const u4*>(bbase[y] + (size_t)s * 1024));
+    for (int y = 0; y < NTW; ++y) regs[y] = load_b_packed(brsrc, voff, btile[y] + s * kFragBytes);
   };
   // A fragments of step s (or, past the last step, a repeat into the spare
   // slot).  When W does not divide MT the last waves copy fragment MT-1 again:
@@ -389,6 +450,9 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE, WPE
       __builtin_amdgcn_s_waitcnt((kWait & 15) | (((kWait >> 4) & 3) << 14) | 0x3f70);
       __builtin_amdgcn_s_barrier();
       const u4* at = &aring[s % (D + 1)][wm * MTW][0];
+      v4i bf[NTW];
+#pragma unroll
+      for (int y = 0; y < NTW; ++y) bf[y] = unpack_b(breg[d][y], odd);
       // A fragments four tiles at a time: fragment e of the next group is
       // read as soon as this group's MFMAs on fragment e have been issued.
       v4i ga[4];
@@ -400,8 +464,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE, WPE
         for (int e = 0; e < 4; ++e) {
 #pragma unroll
           for (int y = 0; y < NTW; ++y)
-            acc[xg * 4 + e][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ga[e], __builtin_bit_cast(v4i, breg[d][y]),
-                                                                     acc[xg * 4 + e][y], 0, 0, 0);
+            acc[xg * 4 + e][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ga[e], bf[y], acc[xg * 4 + e][y], 0, 0, 0);
           if (xg + 1 < kGroups) ga[e] = __builtin_bit_cast(v4i, at[((xg + 1) * 4 + e) * 64 + lane]);
         }
       }
@@ -494,6 +557,8 @@ int launch_packing_mfma(uint32_t* S, int64_t s_client_stride, const uint8_t* til
                         const uint64_t* y_body, int64_t key_client_stride, int count, int64_t instances,
                         hipStream_t s, int variant) {
   if (count < 1) return -1;
+  // Fragment offsets within one (prime, slot) are 32-bit buffer offsets.
+  if (tiled_ntiles(instances) * kKSteps * kFragBytes > INT32_MAX) return -1;
   const int bpad = batch_key_pad(count);
   const int64_t total = 2 * (int64_t)(bpad / 4) * kKSteps * 64;
   hipLaunchKernelGGL(k_batch_afrag, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, afrag, y_body,

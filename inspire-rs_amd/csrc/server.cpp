@@ -412,6 +412,9 @@ static int server_create(const inspire_shape_t* shape, bool lean, inspire_server
   if (shape->instances <= 0 || shape->interpolate_degree <= 0 ||
       shape->instances % shape->interpolate_degree != 0)
     return set_error(INSPIRE_ERR_INVALID, "instances must be a positive multiple of interpolate_degree");
+  // 65536 outputs are 2^27 DB columns, beyond one GPU's memory at any db_rows;
+  // the packing kernels address a slot's key-switch fragments with 32-bit offsets.
+  if (shape->instances > 65536) return set_error(INSPIRE_ERR_INVALID, "instances must be at most 65536");
   if (shape->max_queries <= 0 || shape->max_queries > 255)
     return set_error(INSPIRE_ERR_INVALID, "max_queries must be in [1, 255]");
   auto* s = new inspire_server();
