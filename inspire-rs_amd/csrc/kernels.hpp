@@ -65,7 +65,7 @@ void launch_tile_bold_t(uint8_t* out, const uint64_t* bt, const uint64_t* btb, c
 void launch_tile_bold_t_synth(uint8_t* out, uint64_t seed, uint32_t stream_bt, uint32_t stream_btb,
                               const uint16_t* dlog, int64_t instances, hipStream_t s);
 int batch_key_pad(int count);
-// Device bytes of the client-key fragments for up to `count` clients.
+// Device bytes of the client-key fragments and limb sums for up to `count` clients.
 size_t packing_afrag_bytes(int count);
 // Rotation-term sums of the packing, z-major: S[client][2][N][instances];
 // y_body: [count][3][N] (key_client_stride elements apart); afrag: scratch of

@@ -19,23 +19,30 @@
 // (i, w) that reaches sigma (zero for the two slots no rotation reaches),
 // stored fragment-ordered per z ("tiled").
 //
-// Limbs.  Both operands split into four 7-bit limbs (residues < 2^28), so
-// every limb product is in [0, 127^2] and an int32 accumulator over 6144 terms
-// stays below 2^27.  Rows of a 16x16 tile are (client, limb l) = 4 clients x
-// 4 limbs, columns (output j, limb m) = 4 outputs x 4 limbs; the 16 partial
-// sums of one (client, output) pair recombine as sum_{l,m} 2^(7(l+m)) C_lm.
+// Limbs.  A client-key residue (< 2^28) splits into four 7-bit limbs a_l in
+// [0, 127]; a key-switch residue into limbs of 8, 8, 8 and 4 bits, b_m.  The
+// byte limbs (m < 3) are stored as signed bytes b_m - 128, so the MFMA sums
+// C'_lm = sum_k a_l (b_m - 128), and C_lm = C'_lm + 128 R_l with the client's
+// limb sum R_l = sum_k a_l over the 6144 K entries; the 4-bit limb is stored
+// as is.  Every |term| is at most 127 * 128 and every C_lm at most
+// 127 * 255 * 6144 < 2^31.  Rows of a 16x16 tile are (client, limb l) = 4
+// clients x 4 limbs, columns (output j, limb m) = 4 outputs x 4 limbs; the 16
+// sums of one (client, output) pair recombine as sum_{l,m} 2^(7l + 8m) C_lm.
 //
 // Fragment order: K step s covers kappa = 64s .. 64s+63; lane (row/col r,
 // group g) holds kappa = 64s + 16g + t in byte t (v_mfma_i32_16x16x64_i8).
 //
-// Packed B fragments.  A lane's 16 limb bytes (four dwords w0..w3, every byte
-// below 128) are stored in 14 bytes: dword i < 3 is w_i with bit i of each
-// byte of w3 in its four free top bits, and a 16-bit word h holds bits 3..6 of
-// w3's bytes, one nibble per byte.  Lanes 2m and 2m+1 share 28 bytes at 28m:
-// [d0 d1 d2 of 2m | h of 2m, h of 2m+1 | d0 d1 d2 of 2m+1], so each lane
-// reads its data with one 16-byte load (at 28m and 28m + 12) and a fragment
-// is 896 bytes.  The tiled copy of the key-switch material is thus 28 bits
-// per residue, as the residues are.
+// Packed B fragments.  A lane (K group g = lane / 16, column c = lane % 16,
+// output o = c / 4, limb m = c % 4) holds 16 K entries of one limb.  A byte
+// limb lane keeps its 16 bytes; a 4-bit limb lane keeps 16 nibbles in 8
+// bytes, entries t and t + 8 in byte t of the pair of dwords (n0, n1), so the
+// bytes come back as n0 & M, n1 & M, (n0 >> 4) & M, (n1 >> 4) & M with
+// M = 0x0f0f0f0f.  K group g takes 224 bytes at 224 g, output o 56 of them
+// at 56 o: its three byte-limb lanes (16 bytes each), then its nibble lane
+// (8 bytes).  Every lane reads a 16-byte window; a nibble lane's window starts
+// 8 bytes before its data (the first half is the previous lane's and is
+// ignored), so no window leaves its fragment.  A fragment is 896 bytes and a
+// residue costs its 28 bits.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -81,18 +88,12 @@ size_t tiled_bold_t_bytes(int64_t instances) {
 __device__ u4 raw_buffer_load_b128(i32x4 rsrc, int voffset, int soffset, int aux) __asm("llvm.amdgcn.raw.buffer.load.v4i32");
 constexpr int kNonTemporal = 2;  // buffer-load cache policy: nt
 
-// Packs a lane's limb dwords (every byte < 128) into three dwords and a
-// 16-bit word (layout above).
-__device__ inline void pack_b_lane(const uint32_t (&w)[4], uint32_t (&d)[3], uint16_t& h) {
-#pragma unroll
-  for (int i = 0; i < 3; ++i) d[i] = w[i] | (((w[3] >> i) & 0x01010101u) << 7);
-  const uint32_t n = (w[3] >> 3) & 0x0f0f0f0fu;  // nibble k at bits 8k..8k+3
-  const uint32_t x = (n | (n >> 4)) & 0x00ff00ffu;  // nibbles 0,1 in byte 0; 2,3 in byte 2
-  h = (uint16_t)((x | (x >> 8)) & 0xffffu);
-}
 
-// Byte offset of a lane's 16-byte window in a packed fragment.
-__device__ inline int b_lane_offset(int lane) { return 28 * (lane >> 1) + 12 * (lane & 1); }
+// Byte offset of a lane's data in a packed fragment, and of its 16-byte window.
+__host__ __device__ inline int b_lane_data(int lane) {
+  return 224 * (lane >> 4) + 56 * ((lane >> 2) & 3) + 16 * (lane & 3);
+}
+__host__ __device__ inline int b_lane_offset(int lane) { return b_lane_data(lane) - ((lane & 3) == 3 ? 8 : 0); }
 
 // The lane's 16-byte window of the fragment at byte `frag` (wave-uniform, an
 // SGPR) of the buffer `rsrc`; `voff` = b_lane_offset(lane).  Non-temporal:
@@ -101,14 +102,20 @@ __device__ inline u4 load_b_packed(i32x4 rsrc, int voff, int frag) {
   return raw_buffer_load_b128(rsrc, voff, frag, kNonTemporal);
 }
 
-// The 16 limb bytes (as an MFMA operand) from a lane's window; `odd` = lane & 1.
-__device__ inline v4i unpack_b(const u4 r, bool odd) {
-  const uint32_t d0 = odd ? r.y : r.x, d1 = odd ? r.z : r.y, d2 = odd ? r.w : r.z;
-  const uint32_t h = odd ? r.x >> 16 : r.w & 0xffffu;
-  const uint32_t low = ((d0 >> 7) & 0x01010101u) | ((d1 >> 6) & 0x02020202u) | ((d2 >> 5) & 0x04040404u);
-  uint32_t x = (h | (h << 8)) & 0x00ff00ffu;
-  x = (x | (x << 4)) & 0x0f0f0f0fu;
-  return v4i{(int)(d0 & 0x7f7f7f7fu), (int)(d1 & 0x7f7f7f7fu), (int)(d2 & 0x7f7f7f7fu), (int)((x << 3) | low)};
+// Per-lane constants of unpack_b: a nibble lane (limb 3) masks and shifts.
+struct BLane {
+  uint32_t mask;   // 0x0f0f0f0f for a nibble lane, else all ones
+  uint32_t shift;  // 4 for a nibble lane, else 0
+  bool nib;
+  __device__ explicit BLane(int lane)
+      : mask((lane & 3) == 3 ? 0x0f0f0f0fu : 0xffffffffu), shift((lane & 3) == 3 ? 4u : 0u), nib((lane & 3) == 3) {}
+};
+
+// The 16 limb bytes (as an MFMA operand) from a lane's window (a nibble
+// lane's data is its z, w).
+__device__ inline v4i unpack_b(const u4 r, const BLane& bl) {
+  return v4i{(int)((bl.nib ? r.z : r.x) & bl.mask), (int)((bl.nib ? r.w : r.y) & bl.mask),
+             (int)((r.z >> bl.shift) & bl.mask), (int)((r.w >> bl.shift) & bl.mask)};
 }
 
 // Rotation term reaching slot sigma at NTT slot z: i | (w << 15), where i =
@@ -162,28 +169,34 @@ __global__ void k_tile_bold_t(uint8_t* __restrict__ out, Src src, const uint16_t
     const int64_t j = 4 * nt + (col >> 2);
     const int m = col & 3;
     const uint32_t ez_inv = odd_inverse_mod_2n(2u * (__brev((uint32_t)z) >> 21) + 1u);
-    uint32_t w[4] = {0, 0, 0, 0};
-    if (j < instances) {
-      for (int t = 0; t < 16; ++t) {
-        const int kappa = 64 * s + 16 * g + t;
-        const int k = kappa >> 11, sigma = kappa & (kN - 1);
-        const uint32_t term = slot_term(ez_inv, sigma, dlog);
-        const int i = (int)(term & 0x3ffu);
-        if (i >= kNumRot) continue;
+    // limb m of each of the 16 residues (zero where no rotation reaches the slot)
+    uint32_t limb[16];
+    for (int t = 0; t < 16; ++t) {
+      uint32_t res = 0;
+      const int kappa = 64 * s + 16 * g + t;
+      const int k = kappa >> 11, sigma = kappa & (kN - 1);
+      const uint32_t term = slot_term(ez_inv, sigma, dlog);
+      const int i = (int)(term & 0x3ffu);
+      if (j < instances && i < kNumRot) {
         const uint64_t v = src(term >> 15, ((size_t)j * kNInner + 3 * i + k) * kN + z);
-        const uint32_t res = (uint32_t)(p ? (v >> 32) : v);
-        w[t >> 2] |= ((res >> (7 * m)) & 127u) << (8 * (t & 3));
+        res = (uint32_t)(p ? (v >> 32) : v);
       }
+      limb[t] = (res >> (8 * m)) & (m < 3 ? 255u : 15u);
     }
-    uint32_t d[3];
-    uint16_t h;
-    pack_b_lane(w, d, h);
-    uint8_t* pair = out + (size_t)(e / 64) * kFragBytes + 28 * (lane >> 1);
-    uint32_t* dd = reinterpret_cast<uint32_t*>(pair + ((lane & 1) ? 16 : 0));
-    dd[0] = d[0];
-    dd[1] = d[1];
-    dd[2] = d[2];
-    reinterpret_cast<uint16_t*>(pair + 12)[lane & 1] = h;
+    uint8_t* dst = out + (size_t)(e / 64) * kFragBytes + b_lane_data(lane);
+    if (m < 3) {
+      uint32_t w[4] = {0, 0, 0, 0};
+      for (int t = 0; t < 16; ++t) w[t >> 2] |= (limb[t] ^ 0x80u) << (8 * (t & 3));  // b - 128 as a signed byte
+      *reinterpret_cast<u4*>(dst) = u4{w[0], w[1], w[2], w[3]};
+    } else {
+      uint32_t n0 = 0, n1 = 0;
+      for (int t = 0; t < 4; ++t) {
+        n0 |= (limb[t] | limb[t + 8] << 4) << (8 * t);
+        n1 |= (limb[t + 4] | limb[t + 12] << 4) << (8 * t);
+      }
+      reinterpret_cast<uint32_t*>(dst)[0] = n0;
+      reinterpret_cast<uint32_t*>(dst)[1] = n1;
+    }
   }
 }
 
@@ -217,10 +230,41 @@ __global__ void k_batch_afrag(uint8_t* __restrict__ afrag, const uint64_t* __res
   }
 }
 
+// R[p][b][l] = sum over the 6144 K entries of limb l of client b's keys
+// (zero for padding clients): the correction of the signed byte limbs.
+// One block per (p, b).
+__global__ __launch_bounds__(256) void k_batch_rowsum(uint32_t* __restrict__ rsum, const uint64_t* __restrict__ y_body,
+                                                      int64_t client_stride, int count, int bpad) {
+  const int p = blockIdx.x / bpad, b = blockIdx.x % bpad;
+  uint32_t part[4] = {0, 0, 0, 0};
+  if (b < count) {
+    const uint64_t* yb = y_body + (int64_t)b * client_stride;
+    for (int kappa = threadIdx.x; kappa < kK; kappa += 256) {
+      const uint64_t v = yb[kappa];
+      const uint32_t res = (uint32_t)(p ? (v >> 32) : v);
+#pragma unroll
+      for (int l = 0; l < 4; ++l) part[l] += (res >> (7 * l)) & 127u;
+    }
+  }
+  __shared__ uint32_t red[4][4];
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    uint32_t x = part[l];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][l] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4)
+    rsum[((size_t)p * bpad + b) * 4 + threadIdx.x] =
+        red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+}
+
 struct PackArgs {
   uint32_t* S;               // [count][2][N][instances] (z-major)
   const uint8_t* tiled;      // [2][N][ntiles][96][64][16]
   const uint8_t* afrag;      // [2][bpad/4][96][64][16]
+  const uint32_t* rsum;      // [2][bpad][4] client limb sums
   int64_t instances;
   int64_t ntiles;
   int bpad;
@@ -231,8 +275,9 @@ struct PackArgs {
 };
 
 // Epilogue of one 16x16 tile.  Lane (col c = lane%16, G = lane/16) holds
-// C[4G + l][c], l = 0..3: client 4*mt + G, output 4*nt + c/4, output limb
-// m = c%4.  Recombines the limbs and stores the rotation-term sum in the
+// C'[4G + l][c], l = 0..3: client 4*mt + G, output 4*nt + c/4, output limb
+// m = c%4.  Corrects the signed byte limbs, recombines the limbs and stores
+// the rotation-term sum in the
 // z-major layout S[b][p][z][j], so a block's stores fill whole lines; the
 // z_body terms are added by the post-process (k_post_zmajor).
 __device__ inline void store_tile(const PackArgs& a, const v4i& acc, int p, int z, int mt, int64_t nt) {
@@ -240,17 +285,20 @@ __device__ inline void store_tile(const PackArgs& a, const v4i& acc, int p, int 
   const int c = lane & 15, G = lane >> 4;
   const int m = c & 3;
   const uint32_t q = prime(p);
-  const uint32_t pw = (uint32_t)(((uint64_t)1 << (7 * m)) % q);  // 2^(7m) mod q
+  const uint32_t pw = (uint32_t)(((uint64_t)1 << (8 * m)) % q);  // 2^(8m) mod q
+  const int b = 4 * mt + G;
+  const u4 r = *reinterpret_cast<const u4*>(a.rsum + ((size_t)p * a.bpad + b) * 4);
+  const uint32_t corr = m < 3 ? 128u : 0u;
+  const uint32_t rl[4] = {r.x, r.y, r.z, r.w};
   uint64_t X = 0;
 #pragma unroll
-  for (int l = 0; l < 4; ++l) X += (uint64_t)(uint32_t)acc[l] << (7 * l);
+  for (int l = 0; l < 4; ++l) X += (uint64_t)((uint32_t)acc[l] + corr * rl[l]) << (7 * l);
   uint32_t v = (uint32_t)barrett64(X, cr1(p), q);
   v = p ? mulmod<1>(v, pw) : mulmod<0>(v, pw);
   uint32_t tot = v + __shfl_xor(v, 1, 64);
   tot = (tot >= q) ? tot - q : tot;
   uint32_t tot2 = tot + __shfl_xor(tot, 2, 64);
   tot2 = (tot2 >= q) ? tot2 - q : tot2;
-  const int b = 4 * mt + G;
   const int64_t j = 4 * nt + (c >> 2);
   if (m == 0 && b < a.count && j < a.instances)
     a.S[(size_t)b * a.s_client_stride + ((size_t)p * kN + z) * a.instances + j] = tot2;
@@ -286,7 +334,7 @@ __global__ __launch_bounds__(256) void k_packing_mfma(PackArgs a) {
   const i32x4 brsrc = make_raw_rsrc(a.tiled + (((size_t)p * kN + z) * a.ntiles) * (size_t)kKSteps * kFragBytes,
                                     (uint32_t)(a.ntiles * kKSteps * kFragBytes));
   const int voff = b_lane_offset(lane);
-  const bool odd = lane & 1;
+  const BLane bl(lane);
   const u4* abase = reinterpret_cast<const u4*>(a.afrag) + ((size_t)p * (a.bpad / 4) + mt0) * kKSteps * 64;
   const bool a_thread = tid < MT * 64;
   const int ax = a_thread ? tid >> 6 : 0;  // A row (M-tile) this thread stages
@@ -320,7 +368,7 @@ __global__ __launch_bounds__(256) void k_packing_mfma(PackArgs a) {
     load_b(sn, bnext);
     v4i bf[NTW];
 #pragma unroll
-    for (int y = 0; y < NTW; ++y) bf[y] = unpack_b(bcur[y], odd);
+    for (int y = 0; y < NTW; ++y) bf[y] = unpack_b(bcur[y], bl);
 #pragma unroll
     for (int x = 0; x < MT; ++x) {
       const v4i af = __builtin_bit_cast(v4i, afrag[cur][x][lane]);
@@ -401,7 +449,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE, WPE
     btile[y] = __builtin_amdgcn_readfirstlane((int)(nt * kKSteps * kFragBytes));
   }
   const int voff = b_lane_offset(lane);
-  const bool odd = lane & 1;
+  const BLane bl(lane);
   // The client group's A fragments, [MT][96 steps][64 lanes][16 B], as a
   // buffer resource: a copy reads the lane's 16 bytes (VGPR offset) of
   // fragment (x, step) (wave-uniform offset, an SGPR).
@@ -452,7 +500,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE, WPE
       const u4* at = &aring[s % (D + 1)][wm * MTW][0];
       v4i bf[NTW];
 #pragma unroll
-      for (int y = 0; y < NTW; ++y) bf[y] = unpack_b(breg[d][y], odd);
+      for (int y = 0; y < NTW; ++y) bf[y] = unpack_b(breg[d][y], bl);
       // A fragments four tiles at a time: fragment e of the next group is
       // read as soon as this group's MFMAs on fragment e have been issued.
       v4i ga[4];
@@ -523,7 +571,12 @@ int batch_key_pad(int count) {
   return (count + unit - 1) / unit * unit;
 }
 
-size_t packing_afrag_bytes(int count) { return (size_t)2 * (batch_key_pad(count) / 4) * kKSteps * 1024; }
+// Client-key fragments, then the limb sums R ([2][bpad][4] u32).
+static size_t afrag_frag_bytes(int bpad) { return (size_t)2 * (bpad / 4) * kKSteps * 1024; }
+size_t packing_afrag_bytes(int count) {
+  const int bpad = batch_key_pad(count);
+  return afrag_frag_bytes(bpad) + (size_t)2 * bpad * 4 * sizeof(uint32_t);
+}
 
 template <int MT, int NT>
 static void launch_small(PackArgs a, hipStream_t s) {
@@ -563,7 +616,10 @@ int launch_packing_mfma(uint32_t* S, int64_t s_client_stride, const uint8_t* til
   const int64_t total = 2 * (int64_t)(bpad / 4) * kKSteps * 64;
   hipLaunchKernelGGL(k_batch_afrag, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, afrag, y_body,
                      key_client_stride, count, bpad);
-  PackArgs a{S, tiled, afrag, instances, tiled_ntiles(instances), bpad, count, 0, 0, s_client_stride};
+  uint32_t* rsum = reinterpret_cast<uint32_t*>(afrag + afrag_frag_bytes(bpad));
+  hipLaunchKernelGGL(k_batch_rowsum, dim3((unsigned)(2 * bpad)), dim3(256), 0, s, rsum, y_body, key_client_stride,
+                     count, bpad);
+  PackArgs a{S, tiled, afrag, rsum, instances, tiled_ntiles(instances), bpad, count, 0, 0, s_client_stride};
   if (bpad <= 4)
     launch_small<1, 44>(a, s);    // 1 M-tile x 44 N-tiles (11 per wave)
   else if (bpad <= 16)
