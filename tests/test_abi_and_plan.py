@@ -83,3 +83,16 @@ def test_plan_mixed_db_shapes(kem, pk):
     else:
         p = IA.plan_shape(250_000, 32, pk)
         assert (p["num_items"], p["instances"], p["c"]) == (524_288, 128, 4)
+
+
+def test_server_rejects_bad_shapes_before_touching_a_device():
+    # The shape checks come before any HIP call, so they run without a GPU.
+    with pytest.raises(IA.InspireError, match="multiple of 2048"):
+        IA.Server(3000, 352, 32)
+    with pytest.raises(IA.InspireError, match="multiple of interpolate_degree"):
+        IA.Server(2048, 350, 32)
+    # 65536 outputs bound the packing's 32-bit fragment offsets
+    with pytest.raises(IA.InspireError, match="at most 65536"):
+        IA.Server(2048, 65536 + 32, 32)
+    with pytest.raises(IA.InspireError, match="max_queries"):
+        IA.Server(2048, 352, 32, max_queries=0)
