@@ -257,7 +257,7 @@ int inspire_respond_batch_device(inspire_server_t* srv, const uint8_t* d_queries
                                  int nq, int count, uint8_t* d_out, size_t out_stride, void* stream);
 
 /* Batched kernel variants.  packing_variant: tile configuration of the
- * 64-client-group packing kernel (0..3); gemv_variant: 0 = query and DB
+ * 64-client-group packing kernel (0..4); gemv_variant: 0 = query and DB
  * fragments shared through LDS, 1 = query fragments from L2 (0..1, when the
  * server holds the fragment-ordered DB); fold_variant: 0 = by launch size,
  * 1 = one 1024-thread workgroup per chain (both primes at once), 2 = 512

@@ -401,13 +401,13 @@ __global__ __launch_bounds__(256) void k_packing_mfma(PackArgs a) {
 // is sized for.
 // ---------------------------------------------------------------------------
 
-template <int MT, int NTW, int W, int D, int WPE, int WM>
+template <int MT, int NTW, int W, int D, int WPE, int WM, int GA = 4>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE, WPE))) void k_packing_wide(PackArgs a) {
   constexpr int MTW = MT / WM;                 // M-tiles per wave
   constexpr int NTG = W / WM * NTW;            // N-tiles per block
   constexpr int kAPerWave = (MT + W - 1) / W;  // A fragments each wave copies per step
-  constexpr int kGroups = MTW / 4;
-  static_assert(W % WM == 0 && MTW % 4 == 0 && kKSteps % D == 0, "A reads in fours; D divides the steps");
+  constexpr int kGroups = MTW / GA;            // A reads GA fragments ahead
+  static_assert(W % WM == 0 && MTW % GA == 0 && kKSteps % D == 0, "A reads in groups of GA; D divides the steps");
   // Vector-memory operations a wave issues after its A copies of step s:
   // that step's B loads and D-1 later steps.
   constexpr int kWait = (kAPerWave + NTW) * (D - 1) + NTW;
@@ -501,26 +501,26 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE, WPE
       v4i bf[NTW];
 #pragma unroll
       for (int y = 0; y < NTW; ++y) bf[y] = unpack_b(breg[d][y], bl);
-      // A fragments four tiles at a time: fragment e of the next group is
-      // read as soon as this group's MFMAs on fragment e have been issued.
-      v4i ga[4];
+      // A fragments GA tiles at a time: fragment e of the next group is read
+      // as soon as this group's MFMAs on fragment e have been issued.
+      v4i ga[GA];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) ga[e] = __builtin_bit_cast(v4i, at[e * 64 + lane]);
+      for (int e = 0; e < GA; ++e) ga[e] = __builtin_bit_cast(v4i, at[e * 64 + lane]);
 #pragma unroll
       for (int xg = 0; xg < kGroups; ++xg) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
+        for (int e = 0; e < GA; ++e) {
 #pragma unroll
           for (int y = 0; y < NTW; ++y)
-            acc[xg * 4 + e][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ga[e], bf[y], acc[xg * 4 + e][y], 0, 0, 0);
-          if (xg + 1 < kGroups) ga[e] = __builtin_bit_cast(v4i, at[((xg + 1) * 4 + e) * 64 + lane]);
+            acc[xg * GA + e][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ga[e], bf[y], acc[xg * GA + e][y], 0, 0, 0);
+          if (xg + 1 < kGroups) ga[e] = __builtin_bit_cast(v4i, at[((xg + 1) * GA + e) * 64 + lane]);
         }
       }
-      __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, GA, 0);
 #pragma unroll
       for (int xg = 0; xg < kGroups; ++xg) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
+        for (int e = 0; e < GA; ++e) {
           __builtin_amdgcn_sched_group_barrier(0x008, NTW, 0);
           if (xg + 1 < kGroups) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
         }
@@ -586,13 +586,13 @@ static void launch_small(PackArgs a, hipStream_t s) {
   hipLaunchKernelGGL((k_packing_mfma<MT, NT>), dim3((unsigned)blocks), dim3(256), 0, s, a);
 }
 
-template <int MT, int NTW, int W, int D, int WPE, int WM>
+template <int MT, int NTW, int W, int D, int WPE, int WM, int GA = 4>
 static void launch_wide(PackArgs a, hipStream_t s) {
   constexpr int NTG = W / WM * NTW;
   a.ngroups = (int)((a.ntiles + NTG - 1) / NTG);
   a.mgroups = a.bpad / (4 * MT);
   const int64_t blocks = (int64_t)kN * 2 * a.ngroups * a.mgroups;
-  hipLaunchKernelGGL((k_packing_wide<MT, NTW, W, D, WPE, WM>), dim3((unsigned)blocks), dim3(64 * W), 0, s, a);
+  hipLaunchKernelGGL((k_packing_wide<MT, NTW, W, D, WPE, WM, GA>), dim3((unsigned)blocks), dim3(64 * W), 0, s, a);
 }
 
 // Tile configurations for 64-client groups (MT, NTW, W, D, WPE, WM); the
@@ -600,8 +600,9 @@ static void launch_wide(PackArgs a, hipStream_t s) {
 const char* const kWideVariantNames[] = {"16x2 tiles, 11 waves, ring 2, 3 waves/SIMD",
                                          "16x2 tiles, 4 waves, ring 3, 2 blocks/CU",
                                          "16x1 tiles, 8 waves, ring 3, 2 blocks/CU",
-                                         "16x2 tiles, 8 waves, ring 6, 1 block/CU"};
-int packing_wide_variants() { return 4; }
+                                         "16x2 tiles, 8 waves, ring 6, 1 block/CU",
+                                         "16x2 tiles, 11 waves, ring 3, A read 2 ahead, 3 waves/SIMD"};
+int packing_wide_variants() { return 5; }
 const char* packing_wide_variant_name(int v) {
   return v >= 0 && v < packing_wide_variants() ? kWideVariantNames[v] : "";
 }
@@ -633,6 +634,7 @@ int launch_packing_mfma(uint32_t* S, int64_t s_client_stride, const uint8_t* til
       case 1: launch_wide<16, 2, 4, 3, 2, 1>(a, s); break;
       case 2: launch_wide<16, 1, 8, 3, 4, 1>(a, s); break;
       case 3: launch_wide<16, 2, 8, 6, 2, 1>(a, s); break;
+      case 4: launch_wide<16, 2, 11, 3, 3, 1, 2>(a, s); break;
       default: launch_wide<16, 2, 11, 2, 3, 1>(a, s); break;
     }
   }
