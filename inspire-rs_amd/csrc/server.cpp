@@ -276,6 +276,12 @@ struct inspire_server {
   size_t max_query_bytes = 0;
   std::vector<Slot> slots;
   size_t next_slot = 0;
+  // Recorded after each single-query packing; the next request's packing waits
+  // for it, so requests in flight stream the key-switch material one after the
+  // other at full bandwidth and each one's GEMV and fold overlap the next
+  // one's packing (two packings at once would each run at half speed and
+  // reach their folds together, leaving HBM idle).
+  hipEvent_t pack_done = nullptr;
   int gemv_kind = INSPIRE_GEMV_MFMA;
   int packing_kind = INSPIRE_PACKING_VALU;
   uint8_t* tiled = nullptr;      // fragment-ordered int8 limbs of bold_t (MFMA packing)
@@ -436,6 +442,8 @@ static int server_create(const inspire_shape_t* shape, bool lean, inspire_server
   s->lean = lean;
   if (lean) s->packing_kind = INSPIRE_PACKING_MFMA;
   int rc = INSPIRE_OK;
+  if (hipEventCreateWithFlags(&s->pack_done, hipEventDisableTiming) != hipSuccess)
+    rc = set_error(INSPIRE_ERR_HIP, "hipEventCreate failed");
   if (!rc) rc = dev_alloc(&s->db, (size_t)rows * s->db_cols, "database");
   if (!rc && lean) rc = dev_alloc(&s->tiled, tiled_bold_t_bytes(inst), "tiled bold_t");
   if (!rc && !lean) rc = dev_alloc(&s->bt, (size_t)inst * kNInner * kN, "bold_t");
@@ -552,6 +560,7 @@ extern "C" void inspire_server_destroy(inspire_server_t* s) {
   if (s->stream) (void)hipStreamDestroy(s->stream);
   for (auto& e : s->ev)
     if (e) (void)hipEventDestroy(e);
+  if (s->pack_done) (void)hipEventDestroy(s->pack_done);
   delete s;
 }
 
@@ -632,7 +641,7 @@ extern "C" int inspire_server_autotune(inspire_server_t* s, int count, int nq, i
     return set_error(INSPIRE_ERR_HIP, "hipEventCreate failed");
   }
   auto pack = [&](int n, int v) {
-    launch_packing_mfma(b.S, inst * 2 * kN, s->tiled, b.keys, b.y_body, (int64_t)kTExp * kN, n, inst, st, v);
+    (void)launch_packing_mfma(b.S, inst * 2 * kN, s->tiled, b.keys, b.y_body, (int64_t)kTExp * kN, n, inst, st, v);
   };
   auto gemv = [&](int n, int v) {
     (void)launch_gemv_mfma_batch(b.cvec, (int64_t)nq * s->db_cols, b.rows, (int64_t)nq * rows, nq, s->db, s->dbfrag,
@@ -824,8 +833,9 @@ static int launch_batch(inspire_server_t* s, const uint8_t* d_queries, size_t q_
   launch_unpack_query_batch(d_queries, (int64_t)q_stride, nq, (int)rows, count, b.y_body, b.z_body, b.rows, b.bodies,
                             st);
   mark(1);
-  launch_packing_mfma(b.S, inst * 2 * kN, s->tiled, b.keys, b.y_body, (int64_t)kTExp * kN, count, inst, st,
-                      s->pack_variant);
+  if (launch_packing_mfma(b.S, inst * 2 * kN, s->tiled, b.keys, b.y_body, (int64_t)kTExp * kN, count, inst, st,
+                          s->pack_variant))
+    return set_error(INSPIRE_ERR_INVALID, "packing: shape beyond the kernels' 32-bit fragment offsets");
   mark(2);
   launch_post_zmajor(b.P, b.S, b.z_body, (int64_t)kTExp * kN, s->bth, s->consts->tw, count, inst, st);
   mark(3);
@@ -1065,13 +1075,17 @@ int run_pipeline(inspire_server_t* s, Slot& sl, const uint8_t* d_query, int nq, 
   mark(0);
   launch_unpack_query(d_query, nq, (int)rows, sl.y_body, sl.z_body, sl.rows, sl.bodies, st);
   mark(1);
+  HIP_TRY(hipStreamWaitEvent(st, s->pack_done, 0), "wait for the previous packing");
   if (s->packing_kind == INSPIRE_PACKING_MFMA) {
     if (!s->tiled_valid) return set_error(INSPIRE_ERR_STATE, "tiled key-switch material not built");
-    launch_packing_mfma(sl.S, inst * 2 * kN, s->tiled, sl.keys, sl.y_body, (int64_t)kTExp * kN, 1, inst, st);
+    if (launch_packing_mfma(sl.S, inst * 2 * kN, s->tiled, sl.keys, sl.y_body, (int64_t)kTExp * kN, 1, inst, st))
+      return set_error(INSPIRE_ERR_INVALID, "packing: shape beyond the kernels' 32-bit fragment offsets");
+    HIP_TRY(hipEventRecord(s->pack_done, st), "packing done");
     mark(2);
     launch_post_zmajor(sl.P, sl.S, sl.z_body, (int64_t)kTExp * kN, s->bth, s->consts->tw, 1, inst, st);
   } else {
     launch_packing(sl.S, s->bt, s->btb, s->bth, sl.y_body, sl.z_body, (int)inst, st);
+    HIP_TRY(hipEventRecord(s->pack_done, st), "packing done");
     mark(2);
     launch_post(sl.P, sl.S, s->consts->tw, (int)inst, st);
   }
