@@ -276,11 +276,12 @@ struct inspire_server {
   size_t max_query_bytes = 0;
   std::vector<Slot> slots;
   size_t next_slot = 0;
-  // Recorded after each single-query packing; the next request's packing waits
-  // for it, so requests in flight stream the key-switch material one after the
-  // other at full bandwidth and each one's GEMV and fold overlap the next
-  // one's packing (two packings at once would each run at half speed and
-  // reach their folds together, leaving HBM idle).
+  // Recorded after each single-query request's packing and GEMV; the next
+  // request's packing waits for it.  Requests in flight then stream the
+  // key-switch material and the DB one after the other at full bandwidth, and
+  // each request's fold (VALU) overlaps the next one's packing.  Two packings
+  // at once would each run at half speed and reach their folds together,
+  // leaving HBM idle.
   hipEvent_t pack_done = nullptr;
   int gemv_kind = INSPIRE_GEMV_MFMA;
   int packing_kind = INSPIRE_PACKING_VALU;
@@ -1075,17 +1076,15 @@ int run_pipeline(inspire_server_t* s, Slot& sl, const uint8_t* d_query, int nq, 
   mark(0);
   launch_unpack_query(d_query, nq, (int)rows, sl.y_body, sl.z_body, sl.rows, sl.bodies, st);
   mark(1);
-  HIP_TRY(hipStreamWaitEvent(st, s->pack_done, 0), "wait for the previous packing");
+  HIP_TRY(hipStreamWaitEvent(st, s->pack_done, 0), "wait for the previous request's packing and GEMV");
   if (s->packing_kind == INSPIRE_PACKING_MFMA) {
     if (!s->tiled_valid) return set_error(INSPIRE_ERR_STATE, "tiled key-switch material not built");
     if (launch_packing_mfma(sl.S, inst * 2 * kN, s->tiled, sl.keys, sl.y_body, (int64_t)kTExp * kN, 1, inst, st))
       return set_error(INSPIRE_ERR_INVALID, "packing: shape beyond the kernels' 32-bit fragment offsets");
-    HIP_TRY(hipEventRecord(s->pack_done, st), "packing done");
     mark(2);
     launch_post_zmajor(sl.P, sl.S, sl.z_body, (int64_t)kTExp * kN, s->bth, s->consts->tw, 1, inst, st);
   } else {
     launch_packing(sl.S, s->bt, s->btb, s->bth, sl.y_body, sl.z_body, (int)inst, st);
-    HIP_TRY(hipEventRecord(s->pack_done, st), "packing done");
     mark(2);
     launch_post(sl.P, sl.S, s->consts->tw, (int)inst, st);
   }
@@ -1101,6 +1100,7 @@ int run_pipeline(inspire_server_t* s, Slot& sl, const uint8_t* d_query, int nq, 
   } else if (launch_gemv(sl.cvec, sl.rows, nq, s->db, (int)rows, s->db_cols, st)) {
     return set_error(INSPIRE_ERR_INVALID, "unsupported GEMV shape");
   }
+  HIP_TRY(hipEventRecord(s->pack_done, st), "packing and GEMV done");
   mark(4);
   if (launch_fold(d_out, s->a_hat, sl.cvec, sl.P, sl.bodies, s->consts->neg_a, s->consts->tw, inst,
                   s->shape.interpolate_degree, nq, st))
