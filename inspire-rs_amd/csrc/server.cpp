@@ -276,12 +276,12 @@ struct inspire_server {
   size_t max_query_bytes = 0;
   std::vector<Slot> slots;
   size_t next_slot = 0;
-  // Recorded after each single-query request's packing and GEMV; the next
-  // request's packing waits for it.  Requests in flight then stream the
-  // key-switch material and the DB one after the other at full bandwidth, and
-  // each request's fold (VALU) overlaps the next one's packing.  Two packings
-  // at once would each run at half speed and reach their folds together,
-  // leaving HBM idle.
+  // Recorded after each single-query request's GEMV and packing; the next
+  // request's GEMV waits for it.  Requests in flight then stream the DB and
+  // the key-switch material one after the other at full bandwidth, and each
+  // request's post-process and fold overlap the next one's GEMV and packing.
+  // Two packings at once would each run at half speed and reach their folds
+  // together, leaving HBM idle.
   hipEvent_t pack_done = nullptr;
   int gemv_kind = INSPIRE_GEMV_MFMA;
   int packing_kind = INSPIRE_PACKING_VALU;
@@ -1076,19 +1076,9 @@ int run_pipeline(inspire_server_t* s, Slot& sl, const uint8_t* d_query, int nq, 
   mark(0);
   launch_unpack_query(d_query, nq, (int)rows, sl.y_body, sl.z_body, sl.rows, sl.bodies, st);
   mark(1);
-  HIP_TRY(hipStreamWaitEvent(st, s->pack_done, 0), "wait for the previous request's packing and GEMV");
-  if (s->packing_kind == INSPIRE_PACKING_MFMA) {
-    if (!s->tiled_valid) return set_error(INSPIRE_ERR_STATE, "tiled key-switch material not built");
-    if (launch_packing_mfma(sl.S, inst * 2 * kN, s->tiled, sl.keys, sl.y_body, (int64_t)kTExp * kN, 1, inst, st))
-      return set_error(INSPIRE_ERR_INVALID, "packing: shape beyond the kernels' 32-bit fragment offsets");
-    mark(2);
-    launch_post_zmajor(sl.P, sl.S, sl.z_body, (int64_t)kTExp * kN, s->bth, s->consts->tw, 1, inst, st);
-  } else {
-    launch_packing(sl.S, s->bt, s->btb, s->bth, sl.y_body, sl.z_body, (int)inst, st);
-    mark(2);
-    launch_post(sl.P, sl.S, s->consts->tw, (int)inst, st);
-  }
-  mark(3);
+  // GEMV, then packing: the HBM-streaming stages run back to back (after the
+  // previous request's), and the post-process waits off that critical chain.
+  HIP_TRY(hipStreamWaitEvent(st, s->pack_done, 0), "wait for the previous request's GEMV and packing");
   if (s->gemv_kind == INSPIRE_GEMV_MFMA) {
     // two hash queries per MFMA tile; larger nq runs in pairs
     for (int h = 0; h < nq; h += 2) {
@@ -1100,7 +1090,20 @@ int run_pipeline(inspire_server_t* s, Slot& sl, const uint8_t* d_query, int nq, 
   } else if (launch_gemv(sl.cvec, sl.rows, nq, s->db, (int)rows, s->db_cols, st)) {
     return set_error(INSPIRE_ERR_INVALID, "unsupported GEMV shape");
   }
-  HIP_TRY(hipEventRecord(s->pack_done, st), "packing and GEMV done");
+  mark(2);
+  if (s->packing_kind == INSPIRE_PACKING_MFMA) {
+    if (!s->tiled_valid) return set_error(INSPIRE_ERR_STATE, "tiled key-switch material not built");
+    if (launch_packing_mfma(sl.S, inst * 2 * kN, s->tiled, sl.keys, sl.y_body, (int64_t)kTExp * kN, 1, inst, st))
+      return set_error(INSPIRE_ERR_INVALID, "packing: shape beyond the kernels' 32-bit fragment offsets");
+    HIP_TRY(hipEventRecord(s->pack_done, st), "GEMV and packing done");
+    mark(3);
+    launch_post_zmajor(sl.P, sl.S, sl.z_body, (int64_t)kTExp * kN, s->bth, s->consts->tw, 1, inst, st);
+  } else {
+    launch_packing(sl.S, s->bt, s->btb, s->bth, sl.y_body, sl.z_body, (int)inst, st);
+    HIP_TRY(hipEventRecord(s->pack_done, st), "GEMV and packing done");
+    mark(3);
+    launch_post(sl.P, sl.S, s->consts->tw, (int)inst, st);
+  }
   mark(4);
   if (launch_fold(d_out, s->a_hat, sl.cvec, sl.P, sl.bodies, s->consts->neg_a, s->consts->tw, inst,
                   s->shape.interpolate_degree, nq, st))
@@ -1109,11 +1112,14 @@ int run_pipeline(inspire_server_t* s, Slot& sl, const uint8_t* d_query, int nq, 
   HIP_TRY(hipGetLastError(), "respond launch");
   HIP_TRY(hipEventRecord(sl.done, st), "slot done");
   if (s->profile) {
+    // event pairs in launch order (unpack, GEMV, packing, post, fold) and the
+    // stage each one times (stage order: unpack, packing, post, GEMV, fold)
+    static constexpr int kStage[5] = {0, 3, 1, 2, 4};
     HIP_TRY(hipEventSynchronize(s->ev[5]), "profile sync");
     for (int i = 0; i < 5; ++i) {
       float ms = 0.f;
       HIP_TRY(hipEventElapsedTime(&ms, s->ev[i], s->ev[i + 1]), "profile elapsed");
-      s->stage_ms[i] += ms;
+      s->stage_ms[kStage[i]] += ms;
     }
     ++s->profiled;
   }
