@@ -171,15 +171,14 @@ __global__ __launch_bounds__(256) void k_gemv_mfma(uint64_t* __restrict__ c, con
     }
   };
 
+  // The DB ring fills while the first chunk's query fragments are staged.
+#pragma unroll
+  for (int r = 0; r < D; ++r) load(r, ring[r]);
   for (int k0 = 0; k0 < steps; k0 += kMfmaStepsPerChunk) {
     __syncthreads();
     const v4i* src = reinterpret_cast<const v4i*>(afrag) + (int64_t)k0 * 64;
     for (int e = threadIdx.x; e < kMfmaStepsPerChunk * 64; e += 256) a_lds[e / 64][e % 64] = src[e];
     __syncthreads();
-    if (k0 == 0) {
-#pragma unroll
-      for (int r = 0; r < D; ++r) load(r, ring[r]);
-    }
     for (int s = 0; s < kMfmaStepsPerChunk; s += D) {
 #pragma unroll
       for (int r = 0; r < D; ++r) {
