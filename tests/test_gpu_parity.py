@@ -311,6 +311,46 @@ def _queries(db_rows, seeds, nq=2, small=False):
     return out
 
 
+def _split_response(resp):
+    """The hash queries' blobs of a serialized keyword response (wire.rs:708-726)."""
+    n = int.from_bytes(resp[:4], "little")
+    out, o = [], 4
+    for _ in range(n):
+        ln = int.from_bytes(resp[o:o + 4], "little")
+        out.append(resp[o + 4:o + 4 + ln])
+        o += 4 + ln
+    return out
+
+
+@pytest.mark.parametrize("kind", [IA.PACKING_VALU, IA.PACKING_MFMA])
+def test_single_query_requests_in_flight_equal_solo(torch, kind):
+    # Nine single-query responds issued back to back on three streams over
+    # three request slots, with no host synchronization between them (the
+    # server orders the requests' GEMV and packing stages with an event and
+    # reuses each slot after its previous request): every response equals the
+    # same query answered alone.
+    say("requests in flight, packing kind", kind)
+    db_rows, inst, d, nq = 2048, 8, 4, 2
+    srv = IA.Server(db_rows, inst, d)
+    srv.fill_synthetic(61)
+    srv.set_packing_kernel(kind)
+    srv.set_slots(3)
+    qs = _queries(db_rows, [700 + i for i in range(9)])
+    solo = [b"".join(_split_response(srv.respond_serialized(q))) for q, _ in qs]
+    blob = nq * (inst // d) * O.RESP_BYTES
+    assert all(len(x) == blob for x in solo)
+    streams = [torch.cuda.Stream() for _ in range(3)]
+    dqs = [dev_from(torch, np.frombuffer(q + bytes(16), dtype=np.uint8)) for q, _ in qs]
+    outs = [empty(torch, blob, np.uint8) for _ in qs]
+    torch.cuda.synchronize()
+    for i, (q, _) in enumerate(qs):
+        srv.respond_device(dqs[i], len(q), nq, outs[i], streams[i % 3].cuda_stream)
+    torch.cuda.synchronize()
+    for i in range(len(qs)):
+        assert outs[i].cpu().numpy().tobytes() == solo[i], i
+    srv.close()
+
+
 @pytest.mark.parametrize("db_rows,inst,d", [(2048, 8, 8), (4096, 12, 4), (2048, 2, 1)])
 def test_packing_mfma_respond_matches_valu(torch, db_rows, inst, d):
     say("packing mfma vs valu respond", db_rows, inst, d)
