@@ -262,7 +262,7 @@ __global__ __launch_bounds__(256) void k_batch_rowsum(uint32_t* __restrict__ rsu
 
 struct PackArgs {
   uint32_t* S;               // [count][2][N][instances] (z-major)
-  const uint8_t* tiled;      // [2][N][ntiles][96][64][16]
+  const uint8_t* tiled;      // [2][N][ntiles][96] packed fragments of 896 B
   const uint8_t* afrag;      // [2][bpad/4][96][64][16]
   const uint32_t* rsum;      // [2][bpad][4] client limb sums
   int64_t instances;
@@ -391,14 +391,16 @@ __global__ __launch_bounds__(256) void k_packing_mfma(PackArgs a) {
 // Large batches: block = W waves over MT M-tiles (a client group) x
 // (W / WM) * NTW N-tiles; wave w owns M-tiles [(w % WM) * MT/WM, ...) and
 // N-tiles (w / WM) * NTW ..., so every B fragment leaves HBM once per group
-// (the WM waves that share it meet in L2) and feeds MT MFMAs.  B fragments go through a D-step register
-// ring; the A fragments (MT KiB per step, the same for every slot) are copied
-// into an LDS ring of D+1 steps, shared by all W waves, by the LDS DMA path
-// (global_load_lds_dwordx4), so they take no registers.  A step waits with an
+// (the WM waves that share it meet in L2) and feeds MT MFMAs.  Packed B
+// fragments go through a D-step register ring and are unpacked after the
+// step's barrier; the A fragments (MT KiB per step, the same for every slot)
+// are copied into an LDS ring of D+1 steps, shared by all W waves, by the LDS
+// DMA path (buffer_load_dwordx4 ... lds), so they take no registers.  Both
+// are buffer loads whose per-step offset is an SGPR.  A step waits with an
 // explicit vector-memory count for its own A copies (issued D steps earlier)
-// and a barrier for everyone else's; each A fragment is read from LDS four
-// fragments' MFMAs ahead of its use.  WPE = waves per SIMD the register budget
-// is sized for.
+// and a barrier for everyone else's; each A fragment is read from LDS GA
+// fragments' MFMAs ahead of its use.  WPE = waves per SIMD the register
+// budget is sized for.
 // ---------------------------------------------------------------------------
 
 template <int MT, int NTW, int W, int D, int WPE, int WM, int GA = 4>
