@@ -172,7 +172,7 @@ def bench_mixed(IA, torch, rounds=12, seed=37):
     return res
 
 
-def bench_32m(IA, torch, rates=(100, 300, 600), serve_seconds=2.0):
+def bench_32m(IA, torch, rates=(100, 300, 600, 700), serve_seconds=2.0):
     """32M ML-KEM-768 keys at dim0 65536 (DB 94.5 GB, bold_t 35.4 GB) on a lean
     server, synthetic state: single-query respond, batched respond and
     continuous-batching service under Poisson arrivals."""
@@ -310,7 +310,7 @@ def main():
     ap.add_argument("--queries", type=int, default=8, help="distinct resident queries cycled through")
     ap.add_argument("--slots", type=int, default=2, help="requests in flight (request slots / streams)")
     ap.add_argument("--batches", default="1,8,32,64,128", help="batch sizes for the batched respond ('' to skip)")
-    ap.add_argument("--rates", default="100,500,1000,2000,3000",
+    ap.add_argument("--rates", default="100,500,1000,2000,3000,4000",
                     help="Poisson arrival rates (q/s) for the serving run ('' to skip)")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrivals per rate span about this long")
     ap.add_argument("--no-encode", action="store_true", help="skip the DB encode and offline setup measurements")
