@@ -163,6 +163,21 @@ void build_rgsw_neg_a(uint32_t* out, const uint32_t* tables) {
   }
 }
 
+void build_fold_tables(uint32_t* out, const uint32_t* tables, const uint32_t* neg_a) {
+  for (int c = 0; c < 2; ++c) {
+    const uint64_t m = prime(c);
+    const uint64_t ninv = invm(kN, m);
+    const uint32_t* iw = tables + (c * 4 + 2) * kN;  // inverse powers, halved
+    uint32_t* t = out + c * 3 * kN;
+    for (int i = 0; i < kN; ++i) {
+      const uint64_t w = (2 * (uint64_t)iw[i]) % m;  // the inverse power itself
+      t[0 * kN + i] = (uint32_t)w;
+      t[1 * kN + i] = (uint32_t)((w << 32) / m);
+      t[2 * kN + i] = (uint32_t)mulm(neg_a[c * kN + i], ninv, m);
+    }
+  }
+}
+
 }  // namespace inspire
 
 namespace inspire {

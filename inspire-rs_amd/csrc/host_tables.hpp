@@ -19,6 +19,12 @@ void chacha20_u64(const uint8_t seed[32], uint64_t* out, size_t n);
 // out: [2][kN] = NTT(-a mod q_c) for the constant RGSW row-0 polynomial a.
 void build_rgsw_neg_a(uint32_t* out, const uint32_t* tables);
 
+// out: [2 primes][3][kN] u32 for the RGSW fold: the inverse NTT powers (not
+// halved), their w' = floor(w*2^32/q), and n^-1 * NTT(-a) (neg_a as built by
+// build_rgsw_neg_a).  With them the fold's inverse NTTs skip the per-stage
+// halving: n^-1 is applied once, to the product rows before the transform.
+void build_fold_tables(uint32_t* out, const uint32_t* tables, const uint32_t* neg_a);
+
 // out: [N] discrete logs of the odd exponents mod 2N, (-1)^w 5^i -> i | w << 15.
 void build_slot_dlog(uint16_t* out);
 

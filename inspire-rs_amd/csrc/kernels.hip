@@ -558,7 +558,7 @@ struct FoldArgs {
   const uint64_t* cvec;      // [clients][nq][instances*N]
   const uint64_t* P;         // [clients][instances][N]
   const uint32_t* bodies;    // [clients][nq][6][2][N]
-  const uint32_t* neg_a;     // [2][N]
+  const uint32_t* fold;      // [2][3][N]: inverse powers, their w', n^-1 NTT(-a)
   const uint32_t* tw;        // NTT tables
   int64_t instances;
   int64_t out_stride;        // bytes between clients' outputs
@@ -671,14 +671,17 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
           gsum += gk;
           acc += (uint64_t)body[(k * 2 + c) * kN + z] * gk;
         }
+        // both rows carry n^-1 (row 0 in the table, row 1 here): the inverse
+        // transform below skips the per-stage halving
         const uint32_t gs = gsum % q;
-        const uint32_t na = args.neg_a[c * kN + z];
+        const uint32_t na = args.fold[(c * 3 + 2) * kN + z];
+        const uint32_t r1 = (uint32_t)barrett64(acc, cr1(c), q);
         prod[(c * 2 + 0) * kNPad + lds_index(z)] = c == 0 ? mulmod<0>(na, gs) : mulmod<1>(na, gs);
-        prod[(c * 2 + 1) * kNPad + lds_index(z)] = (uint32_t)barrett64(acc, cr1(c), q);
+        prod[(c * 2 + 1) * kNPad + lds_index(z)] = c == 0 ? mulmod<0>(r1, kNInvQ0) : mulmod<1>(r1, kNInvQ1);
       }
     }
     __syncthreads();
-    ntt_inverse_r4<kFoldThreads, 4>(ipolys, iprime, tw);
+    ntt_inverse_r4_unscaled<kFoldThreads, 4>(ipolys, iprime, args.fold);
     const size_t j = (size_t)w * d + i;
 #pragma unroll
     for (int s = 0; s < kFoldSlots; ++s) {
@@ -769,13 +772,15 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
           gsum += gk;
           acc += (uint64_t)body[(k * 2 + c) * kN + z] * gk;
         }
+        // both rows carry n^-1 (see k_fold)
         const uint32_t gs = gsum % q;
-        const uint32_t na = args.neg_a[c * kN + z];
+        const uint32_t na = args.fold[(c * 3 + 2) * kN + z];
+        const uint32_t r1 = (uint32_t)barrett64(acc, cr1(c), q);
         dig[0][lz] = c == 0 ? mulmod<0>(na, gs) : mulmod<1>(na, gs);
-        dig[1][lz] = (uint32_t)barrett64(acc, cr1(c), q);
+        dig[1][lz] = c == 0 ? mulmod<0>(r1, kNInvQ0) : mulmod<1>(r1, kNInvQ1);
       }
       __syncthreads();
-      ntt_inverse_r4_1p<kFold2Threads, 2>(ipolys, c, tw);
+      ntt_inverse_r4_1p_unscaled<kFold2Threads, 2>(ipolys, c, args.fold);
       if (c == 0) {
 #pragma unroll
         for (int s = 0; s < kFold2Slots; ++s) {
@@ -811,16 +816,16 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
 size_t fold_lds_bytes() { return (size_t)(12 + 4) * kNPad * sizeof(uint32_t); }
 
 int launch_fold(uint8_t* out, const uint64_t* a_hat, const uint64_t* cvec, const uint64_t* P,
-                const uint32_t* bodies, const uint32_t* neg_a, const uint32_t* tw, int64_t instances, int d,
+                const uint32_t* bodies, const uint32_t* fold_tab, const uint32_t* tw, int64_t instances, int d,
                 int nq, hipStream_t s) {
-  return launch_fold_batch(out, 0, a_hat, cvec, P, bodies, neg_a, tw, instances, d, nq, 1, s, 0);
+  return launch_fold_batch(out, 0, a_hat, cvec, P, bodies, fold_tab, tw, instances, d, nq, 1, s, 0);
 }
 
 int launch_fold_batch(uint8_t* out, int64_t out_stride, const uint64_t* a_hat, const uint64_t* cvec,
-                      const uint64_t* P, const uint32_t* bodies, const uint32_t* neg_a, const uint32_t* tw,
+                      const uint64_t* P, const uint32_t* bodies, const uint32_t* fold_tab, const uint32_t* tw,
                       int64_t instances, int d, int nq, int clients, hipStream_t s, int variant) {
   if (d <= 0 || instances % d != 0 || nq <= 0 || clients <= 0) return -1;
-  FoldArgs a{out, a_hat, cvec, P, bodies, neg_a, tw, instances, out_stride, d, (int)(instances / d), nq};
+  FoldArgs a{out, a_hat, cvec, P, bodies, fold_tab, tw, instances, out_stride, d, (int)(instances / d), nq};
   static bool attr_set = false;
   if (!attr_set) {
     // Raises the dynamic-LDS ceiling for this kernel (132 KiB of the 160 KiB per CU).

@@ -22,8 +22,9 @@ __global__ void k_zero_u32(uint32_t* p, int n);
 void launch_unpack_query_batch(const uint8_t* q, int64_t q_stride, int nq, int db_rows, int count, uint64_t* y_body,
                                uint64_t* z_body, uint64_t* rows, uint32_t* bodies, hipStream_t s);
 // Fold for `clients` keyword queries; client b's responses at out + b*out_stride.
+// fold_tab: DeviceConstants::fold.
 int launch_fold_batch(uint8_t* out, int64_t out_stride, const uint64_t* a_hat, const uint64_t* cvec,
-                      const uint64_t* P, const uint32_t* bodies, const uint32_t* neg_a, const uint32_t* tw,
+                      const uint64_t* P, const uint32_t* bodies, const uint32_t* fold_tab, const uint32_t* tw,
                       int64_t instances, int d, int nq, int clients, hipStream_t s, int variant = 0);
 
 // DB encode (interpolation of each row's column groups, p = 65535): in and out
@@ -111,7 +112,7 @@ int launch_setup_offline(const uint16_t* db, int64_t db_rows, int64_t instances,
 size_t fold_lds_bytes();
 // out: [nq][instances/d][12288] response blobs.
 int launch_fold(uint8_t* out, const uint64_t* a_hat, const uint64_t* cvec, const uint64_t* P,
-                const uint32_t* bodies, const uint32_t* neg_a, const uint32_t* tw, int64_t instances, int d,
+                const uint32_t* bodies, const uint32_t* fold_tab, const uint32_t* tw, int64_t instances, int d,
                 int nq, hipStream_t s);
 
 }  // namespace inspire

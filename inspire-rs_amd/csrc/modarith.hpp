@@ -33,6 +33,9 @@ constexpr uint64_t kCr1Q1 = 73916747789ull;
 // floor(2^58 / q_i) for the 32-bit Barrett below.
 constexpr uint32_t kMu58Q0 = (uint32_t)((1ull << 58) / kQ0);
 constexpr uint32_t kMu58Q1 = (uint32_t)((1ull << 58) / kQ1);
+// n^{-1} mod q_i for n = 2048.
+constexpr uint32_t kNInvQ0 = 268238881u;
+constexpr uint32_t kNInvQ1 = 249439233u;
 // Garner constant: q0^{-1} mod q1.
 constexpr uint32_t kQ0InvModQ1 = 97389680u;
 

@@ -304,4 +304,96 @@ __device__ inline void ntt_inverse_r4_1p(uint32_t* const (&polys)[NP], int c, co
   }
 }
 
+// Inverse transforms without the per-stage halving, for callers that apply
+// n^-1 to the input themselves (the RGSW fold folds it into its product
+// rows).  Gentleman-Sande with the inverse powers w and w' = floor(w*2^32/q)
+// (DeviceConstants::fold, [prime][3][kN]: w, w', -); values stay in [0, 2q)
+// through the passes and end canonical.  Same layouts and sync contract as the
+// forms above; the result is n times theirs.
+__device__ inline void inv_bf_unscaled(uint32_t& x, uint32_t& y, uint32_t w, uint32_t wp, uint32_t q) {
+  const uint32_t q2 = 2 * q;
+  const uint32_t tt = x + q2 - y;  // (0, 4q)
+  const uint32_t cx = x + y;       // [0, 4q)
+  x = cx >= q2 ? cx - q2 : cx;
+  y = w * tt - mulhi32(tt, wp) * q;
+}
+
+template <int NT, int NP, bool OnePrime>
+__device__ inline void ntt_inverse_unscaled_impl(uint32_t* const (&polys)[NP], const int* prime_of, int c1,
+                                                 const uint32_t* __restrict__ fold_tab) {
+  const int tid = threadIdx.x;
+  auto prime_at = [&](int p) { return OnePrime ? c1 : prime_of[p]; };
+  for (int u = tid; u < kN / 2; u += NT) {
+    const int l0 = lds_index(2 * u), l1 = lds_index(2 * u + 1);
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const int c = prime_at(p);
+      const uint32_t* iw = fold_tab + c * 3 * kN;
+      uint32_t* a = polys[p];
+      uint32_t x = a[l0], y = a[l1];
+      inv_bf_unscaled(x, y, iw[1024 + u], iw[kN + 1024 + u], prime(c));
+      a[l0] = x;
+      a[l1] = y;
+    }
+  }
+  __syncthreads();
+  for (int mm = kLogN - 2; mm >= 1; mm -= 2) {
+    const int h = 1 << mm, t = (kN / 2) >> mm, ni = (kN / 4) / t;
+    for (int u = tid; u < kN / 4; u += NT) {
+      int i, j;
+      if (t >= 32) {
+        i = u / t;
+        j = u % t;
+      } else {
+        i = u % ni;
+        j = u / ni;
+      }
+      const int x0 = 4 * t * i + j;
+      const int l0 = lds_index(x0), l1 = lds_index(x0 + t), l2 = lds_index(x0 + 2 * t), l3 = lds_index(x0 + 3 * t);
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        const int c = prime_at(p);
+        const uint32_t q = prime(c), q2 = 2 * q;
+        const uint32_t* iw = fold_tab + c * 3 * kN;
+        const uint32_t* iwp = iw + kN;
+        uint32_t* a = polys[p];
+        uint32_t a0 = a[l0], a1 = a[l1], a2 = a[l2], a3 = a[l3];
+        inv_bf_unscaled(a0, a1, iw[h + 2 * i], iwp[h + 2 * i], q);
+        inv_bf_unscaled(a2, a3, iw[h + 2 * i + 1], iwp[h + 2 * i + 1], q);
+        inv_bf_unscaled(a0, a2, iw[(h >> 1) + i], iwp[(h >> 1) + i], q);
+        inv_bf_unscaled(a1, a3, iw[(h >> 1) + i], iwp[(h >> 1) + i], q);
+        if (mm == 1) {  // last pass: canonical
+          a0 = a0 >= q2 ? a0 - q2 : a0;
+          a1 = a1 >= q2 ? a1 - q2 : a1;
+          a2 = a2 >= q2 ? a2 - q2 : a2;
+          a3 = a3 >= q2 ? a3 - q2 : a3;
+          a0 = a0 >= q ? a0 - q : a0;
+          a1 = a1 >= q ? a1 - q : a1;
+          a2 = a2 >= q ? a2 - q : a2;
+          a3 = a3 >= q ? a3 - q : a3;
+        }
+        a[l0] = a0;
+        a[l1] = a1;
+        a[l2] = a2;
+        a[l3] = a3;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// NP polynomials over prime c.
+template <int NT, int NP>
+__device__ inline void ntt_inverse_r4_1p_unscaled(uint32_t* const (&polys)[NP], int c,
+                                                  const uint32_t* __restrict__ fold_tab) {
+  ntt_inverse_unscaled_impl<NT, NP, true>(polys, nullptr, c, fold_tab);
+}
+
+// NP polynomials with per-polynomial primes.
+template <int NT, int NP>
+__device__ inline void ntt_inverse_r4_unscaled(uint32_t* const (&polys)[NP], const int (&prime_of)[NP],
+                                               const uint32_t* __restrict__ fold_tab) {
+  ntt_inverse_unscaled_impl<NT, NP, false>(polys, prime_of, 0, fold_tab);
+}
+
 }  // namespace inspire

@@ -13,11 +13,14 @@ int set_error(int code, const std::string& msg);
 int hip_error(hipError_t e, const char* what);
 
 // Device-resident constants built once per device: NTT twiddle tables
-// ([2][4][2048] u32), NTT(-a) for the RGSW row 0 ([2][2048] u32) and the
-// exponent discrete logs of the slot-ordered packing GEMM ([2048] u16).
+// ([2][4][2048] u32), NTT(-a) for the RGSW row 0 ([2][2048] u32), the fold's
+// tables ([2][3][2048] u32: unhalved inverse powers, their Shoup constants,
+// n^-1 NTT(-a)) and the exponent discrete logs of the slot-ordered packing
+// GEMM ([2048] u16).
 struct DeviceConstants {
   uint32_t* tw = nullptr;
   uint32_t* neg_a = nullptr;
+  uint32_t* fold = nullptr;
   uint16_t* slot_dlog = nullptr;
 };
 // Constants for the current device, or nullptr on a HIP failure.

@@ -58,31 +58,35 @@ const DeviceConstants* device_constants() {
   std::lock_guard<std::mutex> lk(g_const_mu);
   DeviceConstants& dc = g_consts[dev];
   if (dc.tw) return &dc;
-  std::vector<uint32_t> tw(2 * 4 * kN), na(2 * kN);
+  std::vector<uint32_t> tw(2 * 4 * kN), na(2 * kN), fo(2 * 3 * kN);
   std::vector<uint16_t> ts(kN);
   build_ntt_tables(tw.data());
   build_rgsw_neg_a(na.data(), tw.data());
+  build_fold_tables(fo.data(), tw.data(), na.data());
   build_slot_dlog(ts.data());
-  uint32_t *dtw = nullptr, *dna = nullptr;
+  uint32_t *dtw = nullptr, *dna = nullptr, *dfo = nullptr;
   uint16_t* dts = nullptr;
   auto release = [&]() {
     if (dtw) (void)hipFree(dtw);
     if (dna) (void)hipFree(dna);
+    if (dfo) (void)hipFree(dfo);
     if (dts) (void)hipFree(dts);
   };
   if (hipMalloc(&dtw, tw.size() * 4) != hipSuccess || hipMalloc(&dna, na.size() * 4) != hipSuccess ||
-      hipMalloc(&dts, ts.size() * 2) != hipSuccess) {
+      hipMalloc(&dfo, fo.size() * 4) != hipSuccess || hipMalloc(&dts, ts.size() * 2) != hipSuccess) {
     release();
     return nullptr;
   }
   if (hipMemcpy(dtw, tw.data(), tw.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(dna, na.data(), na.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dfo, fo.data(), fo.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(dts, ts.data(), ts.size() * 2, hipMemcpyHostToDevice) != hipSuccess) {
     release();
     return nullptr;
   }
   dc.slot_dlog = dts;
   dc.neg_a = dna;
+  dc.fold = dfo;
   dc.tw = dtw;
   return &dc;
 }
@@ -649,7 +653,7 @@ extern "C" int inspire_server_autotune(inspire_server_t* s, int count, int nq, i
                                  (int)rows, s->db_cols, n, b.gemv_scratch, st, v);
   };
   auto fold = [&](int n, int v) {
-    (void)launch_fold_batch(d_out, (int64_t)blob, s->a_hat, b.cvec, b.P, b.bodies, s->consts->neg_a,
+    (void)launch_fold_batch(d_out, (int64_t)blob, s->a_hat, b.cvec, b.P, b.bodies, s->consts->fold,
                             s->consts->tw, inst, s->shape.interpolate_degree, nq, n, st, v);
   };
   // Average milliseconds of `reps` runs after one warm-up; -1 on a HIP error.
@@ -844,7 +848,7 @@ static int launch_batch(inspire_server_t* s, const uint8_t* d_queries, size_t q_
                              (int)rows, s->db_cols, count, b.gemv_scratch, st, s->gemv_variant))
     return set_error(INSPIRE_ERR_INVALID, "unsupported GEMV shape");
   mark(4);
-  if (launch_fold_batch(d_out, (int64_t)out_stride, s->a_hat, b.cvec, b.P, b.bodies, s->consts->neg_a, s->consts->tw,
+  if (launch_fold_batch(d_out, (int64_t)out_stride, s->a_hat, b.cvec, b.P, b.bodies, s->consts->fold, s->consts->tw,
                         inst, s->shape.interpolate_degree, nq, count, st, s->fold_variant))
     return set_error(INSPIRE_ERR_INVALID, "unsupported fold shape");
   mark(5);
@@ -1105,7 +1109,7 @@ int run_pipeline(inspire_server_t* s, Slot& sl, const uint8_t* d_query, int nq, 
     launch_post(sl.P, sl.S, s->consts->tw, (int)inst, st);
   }
   mark(4);
-  if (launch_fold(d_out, s->a_hat, sl.cvec, sl.P, sl.bodies, s->consts->neg_a, s->consts->tw, inst,
+  if (launch_fold(d_out, s->a_hat, sl.cvec, sl.P, sl.bodies, s->consts->fold, s->consts->tw, inst,
                   s->shape.interpolate_degree, nq, st))
     return set_error(INSPIRE_ERR_INVALID, "unsupported fold shape");
   mark(5);
@@ -1310,7 +1314,7 @@ extern "C" int inspire_fold_device(uint8_t* d_out, const uint64_t* d_a_hat, cons
     return set_error(INSPIRE_ERR_INVALID, "bad fold argument");
   const DeviceConstants* dc = device_constants();
   if (!dc) return set_error(INSPIRE_ERR_HIP, "device constants");
-  if (launch_fold(d_out, d_a_hat, d_c, d_P, d_bodies, dc->neg_a, dc->tw, instances, d, nq, (hipStream_t)stream))
+  if (launch_fold(d_out, d_a_hat, d_c, d_P, d_bodies, dc->fold, dc->tw, instances, d, nq, (hipStream_t)stream))
     return set_error(INSPIRE_ERR_INVALID, "instances must be a positive multiple of d");
   HIP_TRY(hipGetLastError(), "fold launch");
   return INSPIRE_OK;
