@@ -469,10 +469,9 @@ __global__ __launch_bounds__(kPostThreads) void k_post_zmajor(uint64_t* __restri
                                                              const uint64_t* __restrict__ z_body,
                                                              int64_t key_client_stride,
                                                              const uint64_t* __restrict__ bth,
-                                                             const uint32_t* __restrict__ tw_tab, int64_t instances,
+                                                             const uint32_t* __restrict__ fold_tab, int64_t instances,
                                                              int clients, int tc) {
   __shared__ uint32_t buf[2 * kPostJ][kNPad];
-  const NttTables tw{tw_tab};
   // Block order: each XCD (block id mod 8) takes a contiguous range of tiles
   // of kPostTJ output groups x tc clients.  The kPostTJ groups of a tile read
   // the same 128-byte lines of S (kPostTJ * kPostJ = 32 outputs), and its tc
@@ -508,6 +507,9 @@ __global__ __launch_bounds__(kPostThreads) void k_post_zmajor(uint64_t* __restri
       v1 = (uint32_t)barrett64(h1, cr1(1), prime(1)) + Sb[((size_t)kN + z) * instances + j];
       v0 = v0 >= prime(0) ? v0 - prime(0) : v0;
       v1 = v1 >= prime(1) ? v1 - prime(1) : v1;
+      // n^-1 here; the inverse transform below skips the per-stage halving
+      v0 = mulmod<0>(v0, kNInvQ0);
+      v1 = mulmod<1>(v1, kNInvQ1);
     }
     buf[jj][lds_index(z)] = v0;
     buf[kPostJ + jj][lds_index(z)] = v1;
@@ -520,7 +522,7 @@ __global__ __launch_bounds__(kPostThreads) void k_post_zmajor(uint64_t* __restri
     polys[i] = buf[i];
     pr[i] = i / kPostJ;
   }
-  ntt_inverse_r4<kPostThreads, 2 * kPostJ>(polys, pr, tw);
+  ntt_inverse_r4_unscaled<kPostThreads, 2 * kPostJ>(polys, pr, fold_tab);
   for (int t = threadIdx.x; t < kN * kPostJ; t += kPostThreads) {
     const int jj = t / kN, z = t % kN;
     if (jj < nj)
@@ -530,13 +532,13 @@ __global__ __launch_bounds__(kPostThreads) void k_post_zmajor(uint64_t* __restri
 }
 
 void launch_post_zmajor(uint64_t* P, const uint32_t* S, const uint64_t* z_body, int64_t key_client_stride,
-                        const uint64_t* bth, const uint32_t* tw, int clients, int64_t instances, hipStream_t s) {
+                        const uint64_t* bth, const uint32_t* fold_tab, int clients, int64_t instances, hipStream_t s) {
   const int tc = clients < kPostTC ? clients : kPostTC;
   const int64_t groups = (instances + kPostJ - 1) / kPostJ;
   const int64_t total = (groups + kPostTJ - 1) / kPostTJ * kPostTJ * ((clients + tc - 1) / tc) * tc;
   const int64_t blocks = (total + 7) / 8 * 8;
   hipLaunchKernelGGL(k_post_zmajor, dim3((unsigned)blocks), dim3(kPostThreads), 0, s, P, S, z_body,
-                     key_client_stride, bth, tw, instances, clients, tc);
+                     key_client_stride, bth, fold_tab, instances, clients, tc);
 }
 
 // ---------------------------------------------------------------------------

@@ -98,7 +98,7 @@ void launch_post(uint64_t* P, const uint32_t* S, const uint32_t* tw, int count, 
 // [clients][2][N][instances] plus the z_body terms sum_k z_body[b][k] *
 // bold_t_hat[j][k] (z_body: [clients][3][N], key_client_stride apart).
 void launch_post_zmajor(uint64_t* P, const uint32_t* S, const uint64_t* z_body, int64_t key_client_stride,
-                        const uint64_t* bth, const uint32_t* tw, int clients, int64_t instances, hipStream_t s);
+                        const uint64_t* bth, const uint32_t* fold_tab, int clients, int64_t instances, hipStream_t s);
 
 
 // Offline precomputation (setup.hip): bold_t, bold_t_bar, bold_t_hat (condensed)
