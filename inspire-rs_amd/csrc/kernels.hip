@@ -675,7 +675,10 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
         }
         // both rows carry n^-1 (row 0 in the table, row 1 here): the inverse
         // transform below skips the per-stage halving
-        const uint32_t gs = gsum % q;
+        // six canonical NTT values: gsum < 6q
+        uint32_t gs = gsum >= 4 * q ? gsum - 4 * q : gsum;
+        gs = gs >= 2 * q ? gs - 2 * q : gs;
+        gs = gs >= q ? gs - q : gs;
         const uint32_t na = args.fold[(c * 3 + 2) * kN + z];
         const uint32_t r1 = (uint32_t)barrett64(acc, cr1(c), q);
         prod[(c * 2 + 0) * kNPad + lds_index(z)] = c == 0 ? mulmod<0>(na, gs) : mulmod<1>(na, gs);
@@ -775,7 +778,10 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
           acc += (uint64_t)body[(k * 2 + c) * kN + z] * gk;
         }
         // both rows carry n^-1 (see k_fold)
-        const uint32_t gs = gsum % q;
+        // six canonical NTT values: gsum < 6q
+        uint32_t gs = gsum >= 4 * q ? gsum - 4 * q : gsum;
+        gs = gs >= 2 * q ? gs - 2 * q : gs;
+        gs = gs >= q ? gs - q : gs;
         const uint32_t na = args.fold[(c * 3 + 2) * kN + z];
         const uint32_t r1 = (uint32_t)barrett64(acc, cr1(c), q);
         dig[0][lz] = c == 0 ? mulmod<0>(na, gs) : mulmod<1>(na, gs);
