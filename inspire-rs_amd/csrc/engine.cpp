@@ -137,8 +137,10 @@ void inspire_engine::dispatch_loop() {
         const auto deadline =
             reqs[pending.front()].t_submit + std::chrono::duration_cast<Clock::duration>(
                                                  std::chrono::duration<double, std::milli>(window_ms));
-        cv_work.wait_until(lk, deadline, [&] { return stop || (int)pending.size() >= max_batch; });
+        cv_work.wait_until(lk, deadline, [&] { return stop || paused || (int)pending.size() >= max_batch; });
         if (stop) return;
+        // a state swap started during the window holds this batch back too
+        if (paused) continue;
       }
       set = free_set();
       while (!pending.empty() && (int)batch.size() < max_batch) {
@@ -374,7 +376,15 @@ extern "C" int inspire_engine_wait(inspire_engine_t* g, uint64_t ticket, uint8_t
   if (!resp || cap < *resp_len) return set_error(INSPIRE_ERR_CAPACITY, "response buffer too small");
   g->cv_done.wait(lk, [&] { return r.state == kDone || r.state == kFailed; });
   int rc = INSPIRE_OK;
-  (void)framing(&old);
+  // A state swap may have run while this request waited (queued, it then ran
+  // on the new state, whose stash can be larger): size the framing again
+  // under the lock that swaps hold, and leave the ticket collectible if the
+  // caller's buffer no longer fits.
+  const size_t need = framing(&old);
+  if (r.state == kDone && cap < need) {
+    *resp_len = need;
+    return set_error(INSPIRE_ERR_CAPACITY, "response buffer too small for the state the query ran on");
+  }
   if (r.state == kFailed) {
     rc = set_error(INSPIRE_ERR_HIP, r.error);
   } else {

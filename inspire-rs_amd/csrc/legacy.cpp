@@ -202,13 +202,20 @@ extern "C" void gpu_gemv_query(uint64_t* h_c, const uint64_t* h_a, const uint16_
     (void)hipFreeAsync(d_a, s);
     LEGACY_CHECK("GPU GEMV", e);
   }
-  LEGACY_CHECK("GPU GEMV", hipMemcpyAsync(d_a, h_a, (size_t)db_rows * 8, hipMemcpyHostToDevice, s));
-  launch_gemv(d_c, d_a, 1, d_db, db_rows, b_cols, s);
-  LEGACY_CHECK("GPU GEMV", hipGetLastError());
-  LEGACY_CHECK("GPU GEMV", hipMemcpyAsync(h_c, d_c, (size_t)b_cols * 8, hipMemcpyDeviceToHost, s));
-  LEGACY_CHECK("GPU GEMV", hipStreamSynchronize(s));
+  // every exit below releases both buffers
+  e = hipMemcpyAsync(d_a, h_a, (size_t)db_rows * 8, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    launch_gemv(d_c, d_a, 1, d_db, db_rows, b_cols, s);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(h_c, d_c, (size_t)b_cols * 8, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
   (void)hipFreeAsync(d_a, s);
   (void)hipFreeAsync(d_c, s);
+  if (e != hipSuccess) {
+    (void)hipStreamSynchronize(s);
+    std::fprintf(stderr, "[GPU GEMV] %s\n", hipGetErrorString(e));
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -901,6 +901,30 @@ void orc_synth_fill(void* out, uint64_t seed, uint32_t stream, uint64_t start, u
     }
 }
 
+/* orc_gemv for nq query rows over columns [j0, j0 + ncols) of a synthetic DB
+ * (kind 2 values of [db_cols][db_rows] at index col * db_rows + row), each
+ * value drawn from the generator where it is used rather than from a host
+ * copy: c[h][j] = sum_k a[h][k] * db[j0 + j][k] mod q, as orc_gemv. */
+void orc_gemv_synth(uint64_t* c, const uint64_t* a, int nq, uint64_t seed, uint32_t stream, int64_t db_rows,
+                    int64_t j0, int64_t ncols) {
+    init_all();
+#pragma omp parallel for schedule(static)
+    for (int64_t j = 0; j < ncols; j++) {
+        uint64_t lo[8] = {0}, hi[8] = {0};
+        const uint64_t base = (uint64_t)(j0 + j) * (uint64_t)db_rows;
+        for (int64_t k = 0; k < db_rows; k++) {
+            const uint64_t v = (orc_synth_word(seed, stream, base + (uint64_t)k) & 0xffffULL) % 65535ULL;
+            for (int h = 0; h < nq; h++) {
+                lo[h] += (a[h * db_rows + k] & 0xffffffffULL) * v;
+                hi[h] += (a[h * db_rows + k] >> 32) * v;
+            }
+        }
+        for (int h = 0; h < nq; h++)
+            c[h * ncols + j] = crt_compose_2(barrett_raw_u64(lo[h], C.cr1[0], MODULI[0]),
+                                             barrett_raw_u64(hi[h], C.cr1[1], MODULI[1]));
+    }
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

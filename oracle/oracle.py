@@ -171,6 +171,19 @@ def gemv(a, db, db_rows, db_cols):
     return c
 
 
+def gemv_synth(rows, seed, stream, db_rows, j0, ncols):
+    """gemv for each query row of `rows` [nq][db_rows] over columns
+    [j0, j0 + ncols) of the synthetic DB (synth kind KIND_U16 of stream
+    `stream`), without a host copy of the DB -> [nq][ncols]."""
+    rows = np.ascontiguousarray(rows, dtype=np.uint64)
+    nq = rows.shape[0]
+    assert rows.shape[1] == db_rows and 1 <= nq <= 8 and db_rows <= (1 << 19)
+    c = np.zeros((nq, ncols), dtype=np.uint64)
+    lib().orc_gemv_synth(_p(c), _p(rows), ctypes.c_int(nq), ctypes.c_uint64(seed), ctypes.c_uint32(stream),
+                         ctypes.c_int64(db_rows), ctypes.c_int64(j0), ctypes.c_int64(ncols))
+    return c
+
+
 def packing(bold_t, bold_t_bar, bold_t_hat, y_body, z_body, instances, j0=0, j1=None):
     """packing_online_kernel.cu:18-92 -> S [j1-j0][2][N]."""
     if j1 is None:

@@ -505,8 +505,10 @@ def test_engine_responses_equal_solo_responses(torch, window_ms, max_batch):
     db_rows, inst, d = 2048, 8, 4
     srv = IA.Server(db_rows, inst, d)
     srv.fill_synthetic(51)
-    srv.set_stash([bytes([i]) * 16 for i in range(3)])
-    qs = [q for q, _ in _queries(db_rows, [200 + i for i in range(24)])]
+    stash = [bytes([i]) * 16 for i in range(3)]
+    srv.set_stash(stash)
+    qparts = _queries(db_rows, [200 + i for i in range(24)])
+    qs = [q for q, _ in qparts]
     solo = [srv.respond_serialized(q) for q in qs]
     eng = IA.Engine(srv, nq=2, max_batch=max_batch, window_ms=window_ms, capacity=64)
     tickets = [None] * len(qs)
@@ -520,15 +522,22 @@ def test_engine_responses_equal_solo_responses(torch, window_ms, max_batch):
         t.start()
     for t in th:
         t.join()
+    got = []
     for i, t in enumerate(tickets):
         resp, lat = eng.wait(t)
         assert resp == solo[i], i
         assert lat > 0
+        got.append(resp)
     st = eng.stats()
     assert st["queries"] == len(qs) and 1 <= st["batches"] <= len(qs)
     assert st["max_batch"] == max_batch
     eng.close()
     srv.close()
+    # the engine's answers are the oracle's (responses and stash framing)
+    state = O.synth_state(51, db_rows, inst)
+    for i in (0, 13, 23):
+        exp = O.respond_raw(state[0], db_rows, inst, d, *state[1:], *qparts[i][1])
+        assert got[i] == O.serialize_response(exp, stash), i
 
 
 def test_engine_rejects_bad_queries_and_full_queue(torch):
@@ -581,13 +590,22 @@ def test_engines_on_two_databases(torch):
         t.start()
     for t in th:
         t.join()
+    got = [[None] * 10 for _ in shapes]
     for s in range(len(shapes)):
         for i in range(10):
-            assert engs[s].wait(tickets[s][i])[0] == solo[s][i], (s, i)
+            got[s][i] = engs[s].wait(tickets[s][i])[0]
+            assert got[s][i] == solo[s][i], (s, i)
     for e in engs:
         e.close()
     for srv in srvs:
         srv.close()
+    # one answer per database against the oracle
+    for s, (rows, inst, d, seed) in enumerate(shapes):
+        state = O.synth_state(seed, rows, inst)
+        i = 3 + 4 * s
+        parts = O.synth_query(seed * 10 + i, rows, nq=2)
+        exp = O.respond_raw(state[0], rows, inst, d, *state[1:], *parts)
+        assert got[s][i] == O.serialize_response(exp, []), (s, i)
 
 
 # --- DB encode (encode_kernel.cu:18-228) -------------------------------------
@@ -704,6 +722,82 @@ def test_legacy_encode_then_gemv(torch):
     L.gpu_encode_free_device_db()
     assert not L.gpu_encode_get_device_db()
     L.gpu_trim_memory_pool()
+
+
+def test_legacy_adopt_path_matches_oracle(torch):
+    # The calls KeywordServer::setup and the offline precomputation make
+    # through the reference's FFI (keyword.rs:404-457, engine.rs:936-939): the
+    # encoded device DB adopted by gpu_gemv_set_device_db and queried with
+    # gpu_gemv_query; device-resident precomputation adopted by
+    # gpu_packing_online_adopt_precomp, keys expanded on the device or uploaded
+    # as host rotations (gpu_packing_online_upload_keys), and a second
+    # precomputation swapped in (gpu_packing_online_swap_precomp).
+    say("legacy adopt path")
+    import ctypes
+    L = IA.lib()
+    L.gpu_encode_get_device_db.restype = ctypes.c_void_p
+    vp = ctypes.c_void_p
+    u = ctypes.c_uint64
+    k = O.constants()
+    ap = lambda x: x.ctypes.data_as(vp)
+    L.gpu_init_memory_pool()
+    db_rows, d, c = 2048, 2, 2
+    inst = c * d
+    db_cols = inst * N
+    raw = np.random.default_rng(43).integers(0, 65536, size=db_rows * db_cols, dtype=np.uint16)
+    L.gpu_encode_db_from_host(ap(raw), None, db_rows, db_cols, d, N, c, ctypes.c_uint32(65535),
+                              ctypes.c_uint32(pow(d, -1, 65535)))
+    enc = L.gpu_encode_get_device_db()
+    assert enc
+    d_db = vp()
+    L.gpu_gemv_set_device_db(ctypes.byref(d_db), vp(enc), ctypes.c_size_t(raw.nbytes))
+    assert d_db.value == enc
+    a = O.synth(85, O.STREAM_ROW, db_rows, O.KIND_CONDENSED)
+    got = np.zeros(db_cols, dtype=np.uint64)
+    L.gpu_gemv_query(ap(got), ap(a), d_db, db_rows, db_cols, u(O.Q0), u(O.Q1), u(k["cr1_q0"]), u(k["cr1_q1"]),
+                     u(k["mod0_inv_mod1"]), u(k["mod1_inv_mod0"]), u(O.Q), u(k["cr0_q"]), u(k["cr1_q"]))
+    assert (got == O.gemv(a, O.encode(raw, db_rows, db_cols, d), db_rows, db_cols)).all()
+    L.gpu_encode_free_device_db()
+
+    tables = np.concatenate([O.automorph_table(t) for t in range(1, 2 * N, 2)]).astype(np.uint32)
+    gen_pows = np.array([pow(5, i, 2 * N) for i in range(O.NROT)], dtype=np.uint32)
+    L.gpu_rotation_upload_tables(ap(tables), ap(gen_pows), N, O.NROT, N)
+    L.gpu_packing_online_ensure_key_buffers(O.NINNER, 3, N)
+    y, z, _, _ = O.synth_query(86, 2048, nq=1)
+    states = []
+    for seed in (87, 88):
+        host_pre = [O.synth(seed, O.STREAM_BT, inst * O.NINNER * N, O.KIND_CONDENSED),
+                    O.synth(seed, O.STREAM_BTB, inst * O.NINNER * N, O.KIND_CONDENSED),
+                    O.synth(seed, O.STREAM_BTH, inst * 3 * N, O.KIND_CONDENSED)]
+        states.append((host_pre, [dev_from(torch, x) for x in host_pre]))
+
+    def compute():
+        out = np.zeros(inst * 2 * N, dtype=np.uint64)
+        L.gpu_packing_online_compute(ap(out), inst, O.NROT, 3, N, 128, u(O.Q0), u(O.Q1), u(k["cr1_q0"]),
+                                     u(k["cr1_q1"]))
+        return out.reshape(inst, 2, N)
+
+    (h0, d0), (h1, d1) = states
+    L.gpu_packing_online_adopt_precomp(*[vp(t.data_ptr()) for t in d0], inst, O.NINNER, 3, N)
+    L.gpu_rotation_expand_keys(vp(L.gpu_packing_online_get_d_y_all()), vp(L.gpu_packing_online_get_d_y_bar_all()),
+                               ap(y), O.NROT, 3, N)
+    L.gpu_packing_online_upload_z_body(ap(z), 3, N)
+    exp0 = O.packing(*h0, y, z, inst)
+    assert (compute() == exp0).all()
+    # the same keys as host rotations: y_all[i][k] = y[k] through the table of
+    # g_i = 5^i, y_bar_all[i][k] through the table of 2n - g_i (rotation_kernel.cu:21-59)
+    tab = tables.reshape(N, N)
+    y_all = np.stack([y[:, tab[(g - 1) // 2]] for g in gen_pows])
+    y_bar_all = np.stack([y[:, tab[(2 * N - g - 1) // 2]] for g in gen_pows])
+    L.gpu_packing_online_upload_keys(ap(np.ascontiguousarray(y_all)), ap(np.ascontiguousarray(y_bar_all)), ap(z),
+                                     O.NINNER, 3, N)
+    assert (compute() == exp0).all()
+    L.gpu_packing_online_swap_precomp(*[vp(t.data_ptr()) for t in d1], inst, O.NINNER, 3, N)
+    assert (compute() == O.packing(*h1, y, z, inst)).all()
+    L.gpu_packing_online_free()
+    L.gpu_rotation_free()
+    torch.cuda.synchronize()
+    del states, d0, d1
 
 
 def test_engine_live_update_swaps_state(torch):

@@ -226,6 +226,24 @@ def test_gemv_closed_form():
         assert int(c[j]) == sum(x[k] * int(col[k]) for k in range(rows)) % O.Q
 
 
+def test_gemv_synth_closed_form():
+    # the generator-fed GEMV (used for the 94.5 GB DB checks) against the
+    # closed form on columns 37..39 of a synthetic DB, 3 query rows, and
+    # against orc_gemv on a wider slice
+    rows, j0, ncols = 2048, 37, 40
+    db = O.synth(71, O.STREAM_DB, (j0 + ncols) * rows, O.KIND_U16)
+    a = O.synth(72, O.STREAM_ROW, 3 * rows, O.KIND_CONDENSED).reshape(3, rows)
+    got = O.gemv_synth(a, 71, O.STREAM_DB, rows, j0, ncols)
+    crt = [[(int(v) & 0xffffffff, int(v) >> 32) for v in a[h]] for h in range(3)]
+    x = [[(lo * O.Q1 * pow(O.Q1, -1, O.Q0) + hi * O.Q0 * pow(O.Q0, -1, O.Q1)) % O.Q for lo, hi in crt[h]]
+         for h in range(3)]
+    for h in range(3):
+        for j in range(3):
+            col = db[(j0 + j) * rows:(j0 + j + 1) * rows]
+            assert int(got[h][j]) == sum(x[h][k] * int(col[k]) for k in range(rows)) % O.Q
+        assert (got[h] == O.gemv(a[h], db[j0 * rows:], rows, ncols)).all()
+
+
 def test_packing_closed_form_small():
     rng = np.random.default_rng(13)
     inst = 1
