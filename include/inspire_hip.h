@@ -225,6 +225,28 @@ int inspire_respond_serialized(inspire_server_t* srv, const uint8_t* query, size
  * `stream` (hipStream_t; NULL = the server's own stream). */
 int inspire_respond_device(inspire_server_t* srv, const uint8_t* d_query, size_t len, int nq, uint8_t* d_out,
                            void* stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): inspire_respond_device with the request's working
+ * buffers (unpacked query, packing and GEMV outputs, fragments) in one
+ * file-scope device array of INSPIRE_NOALLOC_SCRATCH_BYTES per GPU instead of
+ * the server's request slots.  Same arguments, same response bytes; the call
+ * allocates nothing (no hipMalloc, hipFree or allocating library routine).
+ * A request needing more than the array (inspire_noalloc_scratch_bytes; at
+ * 2048 rows and 2 hash queries about 64 KiB per packed output, so up to the
+ * 1024 outputs of a 4M-key ML-KEM-512 database) is refused with
+ * INSPIRE_ERR_INVALID.  Every call on one GPU shares that GPU's array, which
+ * keeps the last call's contents: calls that overlap on one GPU (other
+ * streams, other servers, other threads) race, so the caller serializes them,
+ * e.g. on one stream.  The array stays allocated on each GPU the library has
+ * run on until the process ends. */
+#define INSPIRE_NOALLOC_SCRATCH_BYTES 100663296u /* 96 MiB */
+int inspire_respond_device_noalloc(inspire_server_t* srv, const uint8_t* d_query, size_t len, int nq,
+                                   uint8_t* d_out, void* stream);
+/* Scratch bytes inspire_respond_device_noalloc needs for a shape and nq hash
+ * queries (0 for a bad argument).  Host only. */
+size_t inspire_noalloc_scratch_bytes(const inspire_shape_t* shape, int nq);
+
 /* Number of request slots (default 2, at most 16).  Consecutive respond calls
  * use the slots in turn, so requests issued on different streams overlap on
  * the device; a slot is reused only after its previous request completed. */

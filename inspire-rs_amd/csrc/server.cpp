@@ -1072,8 +1072,10 @@ namespace {
 // same result.
 int run_pipeline(inspire_server_t* s, Slot& sl, const uint8_t* d_query, int nq, uint8_t* d_out, hipStream_t st) {
   const int64_t rows = s->shape.db_rows, inst = s->shape.instances;
-  // the slot's previous request (on whatever stream) must be finished
-  HIP_TRY(hipStreamWaitEvent(st, sl.done, 0), "slot wait");
+  // the slot's previous request (on whatever stream) must be finished; the
+  // scratch-array slot of inspire_respond_device_noalloc has no event (its
+  // callers order their calls themselves)
+  if (sl.done) HIP_TRY(hipStreamWaitEvent(st, sl.done, 0), "slot wait");
   auto mark = [&](int i) {
     if (s->profile) (void)hipEventRecord(s->ev[i], st);
   };
@@ -1114,7 +1116,7 @@ int run_pipeline(inspire_server_t* s, Slot& sl, const uint8_t* d_query, int nq, 
     return set_error(INSPIRE_ERR_INVALID, "unsupported fold shape");
   mark(5);
   HIP_TRY(hipGetLastError(), "respond launch");
-  HIP_TRY(hipEventRecord(sl.done, st), "slot done");
+  if (sl.done) HIP_TRY(hipEventRecord(sl.done, st), "slot done");
   if (s->profile) {
     // event pairs in launch order (unpack, GEMV, packing, post, fold) and the
     // stage each one times (stage order: unpack, packing, post, GEMV, fold)
@@ -1147,6 +1149,104 @@ extern "C" int inspire_respond_device(inspire_server_t* s, const uint8_t* d_quer
   if (nq == 0) return INSPIRE_OK;
   hipStream_t st = stream ? (hipStream_t)stream : s->stream;
   return run_pipeline(s, take_slot(s), d_query, nq, d_out, st);
+}
+
+// ---------------------------------------------------------------------------
+// Respond without allocating (inspire_respond_device_noalloc): the request's
+// working buffers live in one file-scope device array per GPU instead of the
+// server's hipMalloc'd request slots.
+// ---------------------------------------------------------------------------
+
+// Every call on a GPU uses this array; it keeps the last call's contents.
+static __device__ __align__(256) uint8_t scratch[INSPIRE_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_ptr[64] = {};  // device address of `scratch` per device
+
+constexpr size_t align256(size_t n) { return (n + 255) / 256 * 256; }
+
+// Byte offsets of the request buffers inside the array for a shape and nq.
+struct ScratchLayout {
+  size_t y_body, z_body, rows, bodies, S, P, cvec, gemv, keys, total;
+};
+
+ScratchLayout scratch_layout(int64_t db_rows, int64_t instances, int nq) {
+  ScratchLayout l{};
+  size_t o = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = o;
+    o += align256(bytes);
+    return at;
+  };
+  l.y_body = take((size_t)kTExp * kN * 8);
+  l.z_body = take((size_t)kTExp * kN * 8);
+  l.rows = take((size_t)nq * db_rows * 8);
+  l.bodies = take((size_t)nq * 6 * 2 * kN * 4);
+  l.S = take((size_t)instances * 2 * kN * 4);
+  l.P = take((size_t)instances * kN * 8);
+  l.cvec = take((size_t)nq * instances * kN * 8);
+  l.gemv = take(gemv_mfma_scratch_bytes((int)db_rows));
+  l.keys = take(packing_afrag_bytes(1));
+  l.total = o;
+  return l;
+}
+
+// The current device's address of `scratch`, looked up once per device (with
+// that device current); host code passes this address, never the array's
+// name, to kernels.
+int scratch_address(int dev, uint8_t** out) {
+  if (dev < 0 || dev >= 64) return set_error(INSPIRE_ERR_INVALID, "device index beyond 63");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!p) return set_error(INSPIRE_ERR_HIP, "hipGetSymbolAddress(scratch) returned null");
+    g_scratch_ptr[dev] = static_cast<uint8_t*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return INSPIRE_OK;
+}
+
+}  // namespace
+
+extern "C" size_t inspire_noalloc_scratch_bytes(const inspire_shape_t* shape, int nq) {
+  if (!shape || nq < 1 || shape->db_rows <= 0 || shape->instances <= 0) return 0;
+  return scratch_layout(shape->db_rows, shape->instances, nq).total;
+}
+
+extern "C" int inspire_respond_device_noalloc(inspire_server_t* s, const uint8_t* d_query, size_t len, int nq,
+                                              uint8_t* d_out, void* stream) {
+  if (!s || !d_query || !d_out) return set_error(INSPIRE_ERR_INVALID, "null argument");
+  if (!s->have_db || !s->have_precomp) return set_error(INSPIRE_ERR_STATE, "database or precomputation not loaded");
+  if (nq < 0 || nq > s->shape.max_queries) return set_error(INSPIRE_ERR_INVALID, "too many hash queries");
+  if (len < query_bytes(s->shape.db_rows, nq)) return set_error(INSPIRE_ERR_INVALID, "keyword query truncated");
+  if (nq == 0) return INSPIRE_OK;
+  const ScratchLayout l = scratch_layout(s->shape.db_rows, s->shape.instances, nq);
+  if (l.total > (size_t)INSPIRE_NOALLOC_SCRATCH_BYTES)
+    return set_error(INSPIRE_ERR_INVALID,
+                     "request needs " + std::to_string(l.total) + " scratch bytes, beyond the " +
+                         std::to_string((size_t)INSPIRE_NOALLOC_SCRATCH_BYTES) +
+                         "-byte no-allocation array (fewer hash queries or outputs per call)");
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+  if (dev != s->device) return set_error(INSPIRE_ERR_INVALID, "the server's device is not the current device");
+  uint8_t* base = nullptr;
+  int rc = scratch_address(dev, &base);
+  if (rc) return rc;
+  Slot sl;  // no event: calls that overlap on one GPU race on the array
+  sl.y_body = reinterpret_cast<uint64_t*>(base + l.y_body);
+  sl.z_body = reinterpret_cast<uint64_t*>(base + l.z_body);
+  sl.rows = reinterpret_cast<uint64_t*>(base + l.rows);
+  sl.bodies = reinterpret_cast<uint32_t*>(base + l.bodies);
+  sl.S = reinterpret_cast<uint32_t*>(base + l.S);
+  sl.P = reinterpret_cast<uint64_t*>(base + l.P);
+  sl.cvec = reinterpret_cast<uint64_t*>(base + l.cvec);
+  sl.gemv_scratch = base + l.gemv;
+  sl.keys = base + l.keys;
+  hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+  return run_pipeline(s, sl, d_query, nq, d_out, st);
 }
 
 extern "C" int inspire_respond_serialized(inspire_server_t* s, const uint8_t* query, size_t len, uint8_t* resp,

@@ -98,6 +98,9 @@ def lib():
         L.inspire_server_response_bytes.restype = sz
         L.inspire_respond_serialized.argtypes = [vp, vp, sz, vp, sz, ctypes.POINTER(sz)]
         L.inspire_respond_device.argtypes = [vp, vp, sz, i32, vp, vp]
+        L.inspire_respond_device_noalloc.argtypes = [vp, vp, sz, i32, vp, vp]
+        L.inspire_noalloc_scratch_bytes.argtypes = [ctypes.POINTER(ShapeT), i32]
+        L.inspire_noalloc_scratch_bytes.restype = sz
         L.inspire_server_profile_enable.argtypes = [vp, i32]
         L.inspire_server_profile_read.argtypes = [vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64), i32]
         L.inspire_server_stream.argtypes = [vp]
@@ -283,6 +286,11 @@ class Server:
     def respond_device(self, d_query, length, nq, d_out, stream=None):
         check(lib().inspire_respond_device(self.h, _ptr(d_query), length, nq, _ptr(d_out), stream))
 
+    def respond_device_noalloc(self, d_query, length, nq, d_out, stream=None):
+        """respond_device with the working buffers in the library's per-GPU
+        scratch array (no device allocation; calls on one GPU must not overlap)."""
+        check(lib().inspire_respond_device_noalloc(self.h, _ptr(d_query), length, nq, _ptr(d_out), stream))
+
 
 class Engine:
     """Continuous-batching front end over a Server (inspire_engine_* in the C ABI).
@@ -331,6 +339,15 @@ class Engine:
         if self.h:
             lib().inspire_engine_destroy(self.h)
             self.h = None
+
+
+NOALLOC_SCRATCH_BYTES = 96 << 20
+
+
+def noalloc_scratch_bytes(db_rows, instances, interpolate_degree, nq, max_queries=2):
+    """Scratch bytes inspire_respond_device_noalloc needs for a shape (host only)."""
+    return lib().inspire_noalloc_scratch_bytes(ctypes.byref(ShapeT(db_rows, instances, interpolate_degree,
+                                                                   max_queries)), nq)
 
 
 def synth_fill(d_out, seed, stream_id, count, kind, start=0, stream=None):

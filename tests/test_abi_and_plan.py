@@ -96,3 +96,17 @@ def test_server_rejects_bad_shapes_before_touching_a_device():
         IA.Server(2048, 65536 + 32, 32)
     with pytest.raises(IA.InspireError, match="max_queries"):
         IA.Server(2048, 352, 32, max_queries=0)
+
+
+def test_noalloc_scratch_sizes():
+    # host-only sizing of inspire_respond_device_noalloc's per-GPU array: the
+    # 1M-key and 32M-key shapes and the 4M-key ML-KEM-512 database (1024
+    # outputs) fit with 2 hash queries; 100 hash queries over 64 outputs do not
+    fits = [(2048, 352, 32, 2), (65536, 352, 32, 2), (2048, 1024, 32, 2), (2048, 8, 8, 1)]
+    for rows, inst, d, nq in fits:
+        need = IA.noalloc_scratch_bytes(rows, inst, d, nq)
+        assert 0 < need <= IA.NOALLOC_SCRATCH_BYTES, (rows, inst, nq, need)
+    assert IA.noalloc_scratch_bytes(2048, 64, 32, 100, max_queries=100) > IA.NOALLOC_SCRATCH_BYTES
+    # per call: the packing and GEMV outputs dominate (about 64 KiB per output at nq 2)
+    assert IA.noalloc_scratch_bytes(2048, 352, 32, 2) > 352 * 64 * 1024
+    assert IA.noalloc_scratch_bytes(2048, 352, 32, 0) == 0
