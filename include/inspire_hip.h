@@ -300,7 +300,7 @@ int inspire_server_set_batch_kernels(inspire_server_t* srv, int packing_variant,
 typedef struct {
   int packing_variant, gemv_variant, fold_variant; /* chosen */
   int packing_variants, gemv_variants;             /* variants timed */
-  double packing_ms[8], gemv_ms[8], fold_ms[3];    /* per variant (fold: [1], [2]) */
+  double packing_ms[16], gemv_ms[8], fold_ms[3];   /* per variant (fold: [1], [2]) */
   int nbatch;
   int batch_sizes[16];
   double batch_qps[16]; /* keyword queries per second of the batch stages */

@@ -692,7 +692,7 @@ extern "C" int inspire_server_autotune(inspire_server_t* s, int count, int nq, i
   // the packing variants differ only for groups of 64 clients
   out->packing_variants = batch_key_pad(count) >= 64 ? packing_wide_variants() : 1;
   best = 1e30;
-  for (int v = 0; v < out->packing_variants && v < 8; ++v) {
+  for (int v = 0; v < out->packing_variants && v < 16; ++v) {
     out->packing_ms[v] = timed([&] { stages(count, v, 0, 0); });
     if (out->packing_ms[v] < 0) rc = INSPIRE_ERR_HIP;
     if (out->packing_ms[v] >= 0 && out->packing_ms[v] < best) best = out->packing_ms[v], out->packing_variant = v;

@@ -38,7 +38,7 @@ class TuneResult(ctypes.Structure):
     """inspire_tune_t (include/inspire_hip.h)."""
     _fields_ = [("packing_variant", ctypes.c_int), ("gemv_variant", ctypes.c_int), ("fold_variant", ctypes.c_int),
                 ("packing_variants", ctypes.c_int), ("gemv_variants", ctypes.c_int),
-                ("packing_ms", ctypes.c_double * 8), ("gemv_ms", ctypes.c_double * 8), ("fold_ms", ctypes.c_double * 3),
+                ("packing_ms", ctypes.c_double * 16), ("gemv_ms", ctypes.c_double * 8), ("fold_ms", ctypes.c_double * 3),
                 ("nbatch", ctypes.c_int), ("batch_sizes", ctypes.c_int * 16), ("batch_qps", ctypes.c_double * 16),
                 ("suggested_batch", ctypes.c_int)]
 

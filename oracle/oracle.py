@@ -11,7 +11,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+LIB_PATH = os.path.join(HERE, "liboracle.so")          # x86-64-v3 (AVX2)
+LIB_PATH_V4 = os.path.join(HERE, "liboracle_v4.so")    # x86-64-v4 (AVX-512)
+_V4_FLAGS = ("avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl")
 
 N = 2048
 TEXP = 3
@@ -26,15 +28,59 @@ _lib = None
 
 
 def build():
-    subprocess.run(["make", "-s", "-C", HERE, "-B", "liboracle.so"], check=True)
+    subprocess.run(["make", "-s", "-C", HERE, "-B", "all"], check=True)
+
+
+def host_info():
+    """CPU model, hardware threads, RAM and the ISA level of the loaded oracle."""
+    model, flags = "", set()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name") and not model:
+                model = line.split(":", 1)[1].strip()
+            elif line.startswith("flags") and not flags:
+                flags = set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    ram = 0
+    try:
+        for line in open("/proc/meminfo"):
+            if line.startswith("MemTotal"):
+                ram = int(line.split()[1]) * 1024
+    except OSError:
+        pass
+    return {"cpu_model": model, "hw_threads": os.cpu_count(), "ram_bytes": ram,
+            "avx512": all(f in flags for f in _V4_FLAGS), "oracle_isa": lib_isa()}
+
+
+def _has_v4():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("flags"):
+                return all(f in line.split() for f in _V4_FLAGS)
+    except OSError:
+        pass
+    return False
+
+
+_isa = None
+
+
+def lib_isa():
+    lib()
+    return _isa
 
 
 def lib():
-    global _lib
+    """The oracle library: the AVX-512 build on hosts that have AVX-512, else AVX2."""
+    global _lib, _isa
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        _lib = ctypes.CDLL(LIB_PATH)
+        path, _isa = LIB_PATH, "x86-64-v3 (AVX2)"
+        if _has_v4() and os.path.exists(LIB_PATH_V4):
+            path, _isa = LIB_PATH_V4, "x86-64-v4 (AVX-512)"
+        _lib = ctypes.CDLL(path)
         _lib.orc_synth_word.restype = ctypes.c_uint64
         _lib.orc_synth_word.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64]
         _lib.orc_read_bits.restype = ctypes.c_uint64

@@ -440,18 +440,27 @@ def test_batch_kernel_variants_agree_and_autotune(torch, count, inst, d):
         return dout.cpu().numpy().tobytes()
 
     ref = run()
-    for pv in range(5):
+    # the library refuses variant numbers past its last one
+    npack = 0
+    while True:
+        try:
+            srv.set_batch_kernels(npack, 0, 0)
+        except IA.InspireError:
+            break
+        npack += 1
+    assert npack >= 5
+    for pv in range(npack):
         for gv in range(2):
             for fv in range(3):
                 srv.set_batch_kernels(pv, gv, fv)
                 assert run() == ref, (pv, gv, fv)
     t = srv.autotune(count, nq=2, reps=2)
     say("autotune", t)
-    assert len(t["packing_ms"]) == 5 and min(t["packing_ms"]) > 0 and min(t["fold_ms"]) > 0
+    assert len(t["packing_ms"]) == npack and min(t["packing_ms"]) > 0 and min(t["fold_ms"]) > 0
     assert t["suggested_batch"] in t["batch_qps"]
     assert run() == ref
     with pytest.raises(IA.InspireError):
-        srv.set_batch_kernels(5, 0, 0)
+        srv.set_batch_kernels(npack, 0, 0)
     srv.close()
 
 
