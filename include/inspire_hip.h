@@ -409,6 +409,33 @@ int inspire_cuckoo_positions(const uint8_t* key, size_t key_len, int64_t num_buc
  * write, finish.  Host only. */
 uint64_t inspire_siphash24(uint64_t k0, uint64_t k1, const uint8_t* msg, size_t len);
 
+/* ---------------------------------------------------------------------- */
+/* Datasets: DatasetGenerator and the PQKEYV2 writer (host only)           */
+/* ---------------------------------------------------------------------- */
+
+/* DatasetGenerator::stream_to_writer + DatasetWriter (generator.rs:112-169,
+ * writer.rs:17-79): writes a PQKEYV2 file of `count` records for kem_name
+ * ("ml-kem-512", "ml-kem-768" or "ml-kem-1024"), record i = (id, public key)
+ * with the public key of DatasetGenerator::generate_record(i) for `seed`
+ * (generator.rs:170-186, kem.rs:27-44) and id = SHA3-256("pq-key:v1:" ||
+ * kem_name || key) (dataset.rs:36-46).  threads <= 0: one per hardware
+ * thread. */
+int inspire_dataset_generate(const char* path, const char* kem_name, int64_t count, uint64_t seed, int threads);
+/* The records start .. start+count-1 of the same stream into memory:
+ * ids [count][32], keys [count][public key length]. */
+int inspire_dataset_generate_records(const char* kem_name, uint64_t seed, int64_t start, int64_t count,
+                                     uint8_t* ids, uint8_t* keys, int threads);
+/* The 64-byte ML-KEM seed of record `index` (generator.rs:170-173:
+ * ChaCha20Rng::seed_from_u64(seed) at word position 16 * index). */
+int inspire_dataset_record_seed(uint64_t seed, uint64_t index, uint8_t out64[64]);
+/* KemVariant::generate_public_key_into (kem.rs:27-44): the encapsulation key
+ * of ML-KEM.KeyGen_internal(d = seed64[0..32], z = seed64[32..64]) (FIPS 203),
+ * 800 / 1184 / 1568 bytes; ek_cap too small: INSPIRE_ERR_CAPACITY. */
+int inspire_mlkem_keygen(const char* kem_name, const uint8_t seed64[64], uint8_t* ek, size_t ek_cap);
+/* FIPS 202 hashes used above: kind 0 SHA3-256 (out_len 32), 1 SHA3-512
+ * (out_len 64), 2 SHAKE128, 3 SHAKE256 (any out_len). */
+int inspire_keccak(int kind, const uint8_t* in, size_t len, uint8_t* out, size_t out_len);
+
 /* ------------------------------------------------------------------------
  * Continuous-batching engine (no counterpart in the reference, whose server
  * loop answers one query per round trip, server.rs:116-158).  Callers submit

@@ -128,6 +128,11 @@ def lib():
         L.inspire_engine_stats.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(i32)]
         L.inspire_engine_destroy.argtypes = [vp]
         L.inspire_engine_destroy.restype = None
+        L.inspire_dataset_generate.argtypes = [ctypes.c_char_p, ctypes.c_char_p, i64, u64, i32]
+        L.inspire_dataset_generate_records.argtypes = [ctypes.c_char_p, u64, i64, i64, vp, vp, i32]
+        L.inspire_dataset_record_seed.argtypes = [u64, u64, vp]
+        L.inspire_mlkem_keygen.argtypes = [ctypes.c_char_p, vp, vp, sz]
+        L.inspire_keccak.argtypes = [i32, vp, sz, vp, sz]
         L.gpu_available_memory_bytes.restype = u64
         L.gpu_packing_online_get_d_y_all.restype = vp
         L.gpu_packing_online_get_d_y_bar_all.restype = vp
@@ -390,6 +395,48 @@ def _plan_of(par):
     client/keyword.rs:60-118)."""
     kv = par["entry_size"]
     return plan_shape(par["num_items"], kv - 1, 1, buckets=par["num_items"], dim0=par["dim0"])
+
+
+KEM_PUBLIC_KEY_LEN = {"ml-kem-512": 800, "ml-kem-768": 1184, "ml-kem-1024": 1568}
+
+
+def generate_dataset(path, kem_name, count, seed=1, threads=0):
+    """DatasetGenerator::stream_to_writer to a PQKEYV2 file (generator.rs:112-169)."""
+    check(lib().inspire_dataset_generate(os.fsencode(path), kem_name.encode(), count, seed, threads))
+
+
+def generate_records(kem_name, count, seed=1, start=0, threads=0):
+    """Records start..start+count-1 of DatasetGenerator(seed): list of (id, public key)."""
+    pk_len = KEM_PUBLIC_KEY_LEN[kem_name]
+    ids = ctypes.create_string_buffer(32 * count)
+    keys = ctypes.create_string_buffer(pk_len * count)
+    check(lib().inspire_dataset_generate_records(kem_name.encode(), seed, start, count, ctypes.cast(ids, ctypes.c_void_p),
+                                                 ctypes.cast(keys, ctypes.c_void_p), threads))
+    return [(ids.raw[32 * i:32 * i + 32], keys.raw[pk_len * i:pk_len * (i + 1)]) for i in range(count)]
+
+
+def dataset_record_seed(seed, index):
+    out = ctypes.create_string_buffer(64)
+    check(lib().inspire_dataset_record_seed(seed, index, ctypes.cast(out, ctypes.c_void_p)))
+    return out.raw
+
+
+def mlkem_keygen(kem_name, seed64):
+    """kem.rs:27-44: ML-KEM.KeyGen_internal(d, z) encapsulation key."""
+    out = ctypes.create_string_buffer(KEM_PUBLIC_KEY_LEN[kem_name])
+    sb = ctypes.create_string_buffer(bytes(seed64), 64)
+    check(lib().inspire_mlkem_keygen(kem_name.encode(), ctypes.cast(sb, ctypes.c_void_p),
+                                     ctypes.cast(out, ctypes.c_void_p), len(out)))
+    return out.raw
+
+
+def keccak(kind, data, out_len):
+    """0 SHA3-256, 1 SHA3-512, 2 SHAKE128, 3 SHAKE256."""
+    out = ctypes.create_string_buffer(max(out_len, 1))
+    ib = ctypes.create_string_buffer(bytes(data), len(data) or 1)
+    check(lib().inspire_keccak(kind, ctypes.cast(ib, ctypes.c_void_p), len(data), ctypes.cast(out, ctypes.c_void_p),
+                               out_len))
+    return out.raw[:out_len]
 
 
 def cuckoo_positions(key, num_buckets):

@@ -1,10 +1,12 @@
-"""setup_keyword_server end to end on the GPU (tests/e2e_mlkem.rs:8-38 at the
-10,000-key shape): a PQKEYV2 file of 10,000 1184-byte public keys (random
-bytes in place of ML-KEM keys, ids = SHA3-256 as dataset.rs:36-46), the server
-built from it (cuckoo table, raw DB, encode and offline precomputation on the
-device), a client query for present ids, and extract returning the stored
-public key.  Two cases of the reference's own behaviour are checked as such:
-an absent id gives None, and an item whose 608 columns run past the end of a
+"""setup_keyword_server end to end on the GPU, the reference's
+tests/e2e_mlkem.rs:8-38 at its shape: a PQKEYV2 file of 10,000 ML-KEM-768
+public keys from DatasetGenerator seed 7 (ids = SHA3-256 as dataset.rs:36-46),
+the server built from it (cuckoo table, raw DB, encode and offline
+precomputation on the device), a client query for present ids, and extract
+returning the stored public key, whose id matches (public_key_matches_id).
+ML-KEM-512 and ML-KEM-1024 datasets (832- and 1600-byte entries) likewise.
+Two cases of the reference's own behaviour are checked as such: an absent id
+gives None, and an item whose 608 columns run past the end of a
 2048-coefficient polynomial cannot be extracted (the reference's slice runs
 past the selected polynomial's coefficients)."""
 import sys
@@ -27,11 +29,10 @@ def say(*a):
 def keyword_server(tmp_path_factory):
     import torch
     assert torch.cuda.is_available()
-    say("setup_keyword_server: 10,000 keys of 1184 bytes")
-    rng = np.random.default_rng(7)
-    pks = [rng.integers(0, 256, size=1184, dtype=np.uint8).tobytes() for _ in range(10_000)]
+    say("setup_keyword_server: DatasetGenerator(ml-kem-768, 10,000, seed 7)")
     path = tmp_path_factory.mktemp("ds") / "keys.pqkeyv2"
-    recs = O.write_pqkeyv2(str(path), pks)
+    IA.generate_dataset(str(path), "ml-kem-768", 10_000, seed=7)
+    recs = IA.generate_records("ml-kem-768", 10_000, seed=7)
     srv, pp = IA.setup_keyword_server(str(path))
     plan = IA.plan_shape(10_000, 32, 1184)
     yield srv, pp, plan, recs
@@ -74,6 +75,33 @@ def test_extract_returns_stored_public_key(keyword_server, hamming):
         if done == 4:
             break
     assert done == 4
+
+
+@pytest.mark.parametrize("kem,count", [("ml-kem-512", 6000), ("ml-kem-1024", 6000)])
+def test_other_kems_extract(tmp_path, kem, count):
+    say("setup_keyword_server: DatasetGenerator(%s, %d, seed 17)" % (kem, count))
+    path = tmp_path / "keys.pqkeyv2"
+    IA.generate_dataset(str(path), kem, count, seed=17)
+    recs = IA.generate_records(kem, count, seed=17)
+    srv, pp = IA.setup_keyword_server(str(path))
+    pk_len = IA.KEM_PUBLIC_KEY_LEN[kem]
+    assert pp["value_size"] == pk_len and pp["num_records"] == count
+    plan = IA.plan_shape(count, 32, pk_len)
+    client = O.KeywordClient(pp, plan, O.client_secret(7, O.HAMMING_WEIGHT))
+    done = 0
+    for idx in np.random.default_rng(count).permutation(count):
+        rid, pk = recs[idx]
+        if not _usable(client, rid) or (np.frombuffer(rid + pk, dtype=">u2") == 0xFFFF).any():
+            continue
+        pos, q = client.query(rid, seed=int(idx))
+        got = client.extract(rid, pos, srv.respond_serialized(q))
+        assert got == pk, int(idx)
+        assert O.public_key_id(got, kem) == rid
+        done += 1
+        if done == 2:
+            break
+    assert done == 2
+    srv.close()
 
 
 def test_absent_and_straddling_ids(keyword_server):
