@@ -551,6 +551,16 @@ void launch_post_zmajor(uint64_t* P, const uint32_t* S, const uint64_t* z_body, 
 // packed[j] = (a_hat_j, c_h[j*n..] + P_j).
 // ---------------------------------------------------------------------------
 
+// Workgroup x runs logical chain xcd_chunk(x, n) of n: XCD k (block id mod 8)
+// takes the contiguous range of n/8 logical chains (one more for k < n % 8)
+// that starts at k*(n/8) + min(k, n%8), so the chains of one client, which read
+// the same RGSW body at every Horner step and the same P rows, share an XCD's
+// L2.
+__device__ inline int xcd_chunk(int x, int n) {
+  const int q = n / 8, r = n % 8, k = x % 8;
+  return k * q + min(k, r) + x / 8;
+}
+
 constexpr int kFoldThreads = 1024;
 constexpr int kFoldSlots = kN / kFoldThreads;  // NTT slots owned per thread
 
@@ -612,9 +622,10 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
   uint32_t* prod = lds + 12 * kNPad;   // [2 (prime)][2 (row)][kNPad]
   const NttTables tw{args.tw};
   const int per_client = args.nq * args.c;
-  const int b = blockIdx.x / per_client;
-  const int h = (blockIdx.x % per_client) / args.c;
-  const int w = blockIdx.x % args.c;
+  const int chain = xcd_chunk((int)blockIdx.x, (int)gridDim.x);
+  const int b = chain / per_client;
+  const int h = (chain % per_client) / args.c;
+  const int w = chain % args.c;
   const int tid = threadIdx.x;
   const int d = args.d;
   const uint64_t* cv = args.cvec + ((size_t)b * args.nq + h) * args.instances * kN;
@@ -725,9 +736,10 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
   __shared__ uint32_t dig[6][kNPad];
   const NttTables tw{args.tw};
   const int per_client = args.nq * args.c;
-  const int b = blockIdx.x / per_client;
-  const int h = (blockIdx.x % per_client) / args.c;
-  const int w = blockIdx.x % args.c;
+  const int chain = xcd_chunk((int)blockIdx.x, (int)gridDim.x);
+  const int b = chain / per_client;
+  const int h = (chain % per_client) / args.c;
+  const int w = chain % args.c;
   const int tid = threadIdx.x;
   const int d = args.d;
   const uint64_t* cv = args.cvec + ((size_t)b * args.nq + h) * args.instances * kN;
