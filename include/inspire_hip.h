@@ -410,6 +410,45 @@ int inspire_cuckoo_positions(const uint8_t* key, size_t key_len, int64_t num_buc
 uint64_t inspire_siphash24(uint64_t k0, uint64_t k1, const uint8_t* msg, size_t len);
 
 /* ---------------------------------------------------------------------- */
+/* Keyword client (host only): KeywordClient, src/pir/client/keyword.rs    */
+/* ---------------------------------------------------------------------- */
+
+typedef struct inspire_client inspire_client_t;
+
+/* KeywordClient::setup_from_public_params (client/keyword.rs:60-121) from the
+ * server's public_params (the handshake), with the reference's error texts.
+ * hamming = 0 uses the reference client's secret, the zero polynomial of
+ * Client::init (client/base.rs:370-389); hamming = h in [1, 1024] draws a
+ * ternary secret with h coefficients 1 and h coefficients -1
+ * (gen_ternary_mat, client/base.rs:126-139).  Secret randomness (key and
+ * noise) is ChaCha20 under a key from getrandom(2); rng_seed != 0 derives that
+ * key from rng_seed instead (repeatable, for tests).  A client is not
+ * thread-safe: query advances its generator. */
+int inspire_client_new(const inspire_public_params_t* pp, int32_t hamming, uint64_t rng_seed,
+                       inspire_client_t** out);
+void inspire_client_free(inspire_client_t* client);
+/* The secret key, [2048] raw coefficients mod q (Client::get_sk_reg). */
+int inspire_client_secret(const inspire_client_t* client, uint64_t* sk);
+/* KeywordClient::positions (client/keyword.rs:133-145): num_hashes bucket
+ * indices under the handshake's cuckoo seed. */
+int inspire_client_positions(const inspire_client_t* client, const uint8_t* key, size_t key_len,
+                             int64_t* positions);
+/* Bytes of one serialized query (num_hashes hash queries). */
+size_t inspire_client_query_bytes(const inspire_client_t* client);
+/* KeywordClient::query + serialize_query (client/keyword.rs:198-210,
+ * wire.rs:542-615): fills positions[num_hashes] and writes the query;
+ * *len = its size (INSPIRE_ERR_CAPACITY if cap is smaller). */
+int inspire_client_query(inspire_client_t* client, const uint8_t* key, size_t key_len, int64_t* positions,
+                         uint8_t* out, size_t cap, size_t* len);
+/* KeywordClient::extract_from_serialized_response (client/keyword.rs:232-310):
+ * *found = 1 and the value_size bytes of the value in value[] when a chain
+ * blob or a stash entry holds the key, *found = 0 otherwise (the reference's
+ * None).  A malformed response is INSPIRE_ERR_INVALID. */
+int inspire_client_extract(const inspire_client_t* client, const uint8_t* key, size_t key_len,
+                           const int64_t* positions, const uint8_t* response, size_t response_len,
+                           uint8_t* value, size_t cap, size_t* value_len, int32_t* found);
+
+/* ---------------------------------------------------------------------- */
 /* Datasets: DatasetGenerator and the PQKEYV2 writer (host only)           */
 /* ---------------------------------------------------------------------- */
 

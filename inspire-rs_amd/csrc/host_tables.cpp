@@ -138,6 +138,31 @@ void host_ntt_forward(uint32_t* a, int c, const uint32_t* tables) {
   }
 }
 
+void host_ntt_inverse(uint32_t* a, int c, const uint32_t* tables) {
+  // Gentleman-Sande with the halved inverse powers: each stage halves, so the
+  // result carries n^-1.  Values stay canonical.
+  const uint32_t q = prime(c);
+  const uint32_t* w_t = tables + (c * 4 + 2) * kN;
+  for (int mm = kLogN - 1; mm >= 0; --mm) {
+    const int m = 1 << mm, t = kN >> (mm + 1);
+    for (int i = 0; i < m; ++i) {
+      const uint64_t w = w_t[m + i];
+      uint32_t* o = a + 2 * t * i;
+      for (int j = 0; j < t; ++j) {
+        const uint64_t x = o[j], y = o[t + j];
+        uint64_t s = x + y;
+        s = s >= q ? s - q : s;
+        o[j] = (uint32_t)((s & 1) ? (s + q) >> 1 : s >> 1);  // (x + y) / 2
+        o[t + j] = (uint32_t)(((x + q - y) * w) % q);         // (x - y) * w / 2
+      }
+    }
+  }
+}
+
+void chacha20_block_words(const uint8_t key[32], uint64_t counter, uint32_t out[16]) {
+  chacha_block(key, counter, out);
+}
+
 void chacha20_u64(const uint8_t seed[32], uint64_t* out, size_t n) {
   uint32_t blk[16];
   uint64_t counter = 0;

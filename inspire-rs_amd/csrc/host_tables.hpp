@@ -13,8 +13,16 @@ void build_ntt_tables(uint32_t* out);
 // In-place forward NTT of one residue polynomial (canonical in, canonical out).
 void host_ntt_forward(uint32_t* a, int c, const uint32_t* tables);
 
+// In-place inverse NTT of one residue polynomial, scaled by n^-1 (the
+// halved inverse powers of build_ntt_tables); canonical in and out.
+void host_ntt_inverse(uint32_t* a, int c, const uint32_t* tables);
+
 // ChaCha20Rng::from_seed(seed) followed by n u64 draws (low word first).
 void chacha20_u64(const uint8_t seed[32], uint64_t* out, size_t n);
+
+// ChaCha20 block (RFC 8439 state with a 64-bit block counter in words 12-13
+// and a zero stream id, as rand_chacha uses it): 16 output words.
+void chacha20_block_words(const uint8_t key[32], uint64_t counter, uint32_t out[16]);
 
 // out: [2][kN] = NTT(-a mod q_c) for the constant RGSW row-0 polynomial a.
 void build_rgsw_neg_a(uint32_t* out, const uint32_t* tables);

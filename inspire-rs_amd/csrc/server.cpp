@@ -159,6 +159,20 @@ int rgsw_params(int64_t num_items, int64_t item_size_bits, int64_t dim0, RgswPar
 
 }  // namespace
 
+namespace inspire {
+// The scheme shape of (num_items, item_size_bits, dim0), for the client.
+int scheme_shape(int64_t num_items, int64_t item_size_bits, int64_t dim0, int64_t* db_rows, int64_t* instances,
+                 int64_t* interpolate_degree) {
+  RgswParams rp;
+  const int rc = rgsw_params(num_items, item_size_bits, dim0, &rp);
+  if (rc) return rc;
+  *db_rows = rp.db_rows;
+  *instances = rp.instances;
+  *interpolate_degree = rp.interpolate_degree;
+  return INSPIRE_OK;
+}
+}  // namespace inspire
+
 extern "C" int inspire_plan_shape(int64_t record_count, int64_t key_size, int64_t value_size, int64_t buckets,
                                   int64_t dim0, inspire_plan_t* out) {
   if (!out) return set_error(INSPIRE_ERR_INVALID, "null plan");

@@ -133,6 +133,16 @@ def lib():
         L.inspire_dataset_record_seed.argtypes = [u64, u64, vp]
         L.inspire_mlkem_keygen.argtypes = [ctypes.c_char_p, vp, vp, sz]
         L.inspire_keccak.argtypes = [i32, vp, sz, vp, sz]
+        L.inspire_client_new.argtypes = [ctypes.POINTER(PublicParamsT), i32, u64, ctypes.POINTER(vp)]
+        L.inspire_client_free.argtypes = [vp]
+        L.inspire_client_free.restype = None
+        L.inspire_client_secret.argtypes = [vp, vp]
+        L.inspire_client_positions.argtypes = [vp, vp, sz, ctypes.POINTER(i64)]
+        L.inspire_client_query_bytes.argtypes = [vp]
+        L.inspire_client_query_bytes.restype = sz
+        L.inspire_client_query.argtypes = [vp, vp, sz, ctypes.POINTER(i64), vp, sz, ctypes.POINTER(sz)]
+        L.inspire_client_extract.argtypes = [vp, vp, sz, ctypes.POINTER(i64), vp, sz, vp, sz, ctypes.POINTER(sz),
+                                             ctypes.POINTER(i32)]
         L.gpu_available_memory_bytes.restype = u64
         L.gpu_packing_online_get_d_y_all.restype = vp
         L.gpu_packing_online_get_d_y_bar_all.restype = vp
@@ -395,6 +405,111 @@ def _plan_of(par):
     client/keyword.rs:60-118)."""
     kv = par["entry_size"]
     return plan_shape(par["num_items"], kv - 1, 1, buckets=par["num_items"], dim0=par["dim0"])
+
+
+def _pp_struct(par):
+    pp = PublicParamsT()
+    pp.num_items, pp.item_size_bits, pp.dim0 = par["num_items"], par["item_size_bits"], par["dim0"]
+    seed = bytes(par["cuckoo_seed"])
+    if len(seed) != 16:
+        raise InspireError(ERR_INVALID, "cuckoo seed must be 16 bytes")
+    pp.cuckoo_seed[:] = list(seed)
+    pp.num_hashes = par["num_hashes"]
+    pp.key_size, pp.value_size, pp.entry_size = par["key_size"], par["value_size"], par["entry_size"]
+    pp.num_records = par.get("num_records", 0)
+    return pp
+
+
+class KeywordClientQuery:
+    """KeywordClientQuery (client/keyword.rs:26-43): the cuckoo positions and
+    the serialized query."""
+
+    def __init__(self, positions, data):
+        self.positions = list(positions)
+        self.data = data
+
+
+class KeywordClient:
+    """KeywordClient (src/pir/client/keyword.rs:45-311) in the library
+    (inspire_client_*).  public_params: the dict of setup_keyword_server /
+    wire.public_params_from_json.  hamming = 0 is the reference client's zero
+    secret (client/base.rs:370-389); hamming > 0 a ternary secret of that
+    weight.  rng_seed = 0 draws the secret randomness from the OS."""
+
+    def __init__(self, public_params, hamming=0, rng_seed=0):
+        self.public_params = dict(public_params)
+        self._pp = _pp_struct(self.public_params)
+        self._h = ctypes.c_void_p()
+        check(lib().inspire_client_new(ctypes.byref(self._pp), hamming, rng_seed, ctypes.byref(self._h)))
+
+    @classmethod
+    def setup_from_public_params(cls, public_params, hamming=0, rng_seed=0):
+        return cls(public_params, hamming, rng_seed)
+
+    def close(self):
+        if self._h:
+            lib().inspire_client_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def key_size(self):
+        return self.public_params["key_size"]
+
+    def value_size(self):
+        return self.public_params["value_size"]
+
+    def secret(self):
+        import numpy as np
+        sk = np.zeros(2048, dtype=np.uint64)
+        check(lib().inspire_client_secret(self._h, sk.ctypes.data))
+        return sk
+
+    def positions(self, key):
+        key = bytes(key)
+        out = (ctypes.c_int64 * self.public_params["num_hashes"])()
+        kb = ctypes.create_string_buffer(key, len(key))
+        check(lib().inspire_client_positions(self._h, ctypes.cast(kb, ctypes.c_void_p), len(key), out))
+        return list(out)
+
+    def query_bytes(self):
+        return lib().inspire_client_query_bytes(self._h)
+
+    def query(self, key):
+        """query + serialize_query: a KeywordClientQuery (positions, bytes)."""
+        key = bytes(key)
+        cap = self.query_bytes()
+        buf = ctypes.create_string_buffer(cap)
+        pos = (ctypes.c_int64 * self.public_params["num_hashes"])()
+        n = ctypes.c_size_t()
+        kb = ctypes.create_string_buffer(key, len(key))
+        check(lib().inspire_client_query(self._h, ctypes.cast(kb, ctypes.c_void_p), len(key), pos,
+                                         ctypes.cast(buf, ctypes.c_void_p), cap, ctypes.byref(n)))
+        return KeywordClientQuery(list(pos), buf.raw[:n.value])
+
+    def serialize_query(self, query):
+        return query.data
+
+    def extract_from_serialized_response(self, key, query, response):
+        """The value bytes, or None when neither a chain nor the stash holds the key."""
+        key, response = bytes(key), bytes(response)
+        pos = (ctypes.c_int64 * len(query.positions))(*query.positions)
+        cap = max(1, self.public_params["value_size"])
+        out = ctypes.create_string_buffer(cap)
+        vlen, found = ctypes.c_size_t(), ctypes.c_int32()
+        kb = ctypes.create_string_buffer(key, max(1, len(key)))
+        rb = ctypes.create_string_buffer(response, max(1, len(response)))
+        check(lib().inspire_client_extract(self._h, ctypes.cast(kb, ctypes.c_void_p), len(key), pos,
+                                           ctypes.cast(rb, ctypes.c_void_p), len(response),
+                                           ctypes.cast(out, ctypes.c_void_p), cap, ctypes.byref(vlen),
+                                           ctypes.byref(found)))
+        return out.raw[:vlen.value] if found.value else None
+
+    extract = extract_from_serialized_response
 
 
 KEM_PUBLIC_KEY_LEN = {"ml-kem-512": 800, "ml-kem-768": 1184, "ml-kem-1024": 1568}

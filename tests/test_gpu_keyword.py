@@ -157,3 +157,69 @@ def test_million_key_dataset_extracts(tmp_path):
             break
     assert done == 3
     srv.close()
+
+
+def _usable_records(keyword_server, n, seed):
+    srv, pp, plan, recs = keyword_server
+    checker = O.KeywordClient(pp, plan, O.client_secret(1, 0))
+    rng = np.random.default_rng(seed)
+    out = []
+    for idx in rng.permutation(len(recs)):
+        rid, pk = recs[idx]
+        words = np.frombuffer(rid + pk, dtype=">u2")
+        if _usable(checker, rid) and not (words == 0xFFFF).any():
+            out.append((int(idx), rid, pk))
+            if len(out) == n:
+                break
+    return out
+
+
+@pytest.mark.parametrize("hamming", [0, O.HAMMING_WEIGHT])
+def test_library_client_extracts_stored_public_key(keyword_server, hamming):
+    # the library's KeywordClient (inspire_client_*) against the GPU server
+    srv, pp, plan, recs = keyword_server
+    client = IA.KeywordClient(pp, hamming=hamming)
+    for idx, rid, pk in _usable_records(keyword_server, 4, 77 + hamming):
+        say("library client extract record", idx, "hamming", hamming)
+        q = client.query(rid)
+        assert q.positions == O.cuckoo_positions(rid, pp["num_items"])
+        got = client.extract_from_serialized_response(rid, q, srv.respond_serialized(q.data))
+        assert got == pk and O.public_key_id(got) == rid, idx
+    absent = bytes(32)
+    q = client.query(absent)
+    assert client.extract(absent, q, srv.respond_serialized(q.data)) is None
+
+
+def test_tcp_server_answers_concurrent_clients(keyword_server):
+    # src/bin/server.rs + client.rs over 127.0.0.1: the handshake, framed queries
+    # from four connections at once, answered through one batching engine
+    import threading
+    from inspire_amd import client as cli
+    from inspire_amd import server as srvmod
+    srv, pp, plan, recs = keyword_server
+    picks = _usable_records(keyword_server, 8, 5150)
+    say("tcp server with", len(picks), "ids over 4 connections")
+    tcp = srvmod.KeywordTcpServer(srv, pp, listen="127.0.0.1:0", max_batch=8)
+    addr = tcp.start()
+    results, errors = {}, []
+
+    def worker(part):
+        try:
+            for idx, rid, pk in part:
+                results[idx] = (cli.fetch(addr, rid, hamming=O.HAMMING_WEIGHT)[0], pk)
+        except Exception as e:  # surfaced below
+            errors.append(repr(e))
+
+    try:
+        threads = [threading.Thread(target=worker, args=(picks[i::4],)) for i in range(4)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join(timeout=120)
+        assert not errors, errors
+        assert len(results) == len(picks) and all(got == pk for got, pk in results.values())
+        assert cli.fetch(addr, bytes(32))[0] is None
+        assert tcp.answered == len(picks) + 1
+        assert tcp.engine.stats()["queries"] == len(picks) + 1
+    finally:
+        tcp.stop()
