@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "../../include/inspire_hip.h"
+#include "host_tables.hpp"
 #include "runtime.hpp"
 
 using namespace inspire;
@@ -161,44 +162,14 @@ void seed_from_u64(uint64_t state, uint8_t key[32]) {
   }
 }
 
-inline uint32_t rotl32(uint32_t v, int r) { return (v << r) | (v >> (32 - r)); }
-
 // RFC 8439 section 2.3 block function with a 64-bit counter in words 12-13
-// and words 14-15 zero.
+// and words 14-15 zero (the library's one ChaCha20 core, host_tables.cpp),
+// serialized little-endian.
 void chacha20_block(const uint8_t key[32], uint64_t counter, uint8_t out[64]) {
-  uint32_t s[16];
-  s[0] = 0x61707865u;
-  s[1] = 0x3320646eu;
-  s[2] = 0x79622d32u;
-  s[3] = 0x6b206574u;
-  for (int i = 0; i < 8; ++i)
-    s[4 + i] = (uint32_t)key[4 * i] | (uint32_t)key[4 * i + 1] << 8 | (uint32_t)key[4 * i + 2] << 16 |
-               (uint32_t)key[4 * i + 3] << 24;
-  s[12] = (uint32_t)counter;
-  s[13] = (uint32_t)(counter >> 32);
-  s[14] = s[15] = 0;
-  uint32_t x[16];
-  std::memcpy(x, s, sizeof(x));
-  auto qr = [&](int a, int b, int c, int d) {
-    x[a] += x[b]; x[d] = rotl32(x[d] ^ x[a], 16);
-    x[c] += x[d]; x[b] = rotl32(x[b] ^ x[c], 12);
-    x[a] += x[b]; x[d] = rotl32(x[d] ^ x[a], 8);
-    x[c] += x[d]; x[b] = rotl32(x[b] ^ x[c], 7);
-  };
-  for (int r = 0; r < 10; ++r) {
-    qr(0, 4, 8, 12);
-    qr(1, 5, 9, 13);
-    qr(2, 6, 10, 14);
-    qr(3, 7, 11, 15);
-    qr(0, 5, 10, 15);
-    qr(1, 6, 11, 12);
-    qr(2, 7, 8, 13);
-    qr(3, 4, 9, 14);
-  }
-  for (int i = 0; i < 16; ++i) {
-    const uint32_t v = x[i] + s[i];
-    for (int b = 0; b < 4; ++b) out[4 * i + b] = (uint8_t)(v >> (8 * b));
-  }
+  uint32_t w[16];
+  chacha20_block_words(key, counter, w);
+  for (int i = 0; i < 16; ++i)
+    for (int b = 0; b < 4; ++b) out[4 * i + b] = (uint8_t)(w[i] >> (8 * b));
 }
 
 // ---------------------------------------------------------------------------
