@@ -8,7 +8,8 @@ each checked chain by chain against the oracle (slices of the same state):
 - 32M ML-KEM-768 keys at dim0 65536 on a lean server (db_rows 65536, 352
   outputs, DB 94.5 GB: the last chain's columns lie 86 GB into it);
 - the shape of the reference's published run, 5M ML-KEM-768 keys with
-  8,388,608 buckets at dim0 65536 (64 outputs, 2 chains), both packing kernels.
+  8,388,608 buckets at dim0 65536 (64 outputs, 2 chains), both packing kernels,
+  and a query from the library's KeywordClient at that shape.
 """
 import sys
 
@@ -138,4 +139,19 @@ def test_published_5m_shape_match_oracle():
         for w in range(plan["c"]):
             for h in range(2):
                 assert chain_of(resp[h], w) == exp[w][h], (name, w, h)
+    # the library's KeywordClient at this shape: a query row of 32 RLWE
+    # encryptions per hash query, answered chain by chain as the oracle answers it
+    say("5M keys: library client query (ternary secret) against the oracle")
+    pp = dict(num_items=plan["num_items"], item_size_bits=plan["item_size_bits"], dim0=plan["dim0"],
+              cuckoo_seed=O.CUCKOO_SEED, num_hashes=2, key_size=KEY, value_size=1184, entry_size=KEY + 1184,
+              num_records=5_000_000)
+    client = IA.KeywordClient(pp, hamming=O.HAMMING_WEIGHT, rng_seed=55)
+    cq = client.query(bytes(range(KEY)))
+    assert len(cq.data) == client.query_bytes() == 1_175_553   # 1 + (12,288 + 2 x (65,536 + 12,288)) x 7
+    cy, cz, crows, cbodies = O.deserialize_query(cq.data, rows)
+    resp = split_response(srv.respond_serialized(cq.data))
+    for w in range(plan["c"]):
+        exp = chain_blobs(5, rows, d, w, cy, cz, crows, cbodies)
+        for h in range(2):
+            assert chain_of(resp[h], w) == exp[h], ("client", w, h)
     _free(torch, srv)
