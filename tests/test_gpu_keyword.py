@@ -185,7 +185,9 @@ def test_library_client_extracts_stored_public_key(keyword_server, hamming):
         assert q.positions == O.cuckoo_positions(rid, pp["num_items"])
         got = client.extract_from_serialized_response(rid, q, srv.respond_serialized(q.data))
         assert got == pk and O.public_key_id(got) == rid, idx
-    absent = bytes(32)
+    # an id stored nowhere (not the all-zero id: empty buckets hold zero
+    # entries, so that one matches them, in the reference as here)
+    absent = bytes(range(1, 33))
     q = client.query(absent)
     assert client.extract(absent, q, srv.respond_serialized(q.data)) is None
 
@@ -218,8 +220,68 @@ def test_tcp_server_answers_concurrent_clients(keyword_server):
             t.join(timeout=120)
         assert not errors, errors
         assert len(results) == len(picks) and all(got == pk for got, pk in results.values())
-        assert cli.fetch(addr, bytes(32))[0] is None
+        assert cli.fetch(addr, bytes(range(1, 33)))[0] is None
         assert tcp.answered == len(picks) + 1
         assert tcp.engine.stats()["queries"] == len(picks) + 1
     finally:
         tcp.stop()
+
+
+def test_server_and_client_binaries(tmp_path):
+    # python -m inspire_amd.server / inspire_amd.client as processes, as the
+    # reference's bin/server.rs and bin/client.rs are run (tests/e2e_tcp_mlkem.rs)
+    import os
+    import re
+    import subprocess
+    import time
+    path = str(tmp_path / "keys.pqkeyv2")
+    IA.generate_dataset(path, "ml-kem-768", 3000, seed=11)
+    recs = IA.generate_records("ml-kem-768", 3000, seed=11)
+    env = dict(os.environ)
+    env["PYTHONPATH"] = os.pathsep.join([IA.__path__[0] + "/..", env.get("PYTHONPATH", "")])
+    say("server process on 127.0.0.1:0")
+    proc = subprocess.Popen([sys.executable, "-u", "-m", "inspire_amd.server", "--dataset", path,
+                             "--listen", "127.0.0.1:0"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                            text=True, env=env)
+    try:
+        addr, lines, t0 = None, [], time.time()
+        while addr is None and time.time() - t0 < 240:
+            line = proc.stdout.readline()
+            if not line:
+                break
+            lines.append(line)
+            m = re.search(r"serving 3000 ml-kem-768 public keys on (\S+)", line)
+            if m:
+                addr = m.group(1)
+        assert addr, "".join(lines)
+        assert any("startup GPU memory check: require" in x for x in lines)
+        checker = O.KeywordClient({"num_items": 8192, "item_size_bits": 9728, "dim0": 2048,
+                                   "cuckoo_seed": O.CUCKOO_SEED, "key_size": 32, "value_size": 1184,
+                                   "entry_size": 1216}, IA.plan_shape(3000, 32, 1184), O.client_secret(1, 0))
+        done = 0
+        for idx in range(len(recs)):
+            rid, pk = recs[idx]
+            if not _usable(checker, rid) or (np.frombuffer(rid + pk, dtype=">u2") == 0xFFFF).any():
+                continue
+            say("client process for record", idx)
+            out = subprocess.run([sys.executable, "-m", "inspire_amd.client", "--server", addr, "--id", rid.hex()],
+                                 capture_output=True, text=True, env=env, timeout=120)
+            assert out.returncode == 0, out.stderr
+            assert ("public key: %s" % pk.hex()) in out.stdout and ("query id: %s" % rid.hex()) in out.stdout
+            done += 1
+            if done == 2:
+                break
+        out = subprocess.run([sys.executable, "-m", "inspire_amd.client", "--server", addr, "--id",
+                              bytes(range(1, 33)).hex()], capture_output=True, text=True, env=env, timeout=120)
+        assert out.returncode == 0 and "not found" in out.stdout, out.stdout + out.stderr
+        # the all-zero id matches an empty bucket's zero entry, as in the reference
+        out = subprocess.run([sys.executable, "-m", "inspire_amd.client", "--server", addr, "--id", "00" * 32],
+                             capture_output=True, text=True, env=env, timeout=120)
+        assert out.returncode == 0 and ("public key: " + "00" * 1184) in out.stdout, out.stdout + out.stderr
+    finally:
+        proc.terminate()
+        try:
+            proc.wait(timeout=30)
+        except subprocess.TimeoutExpired:
+            proc.kill()
+            proc.wait()
