@@ -520,6 +520,12 @@ int launch_gemv_mfma_batch(uint64_t* c, int64_t c_client_stride, const uint64_t*
                            void* scratch, hipStream_t s, int variant) {
   if (nq < 1 || nq > 2 || count < 1 || db_rows <= 0 || db_rows % 256 != 0 || db_rows > (1 << 17) || db_cols <= 0)
     return -1;
+  // One query: the single-query kernel (8 column tiles per wave, 8-step load
+  // ring) streams the DB faster than a 16-query group that is 15/16 padding.
+  // The layouts agree: rows [nq][db_rows], output [nq][db_cols], and the
+  // scratch of one client is the single-query scratch.
+  if (count == 1 && db_rows % kMfmaChunkRows == 0)
+    return launch_gemv_mfma(c, rows, nq, db, dbfrag, db_rows, db_cols, scratch, s);
   int8_t* afrag = static_cast<int8_t*>(scratch);
   int32_t* rsum = reinterpret_cast<int32_t*>(afrag + (size_t)count * (db_rows / 64) * 64 * 16);
   hipLaunchKernelGGL(k_zero_u32, dim3((count * 16 + 255) / 256), dim3(256), 0, s, reinterpret_cast<uint32_t*>(rsum),
