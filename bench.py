@@ -592,8 +592,11 @@ def main():
         mk = lambda ops, ms, kern: {"bound": "mfma", "kernel": kern, "achieved": ops / (ms / 1e3) / 1e12,
                                     "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
                                     "frac": ops / (ms / 1e3) / 1e12 / INT8_PEAK_TOPS, "ops": ops, "ms": ms}
-        result["batched_roofline"] = {"B": B, "packing": mk(pack_ops, st_b["packing"], "k_packing_wide"),
-                                      "gemv": mk(gemv_ops, st_b["gemv"], "k_gemv_wide")}
+        tune = result.get("autotune") or {}
+        pv, gv = tune.get("packing_variant", 0), tune.get("gemv_variant", 0)
+        gemv_kernel = "k_gemv_lds (LDS-shared fragments)" if gv == 0 else "k_gemv_wide (query fragments from L2)"
+        result["batched_roofline"] = {"B": B, "packing": mk(pack_ops, st_b["packing"], "k_packing_wide variant %d" % pv),
+                                      "gemv": mk(gemv_ops, st_b["gemv"], gemv_kernel)}
         log("batched MFMA at B=%d: packing %.0f TOP/s (%.1f%%), gemv %.0f TOP/s (%.1f%%)" % (
             B, result["batched_roofline"]["packing"]["achieved"], 100 * result["batched_roofline"]["packing"]["frac"],
             result["batched_roofline"]["gemv"]["achieved"], 100 * result["batched_roofline"]["gemv"]["frac"]))
