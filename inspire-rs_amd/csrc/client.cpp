@@ -34,6 +34,7 @@
 // path serves unchanged.
 #include <sys/random.h>
 
+#include <cerrno>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -285,8 +286,8 @@ extern "C" int inspire_client_new(const inspire_public_params_t* pp, int32_t ham
                                   inspire_client_t** out) {
   if (!pp || !out) return set_error(INSPIRE_ERR_INVALID, "null argument");
   *out = nullptr;
-  if (pp->key_size == 0) return set_error(INSPIRE_ERR_INVALID, "server keyword size must be non-zero");
-  if (pp->value_size == 0) return set_error(INSPIRE_ERR_INVALID, "server value size must be non-zero");
+  if (pp->key_size <= 0) return set_error(INSPIRE_ERR_INVALID, "server keyword size must be non-zero");
+  if (pp->value_size <= 0) return set_error(INSPIRE_ERR_INVALID, "server value size must be non-zero");
   if (pp->entry_size < pp->key_size + pp->value_size)
     return set_error(INSPIRE_ERR_INVALID, "server entry size " + std::to_string(pp->entry_size) +
                                               " is smaller than key_size " + std::to_string(pp->key_size) +
@@ -316,6 +317,7 @@ extern "C" int inspire_client_new(const inspire_public_params_t* pp, int32_t ham
     size_t got = 0;
     while (got < sizeof(c->rng.key)) {
       const ssize_t n = getrandom(c->rng.key + got, sizeof(c->rng.key) - got, 0);
+      if (n < 0 && errno == EINTR) continue;
       if (n <= 0) {
         delete c;
         return set_error(INSPIRE_ERR_INVALID, "getrandom failed");
