@@ -277,24 +277,46 @@ struct PackArgs {
 // Epilogue of one 16x16 tile.  Lane (col c = lane%16, G = lane/16) holds
 // C'[4G + l][c], l = 0..3: client 4*mt + G, output 4*nt + c/4, output limb
 // m = c%4.  Corrects the signed byte limbs, recombines the limbs and stores
-// the rotation-term sum in the
-// z-major layout S[b][p][z][j], so a block's stores fill whole lines; the
-// z_body terms are added by the post-process (k_post_zmajor).
-__device__ inline void store_tile(const PackArgs& a, const v4i& acc, int p, int z, int mt, int64_t nt) {
+// the rotation-term sum in the z-major layout S[b][p][z][j], so a block's
+// stores fill whole lines; the z_body terms are added by the post-process
+// (k_post_zmajor).  X = sum_l C_lm 2^(7l) < 2^53 is reduced with 32-bit
+// operations only (reduce58), then multiplied by 2^(8m) < q as a Shoup
+// product; the lane-invariant constants live in TileStore.
+struct TileStore {
+  uint32_t q, mu, w, wp, corr;
+  __device__ TileStore(int p, int lane) {
+    const int m = lane & 3;
+    q = prime(p);
+    mu = mu58(p);
+    w = 1u << (8 * m);  // 2^(8m) < 2^24 < q
+    wp = (uint32_t)(((uint64_t)w << 32) / q);
+    corr = m < 3 ? 128u : 0u;
+  }
+};
+
+__device__ inline uint32_t reduce58q(uint64_t x, uint32_t q, uint32_t mu) {
+  const uint32_t t = mulhi32((uint32_t)(x >> 26), mu);
+  uint32_t r = (uint32_t)x - t * q;  // < 3q
+  r = r >= 2 * q ? r - 2 * q : r;
+  return r >= q ? r - q : r;
+}
+
+__device__ inline void store_tile(const PackArgs& a, const TileStore& ts, const v4i& acc, int p, int z, int mt,
+                                  int64_t nt) {
   const int lane = threadIdx.x & 63;
   const int c = lane & 15, G = lane >> 4;
   const int m = c & 3;
-  const uint32_t q = prime(p);
-  const uint32_t pw = (uint32_t)(((uint64_t)1 << (8 * m)) % q);  // 2^(8m) mod q
+  const uint32_t q = ts.q;
   const int b = 4 * mt + G;
   const u4 r = *reinterpret_cast<const u4*>(a.rsum + ((size_t)p * a.bpad + b) * 4);
-  const uint32_t corr = m < 3 ? 128u : 0u;
   const uint32_t rl[4] = {r.x, r.y, r.z, r.w};
-  uint64_t X = 0;
+  // Horner over the limbs: one 64-bit shift-add per limb
+  uint64_t X = (uint32_t)acc[3] + ts.corr * rl[3];
 #pragma unroll
-  for (int l = 0; l < 4; ++l) X += (uint64_t)((uint32_t)acc[l] + corr * rl[l]) << (7 * l);
-  uint32_t v = (uint32_t)barrett64(X, cr1(p), q);
-  v = p ? mulmod<1>(v, pw) : mulmod<0>(v, pw);
+  for (int l = 2; l >= 0; --l) X = (X << 7) + ((uint32_t)acc[l] + ts.corr * rl[l]);
+  const uint32_t v0 = reduce58q(X, q, ts.mu);
+  uint32_t v = shoup_lazy(v0, ts.w, ts.wp, q);  // [0, 2q)
+  v = v >= q ? v - q : v;
   uint32_t tot = v + __shfl_xor(v, 1, 64);
   tot = (tot >= q) ? tot - q : tot;
   uint32_t tot2 = tot + __shfl_xor(tot, 2, 64);
@@ -381,10 +403,11 @@ __global__ __launch_bounds__(256) void k_packing_mfma(PackArgs a) {
     for (int y = 0; y < NTW; ++y) bcur[y] = bnext[y];
   }
 
+  const TileStore ts(p, lane);
 #pragma unroll
   for (int x = 0; x < MT; ++x)
 #pragma unroll
-    for (int y = 0; y < NTW; ++y) store_tile(a, acc[x][y], p, z, mt0 + x, nt0 + wave * NTW + y);
+    for (int y = 0; y < NTW; ++y) store_tile(a, ts, acc[x][y], p, z, mt0 + x, nt0 + wave * NTW + y);
 }
 
 // ---------------------------------------------------------------------------
@@ -494,6 +517,9 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE, WPE
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       const int s = s0 + d;
+      // Nothing moves across step boundaries: hoisting a later step's unpack
+      // into this one would wait for B loads issued only a step ago.
+      __builtin_amdgcn_sched_barrier(0);
       // This wave's A copies of step s are done once at most kWait newer
       // vector-memory operations are outstanding; the barrier covers the other
       // waves' copies and their reads of the slot refilled below (step s-1's).
@@ -540,6 +566,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE, WPE
   // before the ring is reused.
   __builtin_amdgcn_s_waitcnt(0x3f70);  // vmcnt(0)
   v4i* park = reinterpret_cast<v4i*>(&aring[0][0][0]) + wave * kRound * 64 + lane;
+  const TileStore ts(p, lane);
 #pragma unroll
   for (int h = 0; h < MTW / kRoundX; ++h) {
     __syncthreads();
@@ -549,7 +576,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WPE, WPE
       for (int y = 0; y < NTW; ++y) park[(x * NTW + y) * 64] = acc[h * kRoundX + x][y];
     for (int t = 0; t < kRound; ++t) {
       const int x = h * kRoundX + t / NTW, y = t % NTW;
-      store_tile(a, park[t * 64], p, z, mg * MT + wm * MTW + x, nt0 + wn * NTW + y);
+      store_tile(a, ts, park[t * 64], p, z, mg * MT + wm * MTW + x, nt0 + wn * NTW + y);
     }
   }
 }
