@@ -397,6 +397,8 @@ def main():
     ap.add_argument("--no-setup", action="store_true", help="skip the offline setup measurement")
     ap.add_argument("--no-mixed", action="store_true", help="skip the two-database (4M + 250k) configuration")
     ap.add_argument("--no-32m", action="store_true", help="skip the 32M-key dim0-65536 configuration")
+    ap.add_argument("--rates-32m", default="100,300,600,700",
+                    help="Poisson arrival rates (q/s) of the 32M-key service runs ('' to skip)")
     ap.add_argument("--no-5m", action="store_true", help="skip the 5M-key shape of the reference's published run")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default batched kernels")
     ap.add_argument("--packing", choices=("mfma", "valu"), default="mfma",
@@ -659,7 +661,10 @@ def main():
         if flag:
             continue
         try:
-            result[key] = fn(IA, torch)
+            if fn is bench_32m:
+                result[key] = fn(IA, torch, rates=[float(x) for x in args.rates_32m.split(",") if x])
+            else:
+                result[key] = fn(IA, torch)
         except Exception as e:  # a configuration that does not fit is reported, not fatal
             result[key] = {"error": str(e)}
             log("%s failed: %s" % (key, e))
