@@ -42,6 +42,26 @@ constexpr int kColsPerBlock = 4 * kTilesPerWave * 16;
 // Row k of step s held by lane group g at byte t.
 __host__ __device__ inline int frag_row(int s, int g, int t) { return 64 * s + 8 * g + t + (t >= 8 ? 24 : 0); }
 
+// One column's dot product mod q_p from the lane's four limb rows of its lo-
+// and hi-plane tiles (p = the lane group's prime):
+//   X = sum_r 2^(7r) (256 S_hi,r + S_lo,r + 32896 R_r) = sum_k a_k v_k,
+// every limb term non-negative and X < K 2^44 <= 2^61 for K <= 2^17.  With
+// X = 2^32 X1 + X0 (X1 < 2^29), X1 (2^32 mod q) + X0 < 2^58 is one 32 x 32 +
+// 64 multiply-add, then reduce58 with the lane's prime: one chain of 32-bit
+// operations per lane.
+struct LanePrime {
+  uint32_t q, mu, c32;  // q_p, floor(2^58 / q_p), 2^32 mod q_p
+  __device__ explicit LanePrime(int p) : q(prime(p)), mu(mu58(p)), c32(p ? kC32Q1 : kC32Q0) {}
+};
+
+__device__ inline uint32_t gemv_residue(const v4i& lo, const v4i& hi, const int64_t (&rs)[4], const LanePrime& lp) {
+  int64_t v = 256 * (int64_t)hi[3] + lo[3] + rs[3];
+#pragma unroll
+  for (int r = 2; r >= 0; --r) v = (v << 7) + (256 * (int64_t)hi[r] + lo[r] + rs[r]);
+  const uint64_t x = (uint64_t)v;
+  return reduce58_rt((uint64_t)(uint32_t)(x >> 32) * lp.c32 + (uint32_t)x, lp.q, lp.mu);
+}
+
 // afrag: [db_rows/64][64 lanes][16 B]; rsum: [16] int32 (row i = 8h+4p+l).
 // Called by every thread of a 1-D grid (blockIdx.x, threadIdx.x).
 __device__ void k_gemv_prep_body(const uint64_t* __restrict__ rows, int nq, int db_rows, int8_t* __restrict__ afrag,
@@ -209,19 +229,16 @@ __global__ __launch_bounds__(256) void k_gemv_mfma(uint64_t* __restrict__ c, con
 
   // Lane (j, G) holds rows 4G..4G+3 = limbs 0..3 of (h, p) = (G >> 1, G & 1).
   const int h = g >> 1, p = g & 1;
+  const LanePrime lp(p);
   int64_t rs[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) rs[r] = (int64_t)rsum[4 * g + r] * 32896;
 #pragma unroll
   for (int tt = 0; tt < TPW; ++tt) {
-    int64_t v = 0;
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-      v += ((int64_t)256 * acc_hi[tt][r] + (int64_t)acc_lo[tt][r] + rs[r]) << (7 * r);
-    const uint64_t red = p ? barrett64((uint64_t)v, kCr1Q1, kQ1) : barrett64((uint64_t)v, kCr1Q0, kQ0);
-    const uint64_t other = __shfl_down(red, 16, 64);  // lane group 2h+1 -> 2h
+    const uint32_t red = gemv_residue(acc_lo[tt], acc_hi[tt], rs, lp);
+    const uint32_t other = __shfl_down(red, 16, 64);  // lane group 2h+1 -> 2h
     const int64_t col = col_base + tt * 16 + j;
-    if (p == 0 && h < nq && col < db_cols) c[(int64_t)h * db_cols + col] = crt_compose((uint32_t)red, (uint32_t)other);
+    if (p == 0 && h < nq && col < db_cols) c[(int64_t)h * db_cols + col] = crt_compose(red, other);
   }
 }
 
@@ -335,6 +352,7 @@ __global__ __launch_bounds__(256) void k_gemv_wide(uint64_t* __restrict__ c, int
   // Lane (j, G) holds rows 4G..4G+3 = limbs 0..3 of (h, p) = (G >> 1, G & 1).
   const int j = lane & 15, g = lane >> 4;
   const int h = g >> 1, pr = g & 1;
+  const LanePrime lp(pr);
 #pragma unroll
   for (int q = 0; q < CPW; ++q) {
     const int client = client0 + q;
@@ -344,14 +362,11 @@ __global__ __launch_bounds__(256) void k_gemv_wide(uint64_t* __restrict__ c, int
     for (int r = 0; r < 4; ++r) rs[r] = (int64_t)rsum[rc * 16 + 4 * g + r] * 32896;
 #pragma unroll
     for (int t = 0; t < TW; ++t) {
-      int64_t v = 0;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) v += ((int64_t)256 * acc[q][t][1][r] + (int64_t)acc[q][t][0][r] + rs[r]) << (7 * r);
-      const uint64_t red = pr ? barrett64((uint64_t)v, kCr1Q1, kQ1) : barrett64((uint64_t)v, kCr1Q0, kQ0);
-      const uint64_t other = __shfl_down(red, 16, 64);
+      const uint32_t red = gemv_residue(acc[q][t][0], acc[q][t][1], rs, lp);
+      const uint32_t other = __shfl_down(red, 16, 64);
       const int64_t col = col0 + 16 * t + j;
       if (pr == 0 && h < nq && col < db_cols && client < count)
-        c[(int64_t)client * c_client_stride + (int64_t)h * db_cols + col] = crt_compose((uint32_t)red, (uint32_t)other);
+        c[(int64_t)client * c_client_stride + (int64_t)h * db_cols + col] = crt_compose(red, other);
     }
   }
 }
@@ -475,6 +490,7 @@ __global__ __launch_bounds__(64 * WQ * WT) void k_gemv_lds(uint64_t* __restrict_
   // Lane (j, G) holds rows 4G..4G+3 = limbs 0..3 of (h, p) = (G >> 1, G & 1).
   const int j = lane & 15, g = lane >> 4;
   const int h = g >> 1, pr = g & 1;
+  const LanePrime lp(pr);
 #pragma unroll
   for (int q = 0; q < QW; ++q) {
     const int client = client0 + wq * QW + q;
@@ -484,14 +500,11 @@ __global__ __launch_bounds__(64 * WQ * WT) void k_gemv_lds(uint64_t* __restrict_
     for (int r = 0; r < 4; ++r) rs[r] = (int64_t)rsum[rc * 16 + 4 * g + r] * 32896;
 #pragma unroll
     for (int t = 0; t < TW; ++t) {
-      int64_t v = 0;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) v += ((int64_t)256 * acc[q][t][1][r] + (int64_t)acc[q][t][0][r] + rs[r]) << (7 * r);
-      const uint64_t red = pr ? barrett64((uint64_t)v, kCr1Q1, kQ1) : barrett64((uint64_t)v, kCr1Q0, kQ0);
-      const uint64_t other = __shfl_down(red, 16, 64);
+      const uint32_t red = gemv_residue(acc[q][t][0], acc[q][t][1], rs, lp);
+      const uint32_t other = __shfl_down(red, 16, 64);
       const int64_t col = (cb * TB + wt * TW + t) * 16 + j;
       if (pr == 0 && h < nq && col < db_cols && client < count)
-        c[(int64_t)client * c_client_stride + (int64_t)h * db_cols + col] = crt_compose((uint32_t)red, (uint32_t)other);
+        c[(int64_t)client * c_client_stride + (int64_t)h * db_cols + col] = crt_compose(red, other);
     }
   }
 }

@@ -33,6 +33,9 @@ constexpr uint64_t kCr1Q1 = 73916747789ull;
 // floor(2^58 / q_i) for the 32-bit Barrett below.
 constexpr uint32_t kMu58Q0 = (uint32_t)((1ull << 58) / kQ0);
 constexpr uint32_t kMu58Q1 = (uint32_t)((1ull << 58) / kQ1);
+// 2^32 mod q_i.
+constexpr uint32_t kC32Q0 = (uint32_t)((1ull << 32) % kQ0);
+constexpr uint32_t kC32Q1 = (uint32_t)((1ull << 32) % kQ1);
 // n^{-1} mod q_i for n = 2048.
 constexpr uint32_t kNInvQ0 = 268238881u;
 constexpr uint32_t kNInvQ1 = 249439233u;
@@ -77,6 +80,14 @@ __host__ __device__ inline uint32_t reduce58(uint64_t x) {
   if (r >= prime(c)) r -= prime(c);
   if (r >= prime(c)) r -= prime(c);
   return r;
+}
+
+// reduce58 with the prime as data: x mod q for x < 2^58, mu = floor(2^58/q).
+__host__ __device__ inline uint32_t reduce58_rt(uint64_t x, uint32_t q, uint32_t mu) {
+  const uint32_t t = mulhi32((uint32_t)(x >> 26), mu);
+  uint32_t r = (uint32_t)x - t * q;  // < 3q
+  r = r >= 2 * q ? r - 2 * q : r;
+  return r >= q ? r - q : r;
 }
 
 template <int c>

@@ -294,13 +294,6 @@ struct TileStore {
   }
 };
 
-__device__ inline uint32_t reduce58q(uint64_t x, uint32_t q, uint32_t mu) {
-  const uint32_t t = mulhi32((uint32_t)(x >> 26), mu);
-  uint32_t r = (uint32_t)x - t * q;  // < 3q
-  r = r >= 2 * q ? r - 2 * q : r;
-  return r >= q ? r - q : r;
-}
-
 __device__ inline void store_tile(const PackArgs& a, const TileStore& ts, const v4i& acc, int p, int z, int mt,
                                   int64_t nt) {
   const int lane = threadIdx.x & 63;
@@ -314,7 +307,7 @@ __device__ inline void store_tile(const PackArgs& a, const TileStore& ts, const 
   uint64_t X = (uint32_t)acc[3] + ts.corr * rl[3];
 #pragma unroll
   for (int l = 2; l >= 0; --l) X = (X << 7) + ((uint32_t)acc[l] + ts.corr * rl[l]);
-  const uint32_t v0 = reduce58q(X, q, ts.mu);
+  const uint32_t v0 = reduce58_rt(X, q, ts.mu);
   uint32_t v = shoup_lazy(v0, ts.w, ts.wp, q);  // [0, 2q)
   v = v >= q ? v - q : v;
   uint32_t tot = v + __shfl_xor(v, 1, 64);
