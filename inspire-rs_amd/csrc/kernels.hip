@@ -561,7 +561,7 @@ __device__ inline int xcd_chunk(int x, int n) {
   return k * q + min(k, r) + x / 8;
 }
 
-constexpr int kFoldThreads = 1024;
+constexpr int kFoldThreads = 1024;  // the two-prime NTT forms split the block into one half per prime
 constexpr int kFoldSlots = kN / kFoldThreads;  // NTT slots owned per thread
 
 struct FoldArgs {
@@ -644,12 +644,6 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
     }
   }
 
-  uint32_t* const dpolys[12] = {dig,             dig + kNPad,     dig + 2 * kNPad, dig + 3 * kNPad,
-                                dig + 4 * kNPad, dig + 5 * kNPad, dig + 6 * kNPad, dig + 7 * kNPad,
-                                dig + 8 * kNPad, dig + 9 * kNPad, dig + 10 * kNPad, dig + 11 * kNPad};
-  const int dprime[12] = {0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1};
-  uint32_t* const ipolys[4] = {prod, prod + kNPad, prod + 2 * kNPad, prod + 3 * kNPad};
-  const int iprime[4] = {0, 0, 1, 1};
   const uint64_t mask = (1ull << kGadgetBits) - 1;
 
   for (int i = d - 2; i >= 0; --i) {
@@ -668,7 +662,7 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
         }
     }
     __syncthreads();
-    ntt_forward_r4m<kFoldThreads, 12>(dpolys, dprime, tw);
+    ntt_forward_r4_halves<6>(dig, tw);
     // ct_gsw * G: row 0 = NTT(-a) * sum_k G_k; row 1 = sum_k body_k * G_k.
 #pragma unroll
     for (int s = 0; s < kFoldSlots; ++s) {
@@ -697,7 +691,7 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
       }
     }
     __syncthreads();
-    ntt_inverse_r4_unscaled<kFoldThreads, 4>(ipolys, iprime, args.fold);
+    ntt_inverse_r4_halves_unscaled<2>(prod, args.fold);
     const size_t j = (size_t)w * d + i;
 #pragma unroll
     for (int s = 0; s < kFoldSlots; ++s) {

@@ -382,6 +382,142 @@ __device__ inline void ntt_inverse_unscaled_impl(uint32_t* const (&polys)[NP], c
   }
 }
 
+// Two-prime forms for 1024-thread blocks over 2 NPH consecutive padded
+// polynomials at `base`: threads [512 c, 512 c + 512) transform polynomials
+// c NPH .. c NPH + NPH - 1, all over prime c, so every thread has a radix-4
+// unit in every pass (the per-prime forms above leave threads 512.. idle in
+// the radix-4 passes).  Same transforms, layouts and sync contract as
+// ntt_forward_r4m and ntt_inverse_r4_unscaled.
+template <int NPH>
+__device__ inline void ntt_forward_r4_halves(uint32_t* base, const NttTables& tw) {
+  int u = threadIdx.x & 511;
+  // opaque to the compiler: the unit's addresses are recomputed per call
+  // instead of being hoisted out of the caller's loop and spilled
+  asm volatile("" : "+v"(u));
+  const int c = threadIdx.x >> 9;
+  const uint32_t q = prime(c), q2 = 2 * q;
+  const uint32_t* fw = tw.fw(c);
+  const uint32_t* fwp = tw.fwp(c);
+  auto bf = [q2, q](uint32_t& x, uint32_t& y, uint32_t w, uint32_t wp) {
+    const uint32_t xr = x >= q2 ? x - q2 : x;
+    const uint32_t qn = shoup_lazy(y, w, wp, q);
+    x = xr + qn;
+    y = xr + q2 - qn;
+  };
+#pragma unroll 1
+  for (int mm = 0; mm + 1 < kLogN; mm += 2) {
+    const int m = 1 << mm, t = (kN / 2) >> mm, th = t >> 1;
+    int i, j;
+    if (th >= 32) {
+      i = u / th;
+      j = u % th;
+    } else {
+      i = u % m;
+      j = u / m;
+    }
+    const int x0 = 2 * t * i + j;
+    const int l0 = lds_index(x0), l1 = lds_index(x0 + th), l2 = lds_index(x0 + t), l3 = lds_index(x0 + t + th);
+    const uint32_t w0 = fw[m + i], wp0 = fwp[m + i];
+    const uint32_t w1 = fw[2 * m + 2 * i], wp1 = fwp[2 * m + 2 * i];
+    const uint32_t w2 = fw[2 * m + 2 * i + 1], wp2 = fwp[2 * m + 2 * i + 1];
+#pragma unroll
+    for (int p = 0; p < NPH; ++p) {
+      uint32_t* a = base + (c * NPH + p) * kNPad;
+      uint32_t a0 = a[l0], a1 = a[l1], a2 = a[l2], a3 = a[l3];
+      bf(a0, a2, w0, wp0);
+      bf(a1, a3, w0, wp0);
+      bf(a0, a1, w1, wp1);
+      bf(a2, a3, w2, wp2);
+      a[l0] = a0;
+      a[l1] = a1;
+      a[l2] = a2;
+      a[l3] = a3;
+    }
+    __syncthreads();
+  }
+  // last stage (t = 1), two units per thread, and the final reduction
+  for (int v = u; v < kN / 2; v += 512) {
+    const uint32_t w = fw[1024 + v], wp = fwp[1024 + v];
+    const int l0 = lds_index(2 * v), l1 = lds_index(2 * v + 1);
+#pragma unroll
+    for (int p = 0; p < NPH; ++p) {
+      uint32_t* a = base + (c * NPH + p) * kNPad;
+      uint32_t x = a[l0], y = a[l1];
+      bf(x, y, w, wp);
+      x = x >= q2 ? x - q2 : x;
+      x = x >= q ? x - q : x;
+      y = y >= q2 ? y - q2 : y;
+      y = y >= q ? y - q : y;
+      a[l0] = x;
+      a[l1] = y;
+    }
+  }
+  __syncthreads();
+}
+
+template <int NPH>
+__device__ inline void ntt_inverse_r4_halves_unscaled(uint32_t* base, const uint32_t* __restrict__ fold_tab) {
+  int u = threadIdx.x & 511;
+  asm volatile("" : "+v"(u));  // as in ntt_forward_r4_halves
+  const int c = threadIdx.x >> 9;
+  const uint32_t q = prime(c), q2 = 2 * q;
+  const uint32_t* iw = fold_tab + c * 3 * kN;
+  const uint32_t* iwp = iw + kN;
+  for (int v = u; v < kN / 2; v += 512) {
+    const int l0 = lds_index(2 * v), l1 = lds_index(2 * v + 1);
+#pragma unroll
+    for (int p = 0; p < NPH; ++p) {
+      uint32_t* a = base + (c * NPH + p) * kNPad;
+      uint32_t x = a[l0], y = a[l1];
+      inv_bf_unscaled(x, y, iw[1024 + v], iwp[1024 + v], q);
+      a[l0] = x;
+      a[l1] = y;
+    }
+  }
+  __syncthreads();
+#pragma unroll 1
+  for (int mm = kLogN - 2; mm >= 1; mm -= 2) {
+    const int h = 1 << mm, t = (kN / 2) >> mm, ni = (kN / 4) / t;
+    int i, j;
+    if (t >= 32) {
+      i = u / t;
+      j = u % t;
+    } else {
+      i = u % ni;
+      j = u / ni;
+    }
+    const int x0 = 4 * t * i + j;
+    const int l0 = lds_index(x0), l1 = lds_index(x0 + t), l2 = lds_index(x0 + 2 * t), l3 = lds_index(x0 + 3 * t);
+    const uint32_t wa = iw[h + 2 * i], wpa = iwp[h + 2 * i];
+    const uint32_t wb = iw[h + 2 * i + 1], wpb = iwp[h + 2 * i + 1];
+    const uint32_t wc = iw[(h >> 1) + i], wpc = iwp[(h >> 1) + i];
+#pragma unroll
+    for (int p = 0; p < NPH; ++p) {
+      uint32_t* a = base + (c * NPH + p) * kNPad;
+      uint32_t a0 = a[l0], a1 = a[l1], a2 = a[l2], a3 = a[l3];
+      inv_bf_unscaled(a0, a1, wa, wpa, q);
+      inv_bf_unscaled(a2, a3, wb, wpb, q);
+      inv_bf_unscaled(a0, a2, wc, wpc, q);
+      inv_bf_unscaled(a1, a3, wc, wpc, q);
+      if (mm == 1) {  // last pass: canonical
+        a0 = a0 >= q2 ? a0 - q2 : a0;
+        a1 = a1 >= q2 ? a1 - q2 : a1;
+        a2 = a2 >= q2 ? a2 - q2 : a2;
+        a3 = a3 >= q2 ? a3 - q2 : a3;
+        a0 = a0 >= q ? a0 - q : a0;
+        a1 = a1 >= q ? a1 - q : a1;
+        a2 = a2 >= q ? a2 - q : a2;
+        a3 = a3 >= q ? a3 - q : a3;
+      }
+      a[l0] = a0;
+      a[l1] = a1;
+      a[l2] = a2;
+      a[l3] = a3;
+    }
+    __syncthreads();
+  }
+}
+
 // NP polynomials over prime c.
 template <int NT, int NP>
 __device__ inline void ntt_inverse_r4_1p_unscaled(uint32_t* const (&polys)[NP], int c,
