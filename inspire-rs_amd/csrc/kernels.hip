@@ -453,6 +453,33 @@ void launch_post(uint64_t* P, const uint32_t* S, const uint32_t* tw, int count, 
   hipLaunchKernelGGL(k_post, dim3(count), dim3(1024), 0, s, P, S, tw);
 }
 
+// The fold's row-0 table: out[j][c][z] = n^-1 NTT_c(a_hat_j mod q_c) at NTT
+// slot z, one workgroup per instance j.
+__global__ __launch_bounds__(512) void k_ahat_ntt(uint32_t* __restrict__ out, const uint64_t* __restrict__ a_hat,
+                                                  const uint32_t* __restrict__ tw_tab) {
+  __shared__ uint32_t buf[2][kNPad];
+  const NttTables tw{tw_tab};
+  const size_t j = blockIdx.x;
+  for (int z = threadIdx.x; z < kN; z += 512) {
+    const uint64_t x = a_hat[j * kN + z];
+    buf[0][lds_index(z)] = (uint32_t)barrett64(x, cr1(0), kQ0);
+    buf[1][lds_index(z)] = (uint32_t)barrett64(x, cr1(1), kQ1);
+  }
+  __syncthreads();
+  uint32_t* const polys[2] = {buf[0], buf[1]};
+  const int pr[2] = {0, 1};
+  ntt_forward_r4m<512, 2>(polys, pr, tw);
+  for (int z = threadIdx.x; z < kN; z += 512) {
+    out[(j * 2 + 0) * kN + z] = mulmod<0>(buf[0][lds_index(z)], kNInvQ0);
+    out[(j * 2 + 1) * kN + z] = mulmod<1>(buf[1][lds_index(z)], kNInvQ1);
+  }
+}
+
+void launch_ahat_ntt(uint32_t* ahat_ntt, const uint64_t* a_hat, const uint32_t* tw, int64_t instances,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_ahat_ntt, dim3((unsigned)instances), dim3(512), 0, s, ahat_ntt, a_hat, tw);
+}
+
 // Post-process of the MFMA packing.  S holds the rotation-term sums z-major,
 // S[b][p][z][j]; this kernel adds the z_body terms
 //     sum_k z_body_b[k][z] * bold_t_hat[j][k][z]   (packing_online_kernel.cu:70-88)
@@ -572,6 +599,9 @@ struct FoldArgs {
   const uint32_t* bodies;    // [clients][nq][6][2][N]
   const uint32_t* fold;      // [2][3][N]: inverse powers, their w', n^-1 NTT(-a)
   const uint32_t* tw;        // NTT tables
+  // [instances][2][N]: n^-1 NTT_c(a_hat_j) per prime, or nullptr (fold row 0
+  // from a_hat in the coefficient domain)
+  const uint32_t* ahat_ntt;
   int64_t instances;
   int64_t out_stride;        // bytes between clients' outputs
   int d;
@@ -614,12 +644,45 @@ __device__ inline void fold_write_response(const FoldArgs& args, int b, int h, i
   }
 }
 
+// Row 0 of a Horner sum has a known transform.  With the table
+// Ah_j = n^-1 NTT_c(a_hat_j), the fold adds Ah_j to its row-0 product before
+// the inverse transform, so that product row S (n^-1 scaled) satisfies
+// NTT_c(sum row 0) = n S.  The unsigned base-2^19 digits of row 0 (sum < q <
+// 2^57, so g0 + 2^19 g1 + 2^38 g2 = sum exactly) then give
+//     NTT_c(g2) = 2^-38 (n S - NTT_c(g0) - 2^19 NTT_c(g1))
+//               = 2^-27 S - 2^-38 NTT_c(g0) - 2^-19 NTT_c(g1)   (n = 2^11),
+// one of the six digit transforms of a step per prime.  Inputs canonical;
+// returns the canonical value.
+__device__ inline uint32_t row0_third_digit(uint32_t S, uint32_t g0, uint32_t g1, int c) {
+  constexpr uint32_t kA0 = inv_pow2_mod(kQ0, 27), kA1 = inv_pow2_mod(kQ1, 27);
+  constexpr uint32_t kB0 = inv_pow2_mod(kQ0, 38), kB1 = inv_pow2_mod(kQ1, 38);
+  constexpr uint32_t kC0 = inv_pow2_mod(kQ0, 19), kC1 = inv_pow2_mod(kQ1, 19);
+  const uint32_t q = prime(c);
+  const uint32_t a = shoup_lazy(S, c ? kA1 : kA0, c ? shoup_companion(kA1, kQ1) : shoup_companion(kA0, kQ0), q);
+  const uint32_t b = shoup_lazy(g0, c ? kB1 : kB0, c ? shoup_companion(kB1, kQ1) : shoup_companion(kB0, kQ0), q);
+  const uint32_t e = shoup_lazy(g1, c ? kC1 : kC0, c ? shoup_companion(kC1, kQ1) : shoup_companion(kC0, kQ0), q);
+  uint32_t x = a + 4 * q - b - e;  // (0, 6q)
+  x = x >= 4 * q ? x - 4 * q : x;
+  x = x >= 2 * q ? x - 2 * q : x;
+  return x >= q ? x - q : x;
+}
+
+// LDS row of gadget digit row k (row r + 2 k' for digit k' of row r) among
+// the transformed ones: with the table, row 4 (row 0's third digit) is not
+// transformed and row 5 takes its place.
+template <bool kTab>
+__device__ constexpr int digit_row(int k) {
+  return kTab && k == 5 ? 4 : k;
+}
+
+template <bool kTab>
 __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
-  // LDS (padded polynomial layout of ntt_device.hpp): the 6 gadget-digit
-  // polynomials for both primes and the two product rows of both primes.
+  constexpr int kDig = kTab ? 5 : 6;  // transformed digit rows per prime
+  // LDS (padded polynomial layout of ntt_device.hpp): the transformed gadget
+  // digits for both primes and the two product rows of both primes.
   extern __shared__ uint32_t lds[];
-  uint32_t* dig = lds;                 // [2 (prime)][6][kNPad]
-  uint32_t* prod = lds + 12 * kNPad;   // [2 (prime)][2 (row)][kNPad]
+  uint32_t* dig = lds;                     // [2 (prime)][kDig][kNPad]
+  uint32_t* prod = lds + 2 * kDig * kNPad;  // [2 (prime)][2 (row)][kNPad]
   const NttTables tw{args.tw};
   const int per_client = args.nq * args.c;
   const int chain = xcd_chunk((int)blockIdx.x, (int)gridDim.x);
@@ -632,8 +695,10 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
   const uint64_t* Pc = args.P + (size_t)b * args.instances * kN;
   const uint32_t* body = args.bodies + ((size_t)b * args.nq + h) * 6 * 2 * kN;
 
-  // sum[r][s] for slots z = tid + kFoldThreads*s
+  // sum[r][s] for slots z = tid + kFoldThreads*s; with the table, S[c][s] is
+  // n^-1 NTT_c(sum row 0) at NTT slot z
   uint64_t sum[2][kFoldSlots];
+  uint32_t S[2][kFoldSlots];
   {
     const size_t j = (size_t)w * d + (d - 1);
 #pragma unroll
@@ -641,12 +706,17 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
       const int z = tid + s * kFoldThreads;
       sum[0][s] = args.a_hat[j * kN + z];
       sum[1][s] = reduce_small_q(cv[j * kN + z] + Pc[j * kN + z]);
+      if constexpr (kTab) {
+        S[0][s] = args.ahat_ntt[(j * 2 + 0) * kN + z];
+        S[1][s] = args.ahat_ntt[(j * 2 + 1) * kN + z];
+      }
     }
   }
 
   const uint64_t mask = (1ull << kGadgetBits) - 1;
 
   for (int i = d - 2; i >= 0; --i) {
+    const size_t j = (size_t)w * d + i;
     // G^-1(sum): digit k of row r is gadget row r + 2k (gadget.rs:35-61); the
     // digits (< 2^19) are canonical for both primes.
 #pragma unroll
@@ -656,13 +726,14 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
       for (int r = 0; r < 2; ++r)
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
+          if (kTab && r + 2 * k == 4) continue;
           const uint32_t dv = (uint32_t)((sum[r][s] >> (kGadgetBits * k)) & mask);
-          dig[(r + 2 * k) * kNPad + lz] = dv;
-          dig[(6 + r + 2 * k) * kNPad + lz] = dv;
+          dig[digit_row<kTab>(r + 2 * k) * kNPad + lz] = dv;
+          dig[(kDig + digit_row<kTab>(r + 2 * k)) * kNPad + lz] = dv;
         }
     }
     __syncthreads();
-    ntt_forward_r4_halves<6>(dig, tw);
+    ntt_forward_r4_halves<kDig>(dig, tw);
     // ct_gsw * G: row 0 = NTT(-a) * sum_k G_k; row 1 = sum_k body_k * G_k.
 #pragma unroll
     for (int s = 0; s < kFoldSlots; ++s) {
@@ -670,13 +741,17 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
 #pragma unroll
       for (int c = 0; c < 2; ++c) {
         const uint32_t q = prime(c);
+        uint32_t g[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+          if (!kTab || k != 4) g[k] = dig[(kDig * c + digit_row<kTab>(k)) * kNPad + lds_index(z)];
+        if constexpr (kTab) g[4] = row0_third_digit(S[c][s], g[0], g[2], c);
         uint32_t gsum = 0;
         uint64_t acc = 0;
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
-          const uint32_t gk = dig[(6 * c + k) * kNPad + lds_index(z)];
-          gsum += gk;
-          acc += (uint64_t)body[(k * 2 + c) * kN + z] * gk;
+          gsum += g[k];
+          acc += (uint64_t)body[(k * 2 + c) * kN + z] * g[k];
         }
         // both rows carry n^-1 (row 0 in the table, row 1 here): the inverse
         // transform below skips the per-stage halving
@@ -686,20 +761,25 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
         gs = gs >= q ? gs - q : gs;
         const uint32_t na = args.fold[(c * 3 + 2) * kN + z];
         const uint32_t r1 = (uint32_t)barrett64(acc, cr1(c), q);
-        prod[(c * 2 + 0) * kNPad + lds_index(z)] = c == 0 ? mulmod<0>(na, gs) : mulmod<1>(na, gs);
+        uint32_t p0 = c == 0 ? mulmod<0>(na, gs) : mulmod<1>(na, gs);
+        if constexpr (kTab) {
+          p0 += args.ahat_ntt[(j * 2 + c) * kN + z];
+          p0 = p0 >= q ? p0 - q : p0;
+          S[c][s] = p0;
+        }
+        prod[(c * 2 + 0) * kNPad + lds_index(z)] = p0;
         prod[(c * 2 + 1) * kNPad + lds_index(z)] = c == 0 ? mulmod<0>(r1, kNInvQ0) : mulmod<1>(r1, kNInvQ1);
       }
     }
     __syncthreads();
     ntt_inverse_r4_halves_unscaled<2>(prod, args.fold);
-    const size_t j = (size_t)w * d + i;
 #pragma unroll
     for (int s = 0; s < kFoldSlots; ++s) {
       const int z = tid + s * kFoldThreads;
       const int lz = lds_index(z);
       const uint64_t x0 = crt_compose(prod[0 * kNPad + lz], prod[2 * kNPad + lz]);
       const uint64_t x1 = crt_compose(prod[1 * kNPad + lz], prod[3 * kNPad + lz]);
-      sum[0][s] = reduce_small_q(x0 + args.a_hat[j * kN + z]);
+      sum[0][s] = kTab ? x0 : reduce_small_q(x0 + args.a_hat[j * kN + z]);
       sum[1][s] = reduce_small_q(x1 + cv[j * kN + z] + Pc[j * kN + z]);
     }
     // dig is rewritten next; prod only after two more barriers.
@@ -726,7 +806,9 @@ constexpr int kFold2Threads = 512;
 constexpr int kFoldSeqMinBlocks = 512;
 constexpr int kFold2Slots = kN / kFold2Threads;
 
+template <bool kTab>
 __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_fold_seq(FoldArgs args) {
+  constexpr int kDig = kTab ? 5 : 6;  // transformed digit rows per prime (see row0_third_digit)
   __shared__ uint32_t dig[6][kNPad];
   const NttTables tw{args.tw};
   const int per_client = args.nq * args.c;
@@ -740,7 +822,10 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
   const uint64_t* Pc = args.P + (size_t)b * args.instances * kN;
   const uint32_t* body = args.bodies + ((size_t)b * args.nq + h) * 6 * 2 * kN;
 
+  // With the table: Sc holds n^-1 NTT_c(sum row 0) of the prime being
+  // worked on, So the other prime's; they trade places after each prime.
   uint64_t sum[2][kFold2Slots];
+  uint32_t Sc[kFold2Slots], So[kFold2Slots];
   {
     const size_t j = (size_t)w * d + (d - 1);
 #pragma unroll
@@ -748,13 +833,20 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
       const int z = tid + s * kFold2Threads;
       sum[0][s] = args.a_hat[j * kN + z];
       sum[1][s] = reduce_small_q(cv[j * kN + z] + Pc[j * kN + z]);
+      if constexpr (kTab) {
+        Sc[s] = args.ahat_ntt[(j * 2 + 0) * kN + z];
+        So[s] = args.ahat_ntt[(j * 2 + 1) * kN + z];
+      }
     }
   }
-  uint32_t* const dpolys[6] = {dig[0], dig[1], dig[2], dig[3], dig[4], dig[5]};
+  uint32_t* dpolys[kDig];
+#pragma unroll
+  for (int p = 0; p < kDig; ++p) dpolys[p] = dig[p];
   uint32_t* const ipolys[2] = {dig[0], dig[1]};
   const uint64_t mask = (1ull << kGadgetBits) - 1;
 
   for (int i = d - 2; i >= 0; --i) {
+    const size_t j = (size_t)w * d + i;
     uint32_t res0[2][kFold2Slots];
 #pragma unroll 1  // one prime at a time keeps the NTT within 80 VGPRs (6 waves per SIMD)
     for (int c = 0; c < 2; ++c) {
@@ -766,22 +858,28 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
 #pragma unroll
         for (int r = 0; r < 2; ++r)
 #pragma unroll
-          for (int k = 0; k < 3; ++k) dig[r + 2 * k][lz] = (uint32_t)((sum[r][s] >> (kGadgetBits * k)) & mask);
+          for (int k = 0; k < 3; ++k)
+            if (!kTab || r + 2 * k != 4)
+              dig[digit_row<kTab>(r + 2 * k)][lz] = (uint32_t)((sum[r][s] >> (kGadgetBits * k)) & mask);
       }
       __syncthreads();
-      ntt_forward_r4_1p<kFold2Threads, 6>(dpolys, c, tw);
+      ntt_forward_r4_1p<kFold2Threads, kDig>(dpolys, c, tw);
       const uint32_t q = prime(c);
 #pragma unroll
       for (int s = 0; s < kFold2Slots; ++s) {
         const int z = tid + s * kFold2Threads;
         const int lz = lds_index(z);
+        uint32_t g[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+          if (!kTab || k != 4) g[k] = dig[digit_row<kTab>(k)][lz];
+        if constexpr (kTab) g[4] = row0_third_digit(Sc[s], g[0], g[2], c);
         uint32_t gsum = 0;
         uint64_t acc = 0;
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
-          const uint32_t gk = dig[k][lz];
-          gsum += gk;
-          acc += (uint64_t)body[(k * 2 + c) * kN + z] * gk;
+          gsum += g[k];
+          acc += (uint64_t)body[(k * 2 + c) * kN + z] * g[k];
         }
         // both rows carry n^-1 (see k_fold)
         // six canonical NTT values: gsum < 6q
@@ -790,8 +888,22 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
         gs = gs >= q ? gs - q : gs;
         const uint32_t na = args.fold[(c * 3 + 2) * kN + z];
         const uint32_t r1 = (uint32_t)barrett64(acc, cr1(c), q);
-        dig[0][lz] = c == 0 ? mulmod<0>(na, gs) : mulmod<1>(na, gs);
+        uint32_t p0 = c == 0 ? mulmod<0>(na, gs) : mulmod<1>(na, gs);
+        if constexpr (kTab) {
+          p0 += args.ahat_ntt[(j * 2 + c) * kN + z];
+          p0 = p0 >= q ? p0 - q : p0;
+          Sc[s] = p0;
+        }
+        dig[0][lz] = p0;
         dig[1][lz] = c == 0 ? mulmod<0>(r1, kNInvQ0) : mulmod<1>(r1, kNInvQ1);
+      }
+      if constexpr (kTab) {
+#pragma unroll
+        for (int s = 0; s < kFold2Slots; ++s) {
+          const uint32_t t = Sc[s];
+          Sc[s] = So[s];
+          So[s] = t;
+        }
       }
       __syncthreads();
       ntt_inverse_r4_1p_unscaled<kFold2Threads, 2>(ipolys, c, args.fold);
@@ -804,14 +916,13 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
         }
       }
     }
-    const size_t j = (size_t)w * d + i;
 #pragma unroll
     for (int s = 0; s < kFold2Slots; ++s) {
       const int z = tid + s * kFold2Threads;
       const int lz = lds_index(z);
       const uint64_t x0 = crt_compose(res0[0][s], dig[0][lz]);
       const uint64_t x1 = crt_compose(res0[1][s], dig[1][lz]);
-      sum[0][s] = reduce_small_q(x0 + args.a_hat[j * kN + z]);
+      sum[0][s] = kTab ? x0 : reduce_small_q(x0 + args.a_hat[j * kN + z]);
       sum[1][s] = reduce_small_q(x1 + cv[j * kN + z] + Pc[j * kN + z]);
     }
   }
@@ -827,34 +938,45 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
   fold_write_response<kFold2Threads>(args, b, h, w, dig[0], dig[1]);
 }
 
-size_t fold_lds_bytes() { return (size_t)(12 + 4) * kNPad * sizeof(uint32_t); }
+size_t fold_lds_bytes(bool table) { return (size_t)(2 * (table ? 5 : 6) + 4) * kNPad * sizeof(uint32_t); }
 
-int launch_fold(uint8_t* out, const uint64_t* a_hat, const uint64_t* cvec, const uint64_t* P,
-                const uint32_t* bodies, const uint32_t* fold_tab, const uint32_t* tw, int64_t instances, int d,
-                int nq, hipStream_t s) {
-  return launch_fold_batch(out, 0, a_hat, cvec, P, bodies, fold_tab, tw, instances, d, nq, 1, s, 0);
+int launch_fold(uint8_t* out, const uint64_t* a_hat, const uint32_t* ahat_ntt, const uint64_t* cvec,
+                const uint64_t* P, const uint32_t* bodies, const uint32_t* fold_tab, const uint32_t* tw,
+                int64_t instances, int d, int nq, hipStream_t s) {
+  return launch_fold_batch(out, 0, a_hat, ahat_ntt, cvec, P, bodies, fold_tab, tw, instances, d, nq, 1, s, 0);
 }
 
-int launch_fold_batch(uint8_t* out, int64_t out_stride, const uint64_t* a_hat, const uint64_t* cvec,
-                      const uint64_t* P, const uint32_t* bodies, const uint32_t* fold_tab, const uint32_t* tw,
-                      int64_t instances, int d, int nq, int clients, hipStream_t s, int variant) {
+int launch_fold_batch(uint8_t* out, int64_t out_stride, const uint64_t* a_hat, const uint32_t* ahat_ntt,
+                      const uint64_t* cvec, const uint64_t* P, const uint32_t* bodies, const uint32_t* fold_tab,
+                      const uint32_t* tw, int64_t instances, int d, int nq, int clients, hipStream_t s,
+                      int variant) {
   if (d <= 0 || instances % d != 0 || nq <= 0 || clients <= 0) return -1;
-  FoldArgs a{out, a_hat, cvec, P, bodies, fold_tab, tw, instances, out_stride, d, (int)(instances / d), nq};
+  FoldArgs a{out, a_hat, cvec, P, bodies, fold_tab, tw, ahat_ntt, instances, out_stride, d, (int)(instances / d), nq};
   static bool attr_set = false;
   if (!attr_set) {
-    // Raises the dynamic-LDS ceiling for this kernel (132 KiB of the 160 KiB per CU).
-    (void)hipFuncSetAttribute((const void*)k_fold, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)fold_lds_bytes());
+    // Raises the dynamic-LDS ceiling of the 1024-thread forms (118 and 132 KiB
+    // of the 160 KiB per CU).
+    (void)hipFuncSetAttribute((const void*)k_fold<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)fold_lds_bytes(true));
+    (void)hipFuncSetAttribute((const void*)k_fold<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)fold_lds_bytes(false));
     attr_set = true;
   }
   const int blocks = clients * nq * a.c;
+  const bool table = ahat_ntt != nullptr;
   // Below two workgroups per CU the 1024-thread form (both primes at once)
   // has the shorter chain latency; above, three k_fold_seq per CU win.
   // variant 1 / 2 force k_fold / k_fold_seq.
-  if (variant == 2 || (variant != 1 && blocks >= kFoldSeqMinBlocks))
-    hipLaunchKernelGGL(k_fold_seq, dim3(blocks), dim3(kFold2Threads), 0, s, a);
-  else
-    hipLaunchKernelGGL(k_fold, dim3(blocks), dim3(kFoldThreads), fold_lds_bytes(), s, a);
+  if (variant == 2 || (variant != 1 && blocks >= kFoldSeqMinBlocks)) {
+    if (table)
+      hipLaunchKernelGGL(k_fold_seq<true>, dim3(blocks), dim3(kFold2Threads), 0, s, a);
+    else
+      hipLaunchKernelGGL(k_fold_seq<false>, dim3(blocks), dim3(kFold2Threads), 0, s, a);
+  } else if (table) {
+    hipLaunchKernelGGL(k_fold<true>, dim3(blocks), dim3(kFoldThreads), fold_lds_bytes(true), s, a);
+  } else {
+    hipLaunchKernelGGL(k_fold<false>, dim3(blocks), dim3(kFoldThreads), fold_lds_bytes(false), s, a);
+  }
   return 0;
 }
 

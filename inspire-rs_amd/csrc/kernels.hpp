@@ -22,10 +22,16 @@ __global__ void k_zero_u32(uint32_t* p, int n);
 void launch_unpack_query_batch(const uint8_t* q, int64_t q_stride, int nq, int db_rows, int count, uint64_t* y_body,
                                uint64_t* z_body, uint64_t* rows, uint32_t* bodies, hipStream_t s);
 // Fold for `clients` keyword queries; client b's responses at out + b*out_stride.
-// fold_tab: DeviceConstants::fold.
-int launch_fold_batch(uint8_t* out, int64_t out_stride, const uint64_t* a_hat, const uint64_t* cvec,
-                      const uint64_t* P, const uint32_t* bodies, const uint32_t* fold_tab, const uint32_t* tw,
-                      int64_t instances, int d, int nq, int clients, hipStream_t s, int variant = 0);
+// fold_tab: DeviceConstants::fold.  ahat_ntt: launch_ahat_ntt's table of
+// a_hat, or nullptr (the fold then transforms all six gadget digits per step).
+int launch_fold_batch(uint8_t* out, int64_t out_stride, const uint64_t* a_hat, const uint32_t* ahat_ntt,
+                      const uint64_t* cvec, const uint64_t* P, const uint32_t* bodies, const uint32_t* fold_tab,
+                      const uint32_t* tw, int64_t instances, int d, int nq, int clients, hipStream_t s,
+                      int variant = 0);
+// ahat_ntt[j][c][z] = n^-1 NTT_c(a_hat_j mod q_c) at NTT slot z, for the
+// fold's row-0 shortcut ([instances][2][N] u32).
+void launch_ahat_ntt(uint32_t* ahat_ntt, const uint64_t* a_hat, const uint32_t* tw, int64_t instances,
+                     hipStream_t s);
 
 // DB encode (interpolation of each row's column groups, p = 65535): in and out
 // column-major [db_cols][db_rows] (out may equal in); work holds
@@ -109,10 +115,10 @@ size_t setup_offline_bytes_per_instance();
 int launch_setup_offline(const uint16_t* db, int64_t db_rows, int64_t instances, uint64_t* bt, uint64_t* btb,
                          uint64_t* bth, uint64_t* a_hat, const uint32_t* tw, int64_t group, hipStream_t s);
 
-size_t fold_lds_bytes();
+size_t fold_lds_bytes(bool table);
 // out: [nq][instances/d][12288] response blobs.
-int launch_fold(uint8_t* out, const uint64_t* a_hat, const uint64_t* cvec, const uint64_t* P,
-                const uint32_t* bodies, const uint32_t* fold_tab, const uint32_t* tw, int64_t instances, int d,
-                int nq, hipStream_t s);
+int launch_fold(uint8_t* out, const uint64_t* a_hat, const uint32_t* ahat_ntt, const uint64_t* cvec,
+                const uint64_t* P, const uint32_t* bodies, const uint32_t* fold_tab, const uint32_t* tw,
+                int64_t instances, int d, int nq, hipStream_t s);
 
 }  // namespace inspire

@@ -102,6 +102,16 @@ __host__ __device__ inline uint32_t shoup_lazy(uint32_t y, uint32_t w, uint32_t 
   return w * y - qt * q;
 }
 
+// 2^-k mod q (q odd) and the Shoup companion floor(w*2^32/q), at compile time.
+__host__ __device__ constexpr uint32_t inv_pow2_mod(uint32_t q, int k) {
+  uint64_t r = 1;
+  for (int i = 0; i < k; ++i) r = r * ((q + 1) / 2) % q;
+  return (uint32_t)r;
+}
+__host__ __device__ constexpr uint32_t shoup_companion(uint32_t w, uint32_t q) {
+  return (uint32_t)(((uint64_t)w << 32) / q);
+}
+
 // CRT compose (core/params.rs:203-215): the unique x < q with x = a mod q0 and
 // x = b mod q1, by Garner's formula x = a + q0 * ((b - a) * q0^{-1} mod q1).
 __host__ __device__ inline uint64_t crt_compose(uint32_t a, uint32_t b) {

@@ -284,6 +284,7 @@ struct inspire_server {
   uint16_t* db = nullptr;
   void* dbfrag = nullptr;  // fragment-ordered DB for the MFMA GEMVs (nullptr: column-major kernels)
   uint64_t *bt = nullptr, *btb = nullptr, *bth = nullptr, *a_hat = nullptr;
+  uint32_t* ahat_ntt = nullptr;  // the fold's row-0 table of a_hat (launch_ahat_ntt)
   bool have_db = false, have_precomp = false;
   // Lean servers keep only the fragment-ordered copy of bold_t (MFMA packing
   // only); the condensed copy exists only while a setter fills the tiles.
@@ -469,6 +470,7 @@ static int server_create(const inspire_shape_t* shape, bool lean, inspire_server
   if (!rc && !lean) rc = dev_alloc(&s->btb, (size_t)inst * kNInner * kN, "bold_t_bar");
   if (!rc) rc = dev_alloc(&s->bth, (size_t)inst * kTExp * kN, "bold_t_hat");
   if (!rc) rc = dev_alloc(&s->a_hat, (size_t)inst * kN, "a_hat");
+  if (!rc) rc = dev_alloc(&s->ahat_ntt, (size_t)inst * 2 * kN, "NTT of a_hat");
   if (!rc) rc = alloc_slots(s, kDefaultSlots);
   // Optional, last: the fragment-ordered DB copy (the GEMVs read the
   // column-major DB when it does not fit).
@@ -550,6 +552,7 @@ extern "C" int inspire_server_swap_state(inspire_server_t* live, inspire_server_
   std::swap(live->btb, staged->btb);
   std::swap(live->bth, staged->bth);
   std::swap(live->a_hat, staged->a_hat);
+  std::swap(live->ahat_ntt, staged->ahat_ntt);
   std::swap(live->tiled, staged->tiled);
   std::swap(live->tiled_valid, staged->tiled_valid);
   std::swap(live->lean, staged->lean);
@@ -573,7 +576,7 @@ extern "C" void inspire_server_destroy(inspire_server_t* s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   free_slots(s);
   free_batch(s);
-  void* ptrs[] = {s->db, s->dbfrag, s->bt, s->btb, s->bth, s->a_hat, s->tiled};
+  void* ptrs[] = {s->db, s->dbfrag, s->bt, s->btb, s->bth, s->a_hat, s->ahat_ntt, s->tiled};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -667,7 +670,7 @@ extern "C" int inspire_server_autotune(inspire_server_t* s, int count, int nq, i
                                  (int)rows, s->db_cols, n, b.gemv_scratch, st, v);
   };
   auto fold = [&](int n, int v) {
-    (void)launch_fold_batch(d_out, (int64_t)blob, s->a_hat, b.cvec, b.P, b.bodies, s->consts->fold,
+    (void)launch_fold_batch(d_out, (int64_t)blob, s->a_hat, s->ahat_ntt, b.cvec, b.P, b.bodies, s->consts->fold,
                             s->consts->tw, inst, s->shape.interpolate_degree, nq, n, st, v);
   };
   // Average milliseconds of `reps` runs after one warm-up; -1 on a HIP error.
@@ -862,8 +865,8 @@ static int launch_batch(inspire_server_t* s, const uint8_t* d_queries, size_t q_
                              (int)rows, s->db_cols, count, b.gemv_scratch, st, s->gemv_variant))
     return set_error(INSPIRE_ERR_INVALID, "unsupported GEMV shape");
   mark(4);
-  if (launch_fold_batch(d_out, (int64_t)out_stride, s->a_hat, b.cvec, b.P, b.bodies, s->consts->fold, s->consts->tw,
-                        inst, s->shape.interpolate_degree, nq, count, st, s->fold_variant))
+  if (launch_fold_batch(d_out, (int64_t)out_stride, s->a_hat, s->ahat_ntt, b.cvec, b.P, b.bodies, s->consts->fold,
+                        s->consts->tw, inst, s->shape.interpolate_degree, nq, count, st, s->fold_variant))
     return set_error(INSPIRE_ERR_INVALID, "unsupported fold shape");
   mark(5);
   HIP_TRY(hipGetLastError(), "batch launch");
@@ -921,6 +924,10 @@ extern "C" int inspire_server_set_precomp(inspire_server_t* s, const uint64_t* b
     if (e == hipSuccess) e = hipMemcpyAsync(s->btb, btb, inst * kNInner * kN * 8, k, s->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(s->bth, bth, inst * kTExp * kN * 8, k, s->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(s->a_hat, a_hat, inst * kN * 8, k, s->stream);
+    if (e == hipSuccess) {
+      launch_ahat_ntt(s->ahat_ntt, s->a_hat, s->consts->tw, s->shape.instances, s->stream);
+      e = hipGetLastError();
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     s->have_precomp = e == hipSuccess;
     return release_condensed(s, e == hipSuccess ? INSPIRE_OK : hip_error(e, "copy precomp"));
@@ -929,6 +936,8 @@ extern "C" int inspire_server_set_precomp(inspire_server_t* s, const uint64_t* b
   HIP_TRY(hipMemcpyAsync(s->btb, btb, inst * kNInner * kN * 8, k, s->stream), "copy bold_t_bar");
   HIP_TRY(hipMemcpyAsync(s->bth, bth, inst * kTExp * kN * 8, k, s->stream), "copy bold_t_hat");
   HIP_TRY(hipMemcpyAsync(s->a_hat, a_hat, inst * kN * 8, k, s->stream), "copy a_hat");
+  launch_ahat_ntt(s->ahat_ntt, s->a_hat, s->consts->tw, s->shape.instances, s->stream);
+  HIP_TRY(hipGetLastError(), "NTT of a_hat");
   HIP_TRY(hipStreamSynchronize(s->stream), "copy precomp");
   s->have_precomp = true;
   s->tiled_valid = false;
@@ -971,6 +980,12 @@ extern "C" int inspire_server_setup_offline(inspire_server_t* s) {
   if (rc) return rc;
   rc = launch_setup_offline(s->db, s->shape.db_rows, s->shape.instances, s->bt, s->btb, s->bth, s->a_hat,
                             s->consts->tw, group, s->stream);
+  if (!rc) {
+    launch_ahat_ntt(s->ahat_ntt, s->a_hat, s->consts->tw, s->shape.instances, s->stream);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) rc = hip_error(e, "NTT of a_hat");
+  }
   if (s->lean) {
     s->have_precomp = rc == INSPIRE_OK;
     return release_condensed(s, rc);
@@ -1062,6 +1077,7 @@ extern "C" int inspire_server_fill_synthetic(inspire_server_t* s, uint64_t seed)
   }
   launch_synth_fill(s->bth, seed, kStreamBth, 0, inst * kTExp * kN, kSynthCondensed, s->stream);
   launch_synth_fill(s->a_hat, seed, kStreamAhat, 0, inst * kN, kSynthModQ, s->stream);
+  launch_ahat_ntt(s->ahat_ntt, s->a_hat, s->consts->tw, s->shape.instances, s->stream);
   if (s->dbfrag) launch_tile_db(s->dbfrag, s->db, (int)s->shape.db_rows, s->db_cols, s->stream);
   HIP_TRY(hipGetLastError(), "synthetic fill launch");
   HIP_TRY(hipStreamSynchronize(s->stream), "synthetic fill");
@@ -1125,7 +1141,7 @@ int run_pipeline(inspire_server_t* s, Slot& sl, const uint8_t* d_query, int nq, 
     launch_post(sl.P, sl.S, s->consts->tw, (int)inst, st);
   }
   mark(4);
-  if (launch_fold(d_out, s->a_hat, sl.cvec, sl.P, sl.bodies, s->consts->fold, s->consts->tw, inst,
+  if (launch_fold(d_out, s->a_hat, s->ahat_ntt, sl.cvec, sl.P, sl.bodies, s->consts->fold, s->consts->tw, inst,
                   s->shape.interpolate_degree, nq, st))
     return set_error(INSPIRE_ERR_INVALID, "unsupported fold shape");
   mark(5);
@@ -1428,7 +1444,8 @@ extern "C" int inspire_fold_device(uint8_t* d_out, const uint64_t* d_a_hat, cons
     return set_error(INSPIRE_ERR_INVALID, "bad fold argument");
   const DeviceConstants* dc = device_constants();
   if (!dc) return set_error(INSPIRE_ERR_HIP, "device constants");
-  if (launch_fold(d_out, d_a_hat, d_c, d_P, d_bodies, dc->fold, dc->tw, instances, d, nq, (hipStream_t)stream))
+  if (launch_fold(d_out, d_a_hat, nullptr, d_c, d_P, d_bodies, dc->fold, dc->tw, instances, d, nq,
+                  (hipStream_t)stream))
     return set_error(INSPIRE_ERR_INVALID, "instances must be a positive multiple of d");
   HIP_TRY(hipGetLastError(), "fold launch");
   return INSPIRE_OK;
