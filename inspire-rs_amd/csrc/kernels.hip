@@ -434,13 +434,26 @@ void launch_rotation_legacy(uint64_t* y_all, uint64_t* y_bar_all, const uint64_t
 
 // ---------------------------------------------------------------------------
 // Post-process (packing_online.rs:359-398): P_j = CRT(INTT(S_j)) mod q.
+//
+// The fold of a server adds P_j to its row-1 product before that row's
+// inverse transform, so the server's post-process writes n^-1 NTT_c(P_j)
+// per prime instead, two u32 rows [2][N] in the 8N bytes of P_j's row
+// (ntt_out); the fold transforms the first P of each chain back itself.
 // ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(1024) void k_post(uint64_t* __restrict__ P, const uint32_t* __restrict__ S,
-                                               const uint32_t* __restrict__ tw_tab) {
+                                               const uint32_t* __restrict__ tw_tab, int ntt_out) {
   __shared__ uint32_t buf[2][kNPad];
   const NttTables tw{tw_tab};
   const int j = blockIdx.x;
+  if (ntt_out) {
+    uint32_t* P32 = reinterpret_cast<uint32_t*>(P) + (size_t)j * 2 * kN;
+    for (int t = threadIdx.x; t < 2 * kN; t += 1024) {
+      const uint32_t v = S[(size_t)j * 2 * kN + t];
+      P32[t] = t < kN ? mulmod<0>(v, kNInvQ0) : mulmod<1>(v, kNInvQ1);
+    }
+    return;
+  }
   for (int t = threadIdx.x; t < 2 * kN; t += 1024) buf[t / kN][lds_index(t % kN)] = S[(size_t)j * 2 * kN + t];
   __syncthreads();
   uint32_t* const polys[2] = {buf[0], buf[1]};
@@ -449,8 +462,8 @@ __global__ __launch_bounds__(1024) void k_post(uint64_t* __restrict__ P, const u
   for (int z = threadIdx.x; z < kN; z += 1024) P[(size_t)j * kN + z] = crt_compose(buf[0][lds_index(z)], buf[1][lds_index(z)]);
 }
 
-void launch_post(uint64_t* P, const uint32_t* S, const uint32_t* tw, int count, hipStream_t s) {
-  hipLaunchKernelGGL(k_post, dim3(count), dim3(1024), 0, s, P, S, tw);
+void launch_post(uint64_t* P, const uint32_t* S, const uint32_t* tw, int count, hipStream_t s, int ntt_out) {
+  hipLaunchKernelGGL(k_post, dim3(count), dim3(1024), 0, s, P, S, tw, ntt_out);
 }
 
 // The fold's row-0 table: out[j][c][z] = n^-1 NTT_c(a_hat_j mod q_c) at NTT
@@ -480,92 +493,70 @@ void launch_ahat_ntt(uint32_t* ahat_ntt, const uint64_t* a_hat, const uint32_t* 
   hipLaunchKernelGGL(k_ahat_ntt, dim3((unsigned)instances), dim3(512), 0, s, ahat_ntt, a_hat, tw);
 }
 
-// Post-process of the MFMA packing.  S holds the rotation-term sums z-major,
-// S[b][p][z][j]; this kernel adds the z_body terms
+// Post-process of the MFMA packing for the fold of a server.  S holds the
+// rotation-term sums z-major, S[b][c][z][j]; this kernel adds the z_body terms
 //     sum_k z_body_b[k][z] * bold_t_hat[j][k][z]   (packing_online_kernel.cu:70-88)
-// while staging kPostJ consecutive outputs j of one client into LDS, then runs
-// the inverse NTT and the CRT composition of all of them.  One block per
-// (group of kPostJ outputs, client); P[b][j][z].
-constexpr int kPostJ = 4;
-constexpr int kPostThreads = 512;
-constexpr int kPostTJ = 8;  // output groups per L2 tile: 32 outputs = one 128-byte line of S
-constexpr int kPostTC = 4;  // clients per L2 tile
+// and writes n^-1 NTT_c(P_j), P[b][j][c][z] (u32; see k_post).  One block per
+// tile of 32 outputs j x 64 slots z of one client: S comes in along j (128-byte
+// rows) through a padded LDS tile and leaves along z.  Block order: each XCD
+// (block id mod 8) takes a contiguous range of (j tile, client, z tile), so the
+// clients of a j tile meet its bold_t_hat rows in that XCD's L2.
+constexpr int kPnJ = 32;
+constexpr int kPnZ = 64;
+constexpr int kPnThreads = 256;
 
-__global__ __launch_bounds__(kPostThreads) void k_post_zmajor(uint64_t* __restrict__ P,
-                                                             const uint32_t* __restrict__ S,
-                                                             const uint64_t* __restrict__ z_body,
-                                                             int64_t key_client_stride,
-                                                             const uint64_t* __restrict__ bth,
-                                                             const uint32_t* __restrict__ fold_tab, int64_t instances,
-                                                             int clients, int tc) {
-  __shared__ uint32_t buf[2 * kPostJ][kNPad];
-  // Block order: each XCD (block id mod 8) takes a contiguous range of tiles
-  // of kPostTJ output groups x tc clients.  The kPostTJ groups of a tile read
-  // the same 128-byte lines of S (kPostTJ * kPostJ = 32 outputs), and its tc
-  // clients read the same bold_t_hat rows, so both meet in that XCD's L2.
-  const int64_t groups = (instances + kPostJ - 1) / kPostJ;
-  const int64_t jt_n = (groups + kPostTJ - 1) / kPostTJ, ct_n = (clients + tc - 1) / tc;
-  const int64_t total = jt_n * ct_n * kPostTJ * tc;
+__global__ __launch_bounds__(kPnThreads) void k_post_ntt(uint32_t* __restrict__ P, const uint32_t* __restrict__ S,
+                                                         const uint64_t* __restrict__ z_body,
+                                                         int64_t key_client_stride,
+                                                         const uint64_t* __restrict__ bth, int64_t instances,
+                                                         int clients) {
+  __shared__ uint32_t tile[2][kPnZ][kPnJ + 1];
+  constexpr int zt_n = kN / kPnZ;
+  const int64_t jt_n = (instances + kPnJ - 1) / kPnJ;
+  const int64_t total = jt_n * clients * zt_n;
   const int64_t per_xcd = (total + 7) / 8;
   const int64_t r = (int64_t)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
   if (r >= total) return;
-  const int64_t tile = r / (kPostTJ * tc), o = r % (kPostTJ * tc);
-  const int64_t g = (tile % jt_n) * kPostTJ + o % kPostTJ;
-  const int64_t b = (tile / jt_n) * tc + o / kPostTJ;
-  if (g >= groups || b >= clients) return;
-  const int64_t j0 = g * kPostJ;
-  const int nj = (int)(instances - j0 < kPostJ ? instances - j0 : kPostJ);
+  const int zt = (int)(r % zt_n);
+  const int b = (int)(r / zt_n % clients);
+  const int64_t j0 = r / zt_n / clients * kPnJ;
+  const int z0 = zt * kPnZ;
+  const int nj = (int)(instances - j0 < kPnJ ? instances - j0 : kPnJ);
   const uint32_t* Sb = S + (size_t)b * 2 * kN * instances;
-  const uint64_t* zb = z_body + b * key_client_stride;
-  for (int t = threadIdx.x; t < kN * kPostJ; t += kPostThreads) {
-    const int jj = t % kPostJ, z = t / kPostJ;
-    uint32_t v0 = 0, v1 = 0;
-    if (jj < nj) {
-      const int64_t j = j0 + jj;
-      const uint64_t* th = bth + (size_t)j * kTExp * kN + z;
-      uint64_t h0 = 0, h1 = 0;
-#pragma unroll
-      for (int k = 0; k < kTExp; ++k) {
-        const uint64_t zv = zb[k * kN + z], tv = th[k * kN];
-        h0 += (uint64_t)(uint32_t)zv * (uint32_t)tv;
-        h1 += (zv >> 32) * (tv >> 32);
-      }
-      v0 = (uint32_t)barrett64(h0, cr1(0), prime(0)) + Sb[(size_t)z * instances + j];
-      v1 = (uint32_t)barrett64(h1, cr1(1), prime(1)) + Sb[((size_t)kN + z) * instances + j];
-      v0 = v0 >= prime(0) ? v0 - prime(0) : v0;
-      v1 = v1 >= prime(1) ? v1 - prime(1) : v1;
-      // n^-1 here; the inverse transform below skips the per-stage halving
-      v0 = mulmod<0>(v0, kNInvQ0);
-      v1 = mulmod<1>(v1, kNInvQ1);
-    }
-    buf[jj][lds_index(z)] = v0;
-    buf[kPostJ + jj][lds_index(z)] = v1;
+  for (int e = threadIdx.x; e < 2 * kPnZ * kPnJ; e += kPnThreads) {
+    const int jj = e % kPnJ, zz = e / kPnJ % kPnZ, c = e / (kPnJ * kPnZ);
+    tile[c][zz][jj] = jj < nj ? Sb[((size_t)c * kN + z0 + zz) * instances + j0 + jj] : 0u;
   }
   __syncthreads();
-  uint32_t* polys[2 * kPostJ];
-  int pr[2 * kPostJ];
+  const int zz = threadIdx.x % kPnZ, z = z0 + zz;
+  const uint64_t* zb = z_body + b * key_client_stride + z;
+  const uint64_t zv[3] = {zb[0], zb[kN], zb[2 * kN]};
+  for (int jj = threadIdx.x / kPnZ; jj < nj; jj += kPnThreads / kPnZ) {
+    const int64_t j = j0 + jj;
+    const uint64_t* th = bth + (size_t)j * kTExp * kN + z;
+    uint64_t h0 = 0, h1 = 0;
 #pragma unroll
-  for (int i = 0; i < 2 * kPostJ; ++i) {
-    polys[i] = buf[i];
-    pr[i] = i / kPostJ;
-  }
-  ntt_inverse_r4_unscaled<kPostThreads, 2 * kPostJ>(polys, pr, fold_tab);
-  for (int t = threadIdx.x; t < kN * kPostJ; t += kPostThreads) {
-    const int jj = t / kN, z = t % kN;
-    if (jj < nj)
-      P[((size_t)b * instances + j0 + jj) * kN + z] =
-          crt_compose(buf[jj][lds_index(z)], buf[kPostJ + jj][lds_index(z)]);
+    for (int k = 0; k < kTExp; ++k) {
+      const uint64_t tv = th[k * kN];
+      h0 += (uint64_t)(uint32_t)zv[k] * (uint32_t)tv;
+      h1 += (zv[k] >> 32) * (tv >> 32);
+    }
+    uint32_t v0 = (uint32_t)barrett64(h0, cr1(0), kQ0) + tile[0][zz][jj];
+    uint32_t v1 = (uint32_t)barrett64(h1, cr1(1), kQ1) + tile[1][zz][jj];
+    v0 = v0 >= kQ0 ? v0 - kQ0 : v0;
+    v1 = v1 >= kQ1 ? v1 - kQ1 : v1;
+    uint32_t* out = P + ((size_t)b * instances + j) * 2 * kN + z;
+    out[0] = mulmod<0>(v0, kNInvQ0);
+    out[kN] = mulmod<1>(v1, kNInvQ1);
   }
 }
 
-void launch_post_zmajor(uint64_t* P, const uint32_t* S, const uint64_t* z_body, int64_t key_client_stride,
-                        const uint64_t* bth, const uint32_t* fold_tab, int clients, int64_t instances, hipStream_t s) {
-  const int tc = clients < kPostTC ? clients : kPostTC;
-  const int64_t groups = (instances + kPostJ - 1) / kPostJ;
-  const int64_t total = (groups + kPostTJ - 1) / kPostTJ * kPostTJ * ((clients + tc - 1) / tc) * tc;
+void launch_post_ntt(uint64_t* P, const uint32_t* S, const uint64_t* z_body, int64_t key_client_stride,
+                     const uint64_t* bth, int clients, int64_t instances, hipStream_t s) {
+  const int64_t total = (instances + kPnJ - 1) / kPnJ * clients * (kN / kPnZ);
   const int64_t blocks = (total + 7) / 8 * 8;
-  hipLaunchKernelGGL(k_post_zmajor, dim3((unsigned)blocks), dim3(kPostThreads), 0, s, P, S, z_body,
-                     key_client_stride, bth, fold_tab, instances, clients, tc);
+  hipLaunchKernelGGL(k_post_ntt, dim3((unsigned)blocks), dim3(kPnThreads), 0, s, reinterpret_cast<uint32_t*>(P), S,
+                     z_body, key_client_stride, bth, instances, clients);
 }
 
 // ---------------------------------------------------------------------------
@@ -576,6 +567,12 @@ void launch_post_zmajor(uint64_t* P, const uint32_t* S, const uint64_t* z_body, 
 // where ct_gsw row 0 is NTT(-a) in every column and row 1 is the client's
 // RGSW body.  The first step has sum = 0, so it reduces to sum = packed[w*d+d-1].
 // packed[j] = (a_hat_j, c_h[j*n..] + P_j).
+//
+// Two forms, the same bytes.  The server form (kNttAdd) takes a_hat_j and P_j
+// in the NTT domain, n^-1 scaled (launch_ahat_ntt's table; launch_post_ntt or
+// launch_post with ntt_out), adds them to the product rows before their
+// inverse transforms and transforms the first P of its chain back at the
+// start; the stateless form adds a_hat_j and P_j to the rows after them.
 // ---------------------------------------------------------------------------
 
 // Workgroup x runs logical chain xcd_chunk(x, n) of n: XCD k (block id mod 8)
@@ -668,16 +665,16 @@ __device__ inline uint32_t row0_third_digit(uint32_t S, uint32_t g0, uint32_t g1
 }
 
 // LDS row of gadget digit row k (row r + 2 k' for digit k' of row r) among
-// the transformed ones: with the table, row 4 (row 0's third digit) is not
-// transformed and row 5 takes its place.
-template <bool kTab>
+// the transformed ones: in the server form, row 4 (row 0's third digit) is
+// not transformed and row 5 takes its place.
+template <bool kNttAdd>
 __device__ constexpr int digit_row(int k) {
-  return kTab && k == 5 ? 4 : k;
+  return kNttAdd && k == 5 ? 4 : k;
 }
 
-template <bool kTab>
+template <bool kNttAdd>
 __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
-  constexpr int kDig = kTab ? 5 : 6;  // transformed digit rows per prime
+  constexpr int kDig = kNttAdd ? 5 : 6;  // transformed digit rows per prime
   // LDS (padded polynomial layout of ntt_device.hpp): the transformed gadget
   // digits for both primes and the two product rows of both primes.
   extern __shared__ uint32_t lds[];
@@ -693,9 +690,10 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
   const int d = args.d;
   const uint64_t* cv = args.cvec + ((size_t)b * args.nq + h) * args.instances * kN;
   const uint64_t* Pc = args.P + (size_t)b * args.instances * kN;
+  const uint32_t* P32 = reinterpret_cast<const uint32_t*>(Pc);  // server form: n^-1 NTT_c(P_j) rows
   const uint32_t* body = args.bodies + ((size_t)b * args.nq + h) * 6 * 2 * kN;
 
-  // sum[r][s] for slots z = tid + kFoldThreads*s; with the table, S[c][s] is
+  // sum[r][s] for slots z = tid + kFoldThreads*s; in the server form, S[c][s] is
   // n^-1 NTT_c(sum row 0) at NTT slot z
   uint64_t sum[2][kFoldSlots];
   uint32_t S[2][kFoldSlots];
@@ -705,10 +703,22 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
     for (int s = 0; s < kFoldSlots; ++s) {
       const int z = tid + s * kFoldThreads;
       sum[0][s] = args.a_hat[j * kN + z];
-      sum[1][s] = reduce_small_q(cv[j * kN + z] + Pc[j * kN + z]);
-      if constexpr (kTab) {
+      sum[1][s] = kNttAdd ? cv[j * kN + z] : reduce_small_q(cv[j * kN + z] + Pc[j * kN + z]);
+      if constexpr (kNttAdd) {
         S[0][s] = args.ahat_ntt[(j * 2 + 0) * kN + z];
         S[1][s] = args.ahat_ntt[(j * 2 + 1) * kN + z];
+        prod[0 * kNPad + lds_index(z)] = P32[(j * 2 + 0) * kN + z];
+        prod[1 * kNPad + lds_index(z)] = P32[(j * 2 + 1) * kN + z];
+      }
+    }
+    if constexpr (kNttAdd) {
+      // the chain's first P arrives in the NTT domain too: transform it back
+      __syncthreads();
+      ntt_inverse_r4_halves_unscaled<1>(prod, args.fold);
+#pragma unroll
+      for (int s = 0; s < kFoldSlots; ++s) {
+        const int lz = lds_index(tid + s * kFoldThreads);
+        sum[1][s] = reduce_small_q(sum[1][s] + crt_compose(prod[lz], prod[kNPad + lz]));
       }
     }
   }
@@ -726,10 +736,10 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
       for (int r = 0; r < 2; ++r)
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-          if (kTab && r + 2 * k == 4) continue;
+          if (kNttAdd && r + 2 * k == 4) continue;
           const uint32_t dv = (uint32_t)((sum[r][s] >> (kGadgetBits * k)) & mask);
-          dig[digit_row<kTab>(r + 2 * k) * kNPad + lz] = dv;
-          dig[(kDig + digit_row<kTab>(r + 2 * k)) * kNPad + lz] = dv;
+          dig[digit_row<kNttAdd>(r + 2 * k) * kNPad + lz] = dv;
+          dig[(kDig + digit_row<kNttAdd>(r + 2 * k)) * kNPad + lz] = dv;
         }
     }
     __syncthreads();
@@ -744,8 +754,8 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
         uint32_t g[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k)
-          if (!kTab || k != 4) g[k] = dig[(kDig * c + digit_row<kTab>(k)) * kNPad + lds_index(z)];
-        if constexpr (kTab) g[4] = row0_third_digit(S[c][s], g[0], g[2], c);
+          if (!kNttAdd || k != 4) g[k] = dig[(kDig * c + digit_row<kNttAdd>(k)) * kNPad + lds_index(z)];
+        if constexpr (kNttAdd) g[4] = row0_third_digit(S[c][s], g[0], g[2], c);
         uint32_t gsum = 0;
         uint64_t acc = 0;
 #pragma unroll
@@ -762,13 +772,16 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
         const uint32_t na = args.fold[(c * 3 + 2) * kN + z];
         const uint32_t r1 = (uint32_t)barrett64(acc, cr1(c), q);
         uint32_t p0 = c == 0 ? mulmod<0>(na, gs) : mulmod<1>(na, gs);
-        if constexpr (kTab) {
+        uint32_t p1 = c == 0 ? mulmod<0>(r1, kNInvQ0) : mulmod<1>(r1, kNInvQ1);
+        if constexpr (kNttAdd) {
           p0 += args.ahat_ntt[(j * 2 + c) * kN + z];
           p0 = p0 >= q ? p0 - q : p0;
           S[c][s] = p0;
+          p1 += P32[(j * 2 + c) * kN + z];
+          p1 = p1 >= q ? p1 - q : p1;
         }
         prod[(c * 2 + 0) * kNPad + lds_index(z)] = p0;
-        prod[(c * 2 + 1) * kNPad + lds_index(z)] = c == 0 ? mulmod<0>(r1, kNInvQ0) : mulmod<1>(r1, kNInvQ1);
+        prod[(c * 2 + 1) * kNPad + lds_index(z)] = p1;
       }
     }
     __syncthreads();
@@ -779,8 +792,8 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
       const int lz = lds_index(z);
       const uint64_t x0 = crt_compose(prod[0 * kNPad + lz], prod[2 * kNPad + lz]);
       const uint64_t x1 = crt_compose(prod[1 * kNPad + lz], prod[3 * kNPad + lz]);
-      sum[0][s] = kTab ? x0 : reduce_small_q(x0 + args.a_hat[j * kN + z]);
-      sum[1][s] = reduce_small_q(x1 + cv[j * kN + z] + Pc[j * kN + z]);
+      sum[0][s] = kNttAdd ? x0 : reduce_small_q(x0 + args.a_hat[j * kN + z]);
+      sum[1][s] = reduce_small_q(x1 + cv[j * kN + z] + (kNttAdd ? 0 : Pc[j * kN + z]));
     }
     // dig is rewritten next; prod only after two more barriers.
   }
@@ -806,9 +819,9 @@ constexpr int kFold2Threads = 512;
 constexpr int kFoldSeqMinBlocks = 512;
 constexpr int kFold2Slots = kN / kFold2Threads;
 
-template <bool kTab>
+template <bool kNttAdd>
 __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_fold_seq(FoldArgs args) {
-  constexpr int kDig = kTab ? 5 : 6;  // transformed digit rows per prime (see row0_third_digit)
+  constexpr int kDig = kNttAdd ? 5 : 6;  // transformed digit rows per prime (see row0_third_digit)
   __shared__ uint32_t dig[6][kNPad];
   const NttTables tw{args.tw};
   const int per_client = args.nq * args.c;
@@ -820,22 +833,43 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
   const int d = args.d;
   const uint64_t* cv = args.cvec + ((size_t)b * args.nq + h) * args.instances * kN;
   const uint64_t* Pc = args.P + (size_t)b * args.instances * kN;
+  const uint32_t* P32 = reinterpret_cast<const uint32_t*>(Pc);  // server form: n^-1 NTT_c(P_j) rows
   const uint32_t* body = args.bodies + ((size_t)b * args.nq + h) * 6 * 2 * kN;
 
-  // With the table: Sc holds n^-1 NTT_c(sum row 0) of the prime being
-  // worked on, So the other prime's; they trade places after each prime.
+  // Server form: n^-1 NTT_c(sum row 0) at the thread's slots, prime 0's in
+  // S0 and prime 1's in LDS row 5, which this form does not transform (each
+  // thread reads and writes only its own slots there).
   uint64_t sum[2][kFold2Slots];
-  uint32_t Sc[kFold2Slots], So[kFold2Slots];
+  uint32_t S0[kFold2Slots];
   {
     const size_t j = (size_t)w * d + (d - 1);
 #pragma unroll
     for (int s = 0; s < kFold2Slots; ++s) {
       const int z = tid + s * kFold2Threads;
       sum[0][s] = args.a_hat[j * kN + z];
-      sum[1][s] = reduce_small_q(cv[j * kN + z] + Pc[j * kN + z]);
-      if constexpr (kTab) {
-        Sc[s] = args.ahat_ntt[(j * 2 + 0) * kN + z];
-        So[s] = args.ahat_ntt[(j * 2 + 1) * kN + z];
+      sum[1][s] = kNttAdd ? cv[j * kN + z] : reduce_small_q(cv[j * kN + z] + Pc[j * kN + z]);
+      if constexpr (kNttAdd) {
+        S0[s] = args.ahat_ntt[(j * 2 + 0) * kN + z];
+        dig[5][lds_index(z)] = args.ahat_ntt[(j * 2 + 1) * kN + z];
+      }
+    }
+    if constexpr (kNttAdd) {
+      // the chain's first P arrives in the NTT domain too: transform it back
+      // (rows 0 and 1, one per prime)
+#pragma unroll
+      for (int s = 0; s < kFold2Slots; ++s) {
+        const int z = tid + s * kFold2Threads;
+        dig[0][lds_index(z)] = P32[(j * 2 + 0) * kN + z];
+        dig[1][lds_index(z)] = P32[(j * 2 + 1) * kN + z];
+      }
+      __syncthreads();
+      uint32_t* const ppolys[2] = {dig[0], dig[1]};
+      const int pprime[2] = {0, 1};
+      ntt_inverse_r4_unscaled<kFold2Threads, 2>(ppolys, pprime, args.fold);
+#pragma unroll
+      for (int s = 0; s < kFold2Slots; ++s) {
+        const int lz = lds_index(tid + s * kFold2Threads);
+        sum[1][s] = reduce_small_q(sum[1][s] + crt_compose(dig[0][lz], dig[1][lz]));
       }
     }
   }
@@ -859,8 +893,8 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
         for (int r = 0; r < 2; ++r)
 #pragma unroll
           for (int k = 0; k < 3; ++k)
-            if (!kTab || r + 2 * k != 4)
-              dig[digit_row<kTab>(r + 2 * k)][lz] = (uint32_t)((sum[r][s] >> (kGadgetBits * k)) & mask);
+            if (!kNttAdd || r + 2 * k != 4)
+              dig[digit_row<kNttAdd>(r + 2 * k)][lz] = (uint32_t)((sum[r][s] >> (kGadgetBits * k)) & mask);
       }
       __syncthreads();
       ntt_forward_r4_1p<kFold2Threads, kDig>(dpolys, c, tw);
@@ -872,8 +906,8 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
         uint32_t g[6];
 #pragma unroll
         for (int k = 0; k < 6; ++k)
-          if (!kTab || k != 4) g[k] = dig[digit_row<kTab>(k)][lz];
-        if constexpr (kTab) g[4] = row0_third_digit(Sc[s], g[0], g[2], c);
+          if (!kNttAdd || k != 4) g[k] = dig[digit_row<kNttAdd>(k)][lz];
+        if constexpr (kNttAdd) g[4] = row0_third_digit(c == 0 ? S0[s] : dig[5][lz], g[0], g[2], c);
         uint32_t gsum = 0;
         uint64_t acc = 0;
 #pragma unroll
@@ -889,21 +923,19 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
         const uint32_t na = args.fold[(c * 3 + 2) * kN + z];
         const uint32_t r1 = (uint32_t)barrett64(acc, cr1(c), q);
         uint32_t p0 = c == 0 ? mulmod<0>(na, gs) : mulmod<1>(na, gs);
-        if constexpr (kTab) {
+        uint32_t p1 = c == 0 ? mulmod<0>(r1, kNInvQ0) : mulmod<1>(r1, kNInvQ1);
+        if constexpr (kNttAdd) {
           p0 += args.ahat_ntt[(j * 2 + c) * kN + z];
           p0 = p0 >= q ? p0 - q : p0;
-          Sc[s] = p0;
+          if (c == 0)
+            S0[s] = p0;
+          else
+            dig[5][lz] = p0;
+          p1 += P32[(j * 2 + c) * kN + z];
+          p1 = p1 >= q ? p1 - q : p1;
         }
         dig[0][lz] = p0;
-        dig[1][lz] = c == 0 ? mulmod<0>(r1, kNInvQ0) : mulmod<1>(r1, kNInvQ1);
-      }
-      if constexpr (kTab) {
-#pragma unroll
-        for (int s = 0; s < kFold2Slots; ++s) {
-          const uint32_t t = Sc[s];
-          Sc[s] = So[s];
-          So[s] = t;
-        }
+        dig[1][lz] = p1;
       }
       __syncthreads();
       ntt_inverse_r4_1p_unscaled<kFold2Threads, 2>(ipolys, c, args.fold);
@@ -922,8 +954,8 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
       const int lz = lds_index(z);
       const uint64_t x0 = crt_compose(res0[0][s], dig[0][lz]);
       const uint64_t x1 = crt_compose(res0[1][s], dig[1][lz]);
-      sum[0][s] = kTab ? x0 : reduce_small_q(x0 + args.a_hat[j * kN + z]);
-      sum[1][s] = reduce_small_q(x1 + cv[j * kN + z] + Pc[j * kN + z]);
+      sum[0][s] = kNttAdd ? x0 : reduce_small_q(x0 + args.a_hat[j * kN + z]);
+      sum[1][s] = reduce_small_q(x1 + cv[j * kN + z] + (kNttAdd ? 0 : Pc[j * kN + z]));
     }
   }
 

@@ -76,7 +76,7 @@ int batch_key_pad(int count);
 size_t packing_afrag_bytes(int count);
 // Rotation-term sums of the packing, z-major: S[client][2][N][instances];
 // y_body: [count][3][N] (key_client_stride elements apart); afrag: scratch of
-// packing_afrag_bytes(count) bytes.  k_post_zmajor adds the z_body terms.
+// packing_afrag_bytes(count) bytes.  k_post_ntt adds the z_body terms.
 int launch_packing_mfma(uint32_t* S, int64_t s_client_stride, const uint8_t* tiled, uint8_t* afrag,
                         const uint64_t* y_body, int64_t key_client_stride, int count, int64_t instances,
                         hipStream_t s, int variant = 0);
@@ -98,13 +98,16 @@ void launch_rotation_legacy(uint64_t* y_all, uint64_t* y_bar_all, const uint64_t
                             const uint32_t* gen_pows, int num_rotations, int t_exp, int poly_len,
                             hipStream_t s);
 
-// P: [count][N] canonical mod q from S [count][2][N].
-void launch_post(uint64_t* P, const uint32_t* S, const uint32_t* tw, int count, hipStream_t s);
-// P: [clients][instances][N] from the z-major rotation-term sums S
+// P: [count][N] canonical mod q from S [count][2][N]; with ntt_out, the
+// fold's form instead: n^-1 NTT_c(P_j) as two u32 rows [2][N] in the 8N bytes
+// of row j.
+void launch_post(uint64_t* P, const uint32_t* S, const uint32_t* tw, int count, hipStream_t s, int ntt_out = 0);
+// The fold's form of P ([clients][instances] rows of two u32 rows [2][N],
+// n^-1 NTT_c(P_j)) from the z-major rotation-term sums S
 // [clients][2][N][instances] plus the z_body terms sum_k z_body[b][k] *
 // bold_t_hat[j][k] (z_body: [clients][3][N], key_client_stride apart).
-void launch_post_zmajor(uint64_t* P, const uint32_t* S, const uint64_t* z_body, int64_t key_client_stride,
-                        const uint64_t* bth, const uint32_t* fold_tab, int clients, int64_t instances, hipStream_t s);
+void launch_post_ntt(uint64_t* P, const uint32_t* S, const uint64_t* z_body, int64_t key_client_stride,
+                     const uint64_t* bth, int clients, int64_t instances, hipStream_t s);
 
 
 // Offline precomputation (setup.hip): bold_t, bold_t_bar, bold_t_hat (condensed)

@@ -4,7 +4,7 @@
 // is a GEMM over the key-switch terms:
 //     S_b[j][p][z] = sum_{i,k,w} y_b[k][slot_w(z,i)] * T_w[j][3i+k][z]
 //                  + sum_k z_body_b[k][z] * bold_t_hat[j][k][z]          (mod q_p)
-// The kernels here compute the first sum; k_post_zmajor adds the second.
+// The kernels here compute the first sum; k_post_ntt adds the second.
 // with T_0 = bold_t, T_1 = bold_t_bar and slot_w(z, i) the slot that the
 // automorphism X -> X^(+-5^i) (packing.rs:1639-1652, rotation_kernel.cu:21-59)
 // maps z to: exponent (2 rev(z) + 1) * (+-5^i) mod 2N.
@@ -279,7 +279,7 @@ struct PackArgs {
 // m = c%4.  Corrects the signed byte limbs, recombines the limbs and stores
 // the rotation-term sum in the z-major layout S[b][p][z][j], so a block's
 // stores fill whole lines; the z_body terms are added by the post-process
-// (k_post_zmajor).  X = sum_l C_lm 2^(7l) < 2^53 is reduced with 32-bit
+// (k_post_ntt).  X = sum_l C_lm 2^(7l) < 2^53 is reduced with 32-bit
 // operations only (reduce58), then multiplied by 2^(8m) < q as a Shoup
 // product; the lane-invariant constants live in TileStore.
 struct TileStore {

@@ -702,7 +702,7 @@ extern "C" int inspire_server_autotune(inspire_server_t* s, int count, int nq, i
   // the variants differently from the same kernel between the other stages.
   auto stages = [&](int n, int pv, int gv, int fv) {
     pack(n, pv);
-    launch_post_zmajor(b.P, b.S, b.z_body, (int64_t)kTExp * kN, s->bth, s->consts->fold, n, inst, st);
+    launch_post_ntt(b.P, b.S, b.z_body, (int64_t)kTExp * kN, s->bth, n, inst, st);
     gemv(n, gv);
     fold(n, fv);
   };
@@ -859,7 +859,7 @@ static int launch_batch(inspire_server_t* s, const uint8_t* d_queries, size_t q_
                           s->pack_variant))
     return set_error(INSPIRE_ERR_INVALID, "packing: shape beyond the kernels' 32-bit fragment offsets");
   mark(2);
-  launch_post_zmajor(b.P, b.S, b.z_body, (int64_t)kTExp * kN, s->bth, s->consts->fold, count, inst, st);
+  launch_post_ntt(b.P, b.S, b.z_body, (int64_t)kTExp * kN, s->bth, count, inst, st);
   mark(3);
   if (launch_gemv_mfma_batch(b.cvec, (int64_t)nq * s->db_cols, b.rows, (int64_t)nq * rows, nq, s->db, s->dbfrag,
                              (int)rows, s->db_cols, count, b.gemv_scratch, st, s->gemv_variant))
@@ -1133,12 +1133,12 @@ int run_pipeline(inspire_server_t* s, Slot& sl, const uint8_t* d_query, int nq, 
       return set_error(INSPIRE_ERR_INVALID, "packing: shape beyond the kernels' 32-bit fragment offsets");
     HIP_TRY(hipEventRecord(s->pack_done, st), "GEMV and packing done");
     mark(3);
-    launch_post_zmajor(sl.P, sl.S, sl.z_body, (int64_t)kTExp * kN, s->bth, s->consts->fold, 1, inst, st);
+    launch_post_ntt(sl.P, sl.S, sl.z_body, (int64_t)kTExp * kN, s->bth, 1, inst, st);
   } else {
     launch_packing(sl.S, s->bt, s->btb, s->bth, sl.y_body, sl.z_body, (int)inst, st);
     HIP_TRY(hipEventRecord(s->pack_done, st), "GEMV and packing done");
     mark(3);
-    launch_post(sl.P, sl.S, s->consts->tw, (int)inst, st);
+    launch_post(sl.P, sl.S, s->consts->tw, (int)inst, st, 1);
   }
   mark(4);
   if (launch_fold(d_out, s->a_hat, s->ahat_ntt, sl.cvec, sl.P, sl.bodies, s->consts->fold, s->consts->tw, inst,
