@@ -644,25 +644,9 @@ __device__ inline void fold_write_response(const FoldArgs& args, int b, int h, i
 // Row 0 of a Horner sum has a known transform.  With the table
 // Ah_j = n^-1 NTT_c(a_hat_j), the fold adds Ah_j to its row-0 product before
 // the inverse transform, so that product row S (n^-1 scaled) satisfies
-// NTT_c(sum row 0) = n S.  The unsigned base-2^19 digits of row 0 (sum < q <
-// 2^57, so g0 + 2^19 g1 + 2^38 g2 = sum exactly) then give
-//     NTT_c(g2) = 2^-38 (n S - NTT_c(g0) - 2^19 NTT_c(g1))
-//               = 2^-27 S - 2^-38 NTT_c(g0) - 2^-19 NTT_c(g1)   (n = 2^11),
-// one of the six digit transforms of a step per prime.  Inputs canonical;
-// returns the canonical value.
-__device__ inline uint32_t row0_third_digit(uint32_t S, uint32_t g0, uint32_t g1, int c) {
-  constexpr uint32_t kA0 = inv_pow2_mod(kQ0, 27), kA1 = inv_pow2_mod(kQ1, 27);
-  constexpr uint32_t kB0 = inv_pow2_mod(kQ0, 38), kB1 = inv_pow2_mod(kQ1, 38);
-  constexpr uint32_t kC0 = inv_pow2_mod(kQ0, 19), kC1 = inv_pow2_mod(kQ1, 19);
-  const uint32_t q = prime(c);
-  const uint32_t a = shoup_lazy(S, c ? kA1 : kA0, c ? shoup_companion(kA1, kQ1) : shoup_companion(kA0, kQ0), q);
-  const uint32_t b = shoup_lazy(g0, c ? kB1 : kB0, c ? shoup_companion(kB1, kQ1) : shoup_companion(kB0, kQ0), q);
-  const uint32_t e = shoup_lazy(g1, c ? kC1 : kC0, c ? shoup_companion(kC1, kQ1) : shoup_companion(kC0, kQ0), q);
-  uint32_t x = a + 4 * q - b - e;  // (0, 6q)
-  x = x >= 4 * q ? x - 4 * q : x;
-  x = x >= 2 * q ? x - 2 * q : x;
-  return x >= q ? x - q : x;
-}
+// NTT_c(sum row 0) = n S = 2^11 S, and the transform of row 0's third gadget
+// digit follows from S and the other two (third_gadget_digit_ntt<11>): one of
+// the six digit transforms of a step per prime.
 
 // LDS row of gadget digit row k (row r + 2 k' for digit k' of row r) among
 // the transformed ones: in the server form, row 4 (row 0's third digit) is
@@ -755,7 +739,7 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
 #pragma unroll
         for (int k = 0; k < 6; ++k)
           if (!kNttAdd || k != 4) g[k] = dig[(kDig * c + digit_row<kNttAdd>(k)) * kNPad + lds_index(z)];
-        if constexpr (kNttAdd) g[4] = row0_third_digit(S[c][s], g[0], g[2], c);
+        if constexpr (kNttAdd) g[4] = third_gadget_digit_ntt<kLogN>(S[c][s], g[0], g[2], c);
         uint32_t gsum = 0;
         uint64_t acc = 0;
 #pragma unroll
@@ -821,7 +805,7 @@ constexpr int kFold2Slots = kN / kFold2Threads;
 
 template <bool kNttAdd>
 __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_fold_seq(FoldArgs args) {
-  constexpr int kDig = kNttAdd ? 5 : 6;  // transformed digit rows per prime (see row0_third_digit)
+  constexpr int kDig = kNttAdd ? 5 : 6;  // transformed digit rows per prime (see third_gadget_digit_ntt)
   __shared__ uint32_t dig[6][kNPad];
   const NttTables tw{args.tw};
   const int per_client = args.nq * args.c;
@@ -907,7 +891,7 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
 #pragma unroll
         for (int k = 0; k < 6; ++k)
           if (!kNttAdd || k != 4) g[k] = dig[digit_row<kNttAdd>(k)][lz];
-        if constexpr (kNttAdd) g[4] = row0_third_digit(c == 0 ? S0[s] : dig[5][lz], g[0], g[2], c);
+        if constexpr (kNttAdd) g[4] = third_gadget_digit_ntt<kLogN>(c == 0 ? S0[s] : dig[5][lz], g[0], g[2], c);
         uint32_t gsum = 0;
         uint64_t acc = 0;
 #pragma unroll

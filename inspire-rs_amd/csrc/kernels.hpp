@@ -115,6 +115,9 @@ void launch_post_ntt(uint64_t* P, const uint32_t* S, const uint64_t* z_body, int
 // with setup_offline_bytes_per_instance() bytes of work memory each.  Returns
 // an INSPIRE_* status (the message set on failure); synchronous on s.
 size_t setup_offline_bytes_per_instance();
+// Instances per pass that keep every collapse chain of the pass resident (0 on
+// a HIP failure).
+int64_t setup_offline_resident_instances();
 int launch_setup_offline(const uint16_t* db, int64_t db_rows, int64_t instances, uint64_t* bt, uint64_t* btb,
                          uint64_t* bth, uint64_t* a_hat, const uint32_t* tw, int64_t group, hipStream_t s);
 

@@ -518,6 +518,29 @@ __device__ inline void ntt_inverse_r4_halves_unscaled(uint32_t* base, const uint
   }
 }
 
+// The transform of the third unsigned base-2^19 gadget digit (gadget.rs:35-61)
+// of a polynomial x with coefficients below q < 2^57, so that d0 + 2^19 d1 +
+// 2^38 d2 = x exactly, from S = 2^-kM NTT_c(x) and the other two digits'
+// transforms G0, G1, by linearity:
+//     NTT_c(d2) = 2^(kM-38) S - 2^-38 G0 - 2^-19 G1   (mod q_c).
+// Inputs canonical; returns the canonical value.
+template <int kM>
+__device__ inline uint32_t third_gadget_digit_ntt(uint32_t S, uint32_t g0, uint32_t g1, int c) {
+  static_assert(3 * kGadgetBits >= 57, "three digits must cover q");
+  constexpr int kE = 2 * kGadgetBits - kM;
+  constexpr uint32_t kA0 = inv_pow2_mod(kQ0, kE), kA1 = inv_pow2_mod(kQ1, kE);
+  constexpr uint32_t kB0 = inv_pow2_mod(kQ0, 2 * kGadgetBits), kB1 = inv_pow2_mod(kQ1, 2 * kGadgetBits);
+  constexpr uint32_t kC0 = inv_pow2_mod(kQ0, kGadgetBits), kC1 = inv_pow2_mod(kQ1, kGadgetBits);
+  const uint32_t q = prime(c);
+  const uint32_t a = shoup_lazy(S, c ? kA1 : kA0, c ? shoup_companion(kA1, kQ1) : shoup_companion(kA0, kQ0), q);
+  const uint32_t b = shoup_lazy(g0, c ? kB1 : kB0, c ? shoup_companion(kB1, kQ1) : shoup_companion(kB0, kQ0), q);
+  const uint32_t e = shoup_lazy(g1, c ? kC1 : kC0, c ? shoup_companion(kC1, kQ1) : shoup_companion(kC0, kQ0), q);
+  uint32_t x = a + 4 * q - b - e;  // (0, 6q)
+  x = x >= 4 * q ? x - 4 * q : x;
+  x = x >= 2 * q ? x - 2 * q : x;
+  return x >= q ? x - q : x;
+}
+
 // NP polynomials over prime c.
 template <int NT, int NP>
 __device__ inline void ntt_inverse_r4_1p_unscaled(uint32_t* const (&polys)[NP], int c,

@@ -945,12 +945,16 @@ extern "C" int inspire_server_set_precomp(inspire_server_t* s, const uint64_t* b
 }
 
 // Work group of instances for the offline precomputation: as many as fit in
-// free memory less 1 GiB, at most 256.
+// free memory less 1 GiB, at most as many as keep all their collapse chains
+// resident at once (384 on 256 CUs), so that the latency-bound collapse runs
+// in as few passes as possible.
 static int setup_group(int64_t instances, int64_t* group) {
   size_t free_b = 0, total_b = 0;
   HIP_TRY(hipMemGetInfo(&free_b, &total_b), "memory info");
   const size_t spare = free_b > (1ull << 30) ? free_b - (1ull << 30) : 0;
-  *group = std::min<int64_t>({instances, 256, (int64_t)(spare / setup_offline_bytes_per_instance())});
+  const int64_t resident = setup_offline_resident_instances();
+  if (resident <= 0) return set_error(INSPIRE_ERR_HIP, "offline precomputation: device occupancy query failed");
+  *group = std::min<int64_t>({instances, resident, (int64_t)(spare / setup_offline_bytes_per_instance())});
   if (*group < 1) return set_error(INSPIRE_ERR_NOMEM, "no room for the offline precomputation of one instance");
   return INSPIRE_OK;
 }

@@ -123,10 +123,41 @@ __global__ __launch_bounds__(kSlotThreads) void k_setup_slot_ntt(uint32_t* __res
 #pragma unroll
   for (int p = 0; p < kSlotBlock; ++p) polys[p] = lds + p * kNPad;
   ntt_forward_r4_1p<kSlotThreads, kSlotBlock>(polys, c, T);
-  uint32_t* dst = R + ((g * 2 + c) * kN + zb * kSlotBlock) * (int64_t)kN;
-  for (int e = tid; e < kN * kSlotBlock; e += kSlotThreads) {
-    const int zz = e / kN, k = e % kN;
-    dst[(int64_t)zz * kN + k] = lds[zz * kNPad + lds_index(k)];
+  // Step order: the collapse of chain w reads, at step i and slot k, n^-1 R_z[k]
+  // with e_z = u e_k (mod 2n), u = (-1)^w 5^i; it is stored at R[g][c][w n/2 +
+  // i][k].  This workgroup's 16 slots z0.. are the exponents e = a (mod 256), and
+  // multiplying by u maps that coset to the 16 consecutive slots k0(u).. with
+  // e_k0 = a u^-1 (mod 256), so every (w, i) row gets one contiguous 64-byte run
+  // from here and the collapse reads each step's row in order.
+  const int z0 = zb * kSlotBlock;
+  const uint32_t a = (2u * rev11d((uint32_t)z0) + 1u) & 255u;
+  const int m = tid % kSlotBlock, rb = tid / kSlotBlock;  // rows rb + 32 it of each chain
+  constexpr int kRowStep = kSlotThreads / kSlotBlock;     // 32
+  constexpr uint32_t kMask2n = 2u * kN - 1u;
+  uint32_t p = 1, pinv = 1;
+  for (int x = 0; x < rb; ++x) {
+    p = (p * 5u) & kMask2n;
+    pinv = (pinv * kFiveInv) & kMask2n;
+  }
+  uint32_t step = 1, step_inv = 1;
+  for (int x = 0; x < kRowStep; ++x) {
+    step = (step * 5u) & kMask2n;
+    step_inv = (step_inv * kFiveInv) & kMask2n;
+  }
+  uint32_t* dst = R + (g * 2 + c) * (int64_t)kN * kN;
+  for (int w = 0; w < 2; ++w) {
+    uint32_t u5 = p, u5inv = pinv;  // 5^i and 5^-i for i = rb + 32 it
+    for (int i = rb; i < kN / 2; i += kRowStep) {
+      const uint32_t u = w ? (2u * kN - u5) & kMask2n : u5;
+      const uint32_t uinv = w ? (2u * kN - u5inv) & kMask2n : u5inv;
+      const uint32_t b = (a * uinv) & 255u;  // e_k0 mod 256 = 2 rev7(k0 / 16) + 1
+      const int k = (int)(__brev((b - 1u) >> 1) >> 25) * kSlotBlock + m;
+      const int zz = auto_slot(u, k) - z0;
+      const uint32_t v = lds[zz * kNPad + lds_index(k)];
+      dst[((int64_t)w * (kN / 2) + i) * kN + k] = c ? mulmod<1>(v, kNInvQ1) : mulmod<0>(v, kNInvQ0);
+      u5 = (u5 * step) & kMask2n;
+      u5inv = (u5inv * step_inv) & kMask2n;
+    }
   }
 }
 
@@ -141,14 +172,17 @@ struct CollapseArgs {
   uint32_t g_top;       // 5^(n/2 - 1) mod 2n
 };
 
-// G^-1 of the canonical NTT-domain polynomial in dig[0], dig[1] (both primes):
-// INTT, CRT compose, 19-bit digits into dig[2k + c], forward NTT.  Every thread
-// touches only its own slots between the two transforms.
+// G^-1 of the canonical NTT-domain polynomial x in dig[0], dig[1] (both
+// primes): INTT, CRT compose, the first two 19-bit digits into dig[2k + c]
+// and their forward NTTs.  The third digit's transform follows from x and
+// those two (third_gadget_digit_ntt<0>, with the copy of x the caller keeps in
+// dig[4], dig[5]), so it is never transformed.  Every thread touches only its
+// own slots between the two transforms.
 __device__ inline void gadget_ntt(uint32_t (&dig)[6][kNPad], const NttTables& T) {
   uint32_t* const ip[2] = {dig[0], dig[1]};
   const int ipr[2] = {0, 1};
-  uint32_t* const fp[6] = {dig[0], dig[1], dig[2], dig[3], dig[4], dig[5]};
-  const int fpr[6] = {0, 1, 0, 1, 0, 1};
+  uint32_t* const fp[4] = {dig[0], dig[1], dig[2], dig[3]};
+  const int fpr[4] = {0, 1, 0, 1};
   ntt_inverse_r4<kColThreads, 2>(ip, ipr, T);
   const uint64_t mask = (1ull << kGadgetBits) - 1;
 #pragma unroll
@@ -156,18 +190,20 @@ __device__ inline void gadget_ntt(uint32_t (&dig)[6][kNPad], const NttTables& T)
     const int lz = lds_index(threadIdx.x + s * kColThreads);
     const uint64_t raw = crt_compose(dig[0][lz], dig[1][lz]);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 2; ++k) {
       const uint32_t d = (uint32_t)((raw >> (kGadgetBits * k)) & mask);  // < 2^19: canonical for both primes
       dig[2 * k][lz] = d;
       dig[2 * k + 1][lz] = d;
     }
   }
   __syncthreads();
-  ntt_forward_r4m<kColThreads, 6>(fp, fpr, T);
+  ntt_forward_r4m<kColThreads, 4>(fp, fpr, T);
 }
 
 __global__ __launch_bounds__(kColThreads) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_setup_collapse(
     CollapseArgs a) {
+  // rows 0-3: the transformed digits; rows 4, 5: r_(i+1) per prime at each
+  // thread's own slots (read back by the same thread for the third digit)
   __shared__ uint32_t dig[6][kNPad];
   const NttTables T{a.tw};
   const int tid = threadIdx.x;
@@ -175,41 +211,36 @@ __global__ __launch_bounds__(kColThreads) __attribute__((amdgpu_waves_per_eu(6, 
   const int which = blockIdx.x & 1;  // 0: r chain (bold_t), 1: rbar chain (bold_t_bar)
   const uint32_t* Rg = a.R + g * 2 * (int64_t)kN * kN;
   uint64_t* out = (which ? a.btb : a.bt) + g * (int64_t)kNInner * kN;
-  const uint32_t ni0 = 268238881u, ni1 = 249439233u;  // 2048^-1 mod q_c
 
-  // r_i (or rbar_i) at this thread's slots, and the W mask rotated by t_i.
-  auto load_step = [&](uint32_t gi, uint32_t (&rv)[2][kColSlots], uint32_t (&wv)[3][2][kColSlots]) {
-    const uint32_t t = which ? 2u * kN - gi : gi;
+  // r_i (or rbar_i) at this thread's slots, in step order in R (see
+  // k_setup_slot_ntt).
+  auto load_r = [&](int i, uint32_t (&rr)[2][kColSlots]) {
+    const int64_t row = (int64_t)which * (kN / 2) + i;
 #pragma unroll
     for (int s = 0; s < kColSlots; ++s) {
       const int z = tid + s * kColThreads;
-      const int zs = auto_slot(t, z);
-      rv[0][s] = mulmod<0>(Rg[(int64_t)zs * kN + z], ni0);
-      rv[1][s] = mulmod<1>(Rg[((int64_t)kN + zs) * kN + z], ni1);
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        wv[k][0][s] = a.w[(k * 2 + 0) * kN + zs];
-        wv[k][1][s] = a.w[(k * 2 + 1) * kN + zs];
-      }
+      rr[0][s] = Rg[row * kN + z];
+      rr[1][s] = Rg[((int64_t)kN + row) * kN + z];
     }
   };
 
-  uint32_t cur[2][kColSlots], rv[2][kColSlots], wv[3][2][kColSlots];
+  uint32_t cur[2][kColSlots], rr[2][kColSlots];
   uint32_t gi = a.g_top;  // g_(n/2 - 1)
-  load_step(gi, rv, wv);
+  load_r(kN / 2 - 1, rr);
 #pragma unroll
   for (int s = 0; s < kColSlots; ++s) {
-    cur[0][s] = rv[0][s];
-    cur[1][s] = rv[1][s];
+    cur[0][s] = rr[0][s];
+    cur[1][s] = rr[1][s];
   }
   for (int i = kN / 2 - 2; i >= 0; --i) {
     gi = (gi * kFiveInv) & (2u * kN - 1u);  // g_i = 5^i mod 2n
-    load_step(gi, rv, wv);                  // independent of the chain: in flight during the transforms
+    const uint32_t t = which ? 2u * kN - gi : gi;
+    load_r(i, rr);  // independent of the chain: in flight during the transforms
 #pragma unroll
     for (int s = 0; s < kColSlots; ++s) {
       const int lz = lds_index(tid + s * kColThreads);
-      dig[0][lz] = cur[0][s];
-      dig[1][lz] = cur[1][s];
+      dig[0][lz] = dig[4][lz] = cur[0][s];
+      dig[1][lz] = dig[5][lz] = cur[1][s];
     }
     __syncthreads();
     gadget_ntt(dig, T);
@@ -218,13 +249,21 @@ __global__ __launch_bounds__(kColThreads) __attribute__((amdgpu_waves_per_eu(6, 
     for (int s = 0; s < kColSlots; ++s) {
       const int z = tid + s * kColThreads;
       const int lz = lds_index(z);
-      uint64_t acc0 = rv[0][s], acc1 = rv[1][s];
+      const int zs = auto_slot(t, z);  // the W mask rotated by t_i (cache-resident)
+      uint64_t acc0 = rr[0][s], acc1 = rr[1][s];
+      uint32_t dk[3][2];
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        dk[k][0] = dig[2 * k][lz];
+        dk[k][1] = dig[2 * k + 1][lz];
+      }
+#pragma unroll
+      for (int c = 0; c < 2; ++c) dk[2][c] = third_gadget_digit_ntt<0>(dig[4 + c][lz], dk[0][c], dk[1][c], c);
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        const uint32_t d0 = dig[2 * k][lz], d1 = dig[2 * k + 1][lz];
-        out[((int64_t)i * kTExp + k) * kN + z] = (uint64_t)d0 | ((uint64_t)d1 << 32);
-        acc0 += (uint64_t)wv[k][0][s] * d0;
-        acc1 += (uint64_t)wv[k][1][s] * d1;
+        out[((int64_t)i * kTExp + k) * kN + z] = (uint64_t)dk[k][0] | ((uint64_t)dk[k][1] << 32);
+        acc0 += (uint64_t)a.w[(k * 2 + 0) * kN + zs] * dk[k][0];
+        acc1 += (uint64_t)a.w[(k * 2 + 1) * kN + zs] * dk[k][1];
       }
       cur[0][s] = reduce58<0>(acc0);
       cur[1][s] = reduce58<1>(acc1);
@@ -243,8 +282,8 @@ __global__ __launch_bounds__(kColThreads) __attribute__((amdgpu_waves_per_eu(6, 
 #pragma unroll
   for (int s = 0; s < kColSlots; ++s) {
     const int lz = lds_index(tid + s * kColThreads);
-    dig[0][lz] = cur[0][s];
-    dig[1][lz] = cur[1][s];
+    dig[0][lz] = dig[4][lz] = cur[0][s];
+    dig[1][lz] = dig[5][lz] = cur[1][s];
   }
   __syncthreads();
   gadget_ntt(dig, T);
@@ -252,9 +291,17 @@ __global__ __launch_bounds__(kColThreads) __attribute__((amdgpu_waves_per_eu(6, 
   for (int s = 0; s < kColSlots; ++s) {
     const int z = tid + s * kColThreads;
     const int lz = lds_index(z);
+    uint32_t dk[3][2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      dk[k][0] = dig[2 * k][lz];
+      dk[k][1] = dig[2 * k + 1][lz];
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) dk[2][c] = third_gadget_digit_ntt<0>(dig[4 + c][lz], dk[0][c], dk[1][c], c);
 #pragma unroll
     for (int k = 0; k < 3; ++k)
-      a.bth[(g * kTExp + k) * kN + z] = (uint64_t)dig[2 * k][lz] | ((uint64_t)dig[2 * k + 1][lz] << 32);
+      a.bth[(g * kTExp + k) * kN + z] = (uint64_t)dk[k][0] | ((uint64_t)dk[k][1] << 32);
   }
 }
 
@@ -299,6 +346,19 @@ uint32_t pow5_mod_2n(int e) {
 }  // namespace
 
 size_t setup_offline_bytes_per_instance() { return (size_t)2 * 2 * kN * kN * sizeof(uint32_t) + 2 * kN * 4; }
+
+// Instances whose two collapse chains are all resident on the current device
+// at once (workgroups per CU x CUs / 2).  Each chain's 1023 steps are
+// dependent, so a pass takes about the same time for any number of chains up
+// to that; 0 on a HIP failure.
+int64_t setup_offline_resident_instances() {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_setup_collapse, kColThreads, 0) != hipSuccess)
+    return 0;
+  return (int64_t)cus * per_cu / 2;
+}
 
 int launch_setup_offline(const uint16_t* db, int64_t db_rows, int64_t instances, uint64_t* bt, uint64_t* btb,
                          uint64_t* bth, uint64_t* a_hat, const uint32_t* tw, int64_t group, hipStream_t s) {
