@@ -600,9 +600,9 @@ def main():
         result["batched_roofline"] = {"B": B, "packing": mk(pack_ops, st_b["packing"], "k_packing_wide variant %d" % pv),
                                       "gemv": mk(gemv_ops, st_b["gemv"], gemv_kernel),
                                       "clock_note": "peak is the 2.4 GHz figure; under this load the packing runs at "
-                                                    "1.6 GHz and the GEMV at 1.9 GHz (SQ_BUSY_CYCLES per shader "
-                                                    "engine), where their matrix cores are busy 62% and 54% of the "
-                                                    "kernel: profiles/batched_b64_clock_and_matrix_busy.md"}
+                                                    "1.6 GHz and the GEMV at 1.8-1.9 GHz (SQ_BUSY_CYCLES per shader "
+                                                    "engine), where their matrix cores are busy 61-64% and 54-57% of "
+                                                    "the kernel: profiles/batched_b64_clock_and_matrix_busy.md"}
         log("batched MFMA at B=%d: packing %.0f TOP/s (%.1f%%), gemv %.0f TOP/s (%.1f%%)" % (
             B, result["batched_roofline"]["packing"]["achieved"], 100 * result["batched_roofline"]["packing"]["frac"],
             result["batched_roofline"]["gemv"]["achieved"], 100 * result["batched_roofline"]["gemv"]["frac"]))
