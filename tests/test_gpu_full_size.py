@@ -138,6 +138,31 @@ def test_full_size_sequential_queries_match_batched_and_oracle(server):
             assert chain_of(mine[h * half:(h + 1) * half], w) == exp[h], (i, w, h)
 
 
+def test_offline_setup_in_two_passes_matches_oracle():
+    # One pass holds as many instances as keep all their collapse chains
+    # resident (384 on 256 CUs), so 400 instances run in two passes; the first
+    # and last output of each pass against the oracle.
+    import torch
+    inst = 400
+    say("offline setup, %d outputs in two passes" % inst)
+    ncell = ROWS * inst * N
+    d_db = torch.empty(ncell, dtype=torch.int16, device="cuda")
+    IA.synth_fill(d_db, 29, O.STREAM_DB, ncell, O.KIND_U16)
+    outs = [torch.zeros(n, dtype=torch.int64, device="cuda")
+            for n in (inst * O.NINNER * N, inst * O.NINNER * N, inst * 3 * N, inst * N)]
+    IA.setup_offline_device(d_db, ROWS, inst, *outs)
+    torch.cuda.synchronize()
+    per = (O.NINNER * N, O.NINNER * N, 3 * N, N)
+    for t in (0, 383, 384, inst - 1):
+        db = O.synth(29, O.STREAM_DB, N * ROWS, O.KIND_U16, start=t * N * ROWS)
+        exp = O.setup_precomp(O.setup_atilde(db, ROWS, 0, N), 1)
+        for name, got, e, n in zip(("bold_t", "bold_t_bar", "bold_t_hat", "a_hat"), outs, exp, per):
+            g = got[t * n:(t + 1) * n].cpu().numpy().view(np.uint64)
+            assert np.array_equal(g, e), (t, name)
+    del d_db, outs
+    torch.cuda.empty_cache()
+
+
 def test_full_size_offline_setup_matches_oracle():
     # The offline precomputation over the whole 1M-key DB (synthetic encoded
     # DB) on the device; outputs 0 and 351 against the oracle.

@@ -172,6 +172,29 @@ def test_fold_matches_oracle(torch, instances, d, nq):
             assert got[h, w].tobytes() == O.switch_and_keep(ct), (h, w)
 
 
+def test_fold_many_chains_matches_oracle(torch):
+    # 512 chains in one launch select the 512-thread fold (primes in turn) of
+    # the stateless entry point; a sample of chains against the oracle.
+    instances, d, nq = 1024, 2, 1
+    say("fold, 512 chains", instances, d, nq)
+    c = instances // d
+    a_hat = O.synth(81, O.STREAM_AHAT, instances * N, O.KIND_MODQ)
+    cvec = O.synth(82, 8, nq * instances * N, O.KIND_MODQ)
+    P = O.synth(83, 9, instances * N, O.KIND_MODQ)
+    bodies = O.synth(84, O.STREAM_BODY, nq * 6 * 2 * N, O.KIND_CRT)
+    dout = empty(torch, nq * c * O.RESP_BYTES, np.uint8)
+    IA.fold_device(dout, dev_from(torch, a_hat), dev_from(torch, cvec), dev_from(torch, P),
+                   dev_from(torch, bodies.astype(np.uint32)), instances, d, nq)
+    got = host(torch, dout, np.uint8).reshape(nq, c, O.RESP_BYTES)
+    neg_a, _ = O.rgsw_neg_a_ntt()
+    ah = a_hat.reshape(instances, N)
+    fb = cvec.reshape(nq, instances, N)[0] + P.reshape(instances, N)
+    body = bodies.reshape(nq, 6, 2, N)[0]
+    for w in (0, 1, 255, 256, 510, 511):
+        ct = O.fold_chain(ah[w * d:(w + 1) * d], fb[w * d:(w + 1) * d], neg_a, body)
+        assert got[0, w].tobytes() == O.switch_and_keep(ct), w
+
+
 def _golden():
     return json.load(open(os.path.join(os.path.dirname(__file__), "golden", "respond_small.json")))["cases"]
 
