@@ -23,6 +23,19 @@ constexpr int kNPad = kN + kN / 32;  // 2112 words per polynomial in LDS
 
 __host__ __device__ constexpr int lds_index(int x) { return x + (x >> 5); }
 
+// Forward Cooley-Tukey butterfly with Harvey's lazy reduction: x, y in [0, 4q)
+// -> x + w y, x - w y + 2q in [0, 4q), wp = floor(w 2^32 / q).  The Shoup
+// product is formed negated, nqn = qt q - w y = -(w y mod q in [0, 2q)) mod
+// 2^32, so that y' = xr + 2q + nqn is one three-input add (8 VALU operations
+// per butterfly instead of 9).
+__device__ inline void fwd_bf(uint32_t& x, uint32_t& y, uint32_t w, uint32_t wp, uint32_t q) {
+  const uint32_t q2 = 2 * q;
+  const uint32_t xr = x >= q2 ? x - q2 : x;
+  const uint32_t nqn = mulhi32(y, wp) * q - w * y;
+  x = xr - nqn;
+  y = xr + q2 + nqn;
+}
+
 // Twiddle tables per prime: [fwd w][fwd w'][inv w][inv w'] x 2048, u32.
 struct NttTables {
   const uint32_t* __restrict__ t;  // [2][4][kN]
@@ -38,13 +51,7 @@ struct NttTables {
 template <int NT, int NP>
 __device__ inline void ntt_forward_r4m(uint32_t* const (&polys)[NP], const int (&prime_of)[NP], const NttTables& tw) {
   const int tid = threadIdx.x;
-  auto bf = [](uint32_t& x, uint32_t& y, uint32_t w, uint32_t wp, uint32_t q) {
-    const uint32_t q2 = 2 * q;
-    const uint32_t xr = x >= q2 ? x - q2 : x;
-    const uint32_t qn = shoup_lazy(y, w, wp, q);
-    x = xr + qn;
-    y = xr + q2 - qn;
-  };
+  auto bf = [](uint32_t& x, uint32_t& y, uint32_t w, uint32_t wp, uint32_t q) { fwd_bf(x, y, w, wp, q); };
   // Stages (mm, mm+1): t = N/2 >> mm, units (i < m, j < t/2) at base 2t*i + j.
   for (int mm = 0; mm + 1 < kLogN; mm += 2) {
     const int m = 1 << mm, t = (kN / 2) >> mm, th = t >> 1;
@@ -187,12 +194,7 @@ __device__ inline void ntt_forward_r4_1p(uint32_t* const (&polys)[NP], int c, co
   const uint32_t q = prime(c), q2 = 2 * q;
   const uint32_t* fw = tw.fw(c);
   const uint32_t* fwp = tw.fwp(c);
-  auto bf = [q2, q](uint32_t& x, uint32_t& y, uint32_t w, uint32_t wp) {
-    const uint32_t xr = x >= q2 ? x - q2 : x;
-    const uint32_t qn = shoup_lazy(y, w, wp, q);
-    x = xr + qn;
-    y = xr + q2 - qn;
-  };
+  auto bf = [q](uint32_t& x, uint32_t& y, uint32_t w, uint32_t wp) { fwd_bf(x, y, w, wp, q); };
   for (int mm = 0; mm + 1 < kLogN; mm += 2) {
     const int m = 1 << mm, t = (kN / 2) >> mm, th = t >> 1;
     for (int u = tid; u < kN / 4; u += NT) {
@@ -398,12 +400,7 @@ __device__ inline void ntt_forward_r4_halves(uint32_t* base, const NttTables& tw
   const uint32_t q = prime(c), q2 = 2 * q;
   const uint32_t* fw = tw.fw(c);
   const uint32_t* fwp = tw.fwp(c);
-  auto bf = [q2, q](uint32_t& x, uint32_t& y, uint32_t w, uint32_t wp) {
-    const uint32_t xr = x >= q2 ? x - q2 : x;
-    const uint32_t qn = shoup_lazy(y, w, wp, q);
-    x = xr + qn;
-    y = xr + q2 - qn;
-  };
+  auto bf = [q](uint32_t& x, uint32_t& y, uint32_t w, uint32_t wp) { fwd_bf(x, y, w, wp, q); };
 #pragma unroll 1
   for (int mm = 0; mm + 1 < kLogN; mm += 2) {
     const int m = 1 << mm, t = (kN / 2) >> mm, th = t >> 1;
