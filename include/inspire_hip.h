@@ -343,7 +343,8 @@ int inspire_unpack_query_device(const uint8_t* d_query, int nq, int64_t db_rows,
 /* The offline precomputation of inspire_server_setup_offline on caller-owned
  * device buffers: d_db encoded [instances*2048][db_rows] u16; outputs in the
  * layouts of inspire_server_set_precomp.  Work memory: 64 MiB per instance of
- * a pass, as many instances per pass as fit in free memory less 1 GiB.
+ * a pass; a pass takes as many instances as fit in free memory less 1 GiB, at
+ * most as many as keep all their collapse chains resident (384 on 256 CUs).
  * Synchronous on `stream` (NULL: the null stream). */
 int inspire_setup_offline_device(const uint16_t* d_db, int64_t db_rows, int64_t instances, uint64_t* d_bold_t,
                                  uint64_t* d_bold_t_bar, uint64_t* d_bold_t_hat, uint64_t* d_a_hat, void* stream);
