@@ -575,16 +575,26 @@ int launch_gemv_mfma_batch(uint64_t* c, int64_t c_client_stride, const uint64_t*
 
 size_t gemv_mfma_scratch_bytes(int db_rows) { return (size_t)(db_rows / 64) * 64 * 16 + 16 * sizeof(int32_t); }
 
-int launch_gemv_mfma(uint64_t* c, const uint64_t* rows, int nq, const uint16_t* db, const void* dbfrag,
-                     int db_rows, int64_t db_cols, void* scratch, hipStream_t s) {
-  if (nq < 1 || nq > 2 || db_rows <= 0 || db_rows % kMfmaChunkRows != 0 || db_rows > (1 << 17) || db_cols <= 0)
-    return -1;
+static bool gemv_mfma_shape_ok(int nq, int db_rows, int64_t db_cols) {
+  return nq >= 1 && nq <= 2 && db_rows > 0 && db_rows % kMfmaChunkRows == 0 && db_rows <= (1 << 17) && db_cols > 0;
+}
+
+int launch_gemv_mfma_prep(const uint64_t* rows, int nq, int db_rows, void* scratch, hipStream_t s) {
+  if (!gemv_mfma_shape_ok(nq, db_rows, 1)) return -1;
   int8_t* afrag = static_cast<int8_t*>(scratch);
   int32_t* rsum = reinterpret_cast<int32_t*>(afrag + (size_t)(db_rows / 64) * 64 * 16);
   // rsum is accumulated with atomics: clear it with a kernel (graph-capture safe).
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(rsum), 16);
   const int prep_threads = (db_rows / 64) * 64;
   hipLaunchKernelGGL(k_gemv_prep, dim3((prep_threads + 255) / 256), dim3(256), 0, s, rows, nq, db_rows, afrag, rsum);
+  return 0;
+}
+
+int launch_gemv_mfma_main(uint64_t* c, int nq, const uint16_t* db, const void* dbfrag, int db_rows, int64_t db_cols,
+                          const void* scratch, hipStream_t s) {
+  if (!gemv_mfma_shape_ok(nq, db_rows, db_cols)) return -1;
+  const int8_t* afrag = static_cast<const int8_t*>(scratch);
+  const int32_t* rsum = reinterpret_cast<const int32_t*>(afrag + (size_t)(db_rows / 64) * 64 * 16);
   const unsigned blocks = (unsigned)((db_cols + kColsPerBlock - 1) / kColsPerBlock);
   if (dbfrag && db_cols % 16 == 0)
     hipLaunchKernelGGL((k_gemv_mfma<kTilesPerWave, kGemvRing, true>), dim3(blocks), dim3(256), 0, s, c, afrag, rsum,
@@ -593,6 +603,13 @@ int launch_gemv_mfma(uint64_t* c, const uint64_t* rows, int nq, const uint16_t* 
     hipLaunchKernelGGL((k_gemv_mfma<kTilesPerWave, kGemvRing, false>), dim3(blocks), dim3(256), 0, s, c, afrag, rsum,
                        db, static_cast<const u4*>(nullptr), nq, db_rows, db_cols);
   return 0;
+}
+
+int launch_gemv_mfma(uint64_t* c, const uint64_t* rows, int nq, const uint16_t* db, const void* dbfrag,
+                     int db_rows, int64_t db_cols, void* scratch, hipStream_t s) {
+  if (!gemv_mfma_shape_ok(nq, db_rows, db_cols)) return -1;
+  launch_gemv_mfma_prep(rows, nq, db_rows, scratch, s);
+  return launch_gemv_mfma_main(c, nq, db, dbfrag, db_rows, db_cols, scratch, s);
 }
 
 }  // namespace inspire

@@ -53,6 +53,11 @@ int launch_tile_db(void* out, const uint16_t* db, int db_rows, int64_t db_cols, 
 size_t gemv_mfma_scratch_bytes(int db_rows);
 int launch_gemv_mfma(uint64_t* c, const uint64_t* rows, int nq, const uint16_t* db, const void* dbfrag,
                      int db_rows, int64_t db_cols, void* scratch, hipStream_t s);
+// The same in two halves: the query-fragment build into scratch (which
+// depends on the query alone), then the DB stream that reads it.
+int launch_gemv_mfma_prep(const uint64_t* rows, int nq, int db_rows, void* scratch, hipStream_t s);
+int launch_gemv_mfma_main(uint64_t* c, int nq, const uint16_t* db, const void* dbfrag, int db_rows, int64_t db_cols,
+                          const void* scratch, hipStream_t s);
 
 // Batched: `count` keyword queries of nq <= 2 rows each; c[client][h][db_cols].
 size_t gemv_mfma_batch_scratch_bytes(int db_rows, int count);
@@ -80,6 +85,12 @@ size_t packing_afrag_bytes(int count);
 int launch_packing_mfma(uint32_t* S, int64_t s_client_stride, const uint8_t* tiled, uint8_t* afrag,
                         const uint64_t* y_body, int64_t key_client_stride, int count, int64_t instances,
                         hipStream_t s, int variant = 0);
+// The same in two halves: the client-key fragments and limb sums into afrag
+// (from the keys alone), then the key-switch GEMM that reads them.
+int launch_packing_mfma_prep(uint8_t* afrag, const uint64_t* y_body, int64_t key_client_stride, int count,
+                             hipStream_t s);
+int launch_packing_mfma_main(uint32_t* S, int64_t s_client_stride, const uint8_t* tiled, const uint8_t* afrag,
+                             int count, int64_t instances, hipStream_t s, int variant = 0);
 // Tile configurations of the 64-client-group packing kernel (variant index).
 int packing_wide_variants();
 const char* packing_wide_variant_name(int v);

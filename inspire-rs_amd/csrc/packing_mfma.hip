@@ -629,12 +629,9 @@ const char* packing_wide_variant_name(int v) {
   return v >= 0 && v < packing_wide_variants() ? kWideVariantNames[v] : "";
 }
 
-int launch_packing_mfma(uint32_t* S, int64_t s_client_stride, const uint8_t* tiled, uint8_t* afrag,
-                        const uint64_t* y_body, int64_t key_client_stride, int count, int64_t instances,
-                        hipStream_t s, int variant) {
+int launch_packing_mfma_prep(uint8_t* afrag, const uint64_t* y_body, int64_t key_client_stride, int count,
+                             hipStream_t s) {
   if (count < 1) return -1;
-  // Fragment offsets within one (prime, slot) are 32-bit buffer offsets.
-  if (tiled_ntiles(instances) * kKSteps * kFragBytes > INT32_MAX) return -1;
   const int bpad = batch_key_pad(count);
   const int64_t total = 2 * (int64_t)(bpad / 4) * kKSteps * 64;
   hipLaunchKernelGGL(k_batch_afrag, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, afrag, y_body,
@@ -642,6 +639,24 @@ int launch_packing_mfma(uint32_t* S, int64_t s_client_stride, const uint8_t* til
   uint32_t* rsum = reinterpret_cast<uint32_t*>(afrag + afrag_frag_bytes(bpad));
   hipLaunchKernelGGL(k_batch_rowsum, dim3((unsigned)(2 * bpad)), dim3(256), 0, s, rsum, y_body, key_client_stride,
                      count, bpad);
+  return 0;
+}
+
+int launch_packing_mfma(uint32_t* S, int64_t s_client_stride, const uint8_t* tiled, uint8_t* afrag,
+                        const uint64_t* y_body, int64_t key_client_stride, int count, int64_t instances,
+                        hipStream_t s, int variant) {
+  if (count < 1 || tiled_ntiles(instances) * kKSteps * kFragBytes > INT32_MAX) return -1;
+  launch_packing_mfma_prep(afrag, y_body, key_client_stride, count, s);
+  return launch_packing_mfma_main(S, s_client_stride, tiled, afrag, count, instances, s, variant);
+}
+
+int launch_packing_mfma_main(uint32_t* S, int64_t s_client_stride, const uint8_t* tiled, const uint8_t* afrag,
+                             int count, int64_t instances, hipStream_t s, int variant) {
+  if (count < 1) return -1;
+  // Fragment offsets within one (prime, slot) are 32-bit buffer offsets.
+  if (tiled_ntiles(instances) * kKSteps * kFragBytes > INT32_MAX) return -1;
+  const int bpad = batch_key_pad(count);
+  const uint32_t* rsum = reinterpret_cast<const uint32_t*>(afrag + afrag_frag_bytes(bpad));
   PackArgs a{S, tiled, afrag, rsum, instances, tiled_ntiles(instances), bpad, count, 0, 0, s_client_stride};
   if (bpad <= 4)
     launch_small<1, 44>(a, s);    // 1 M-tile x 44 N-tiles (11 per wave)

@@ -1115,11 +1115,23 @@ int run_pipeline(inspire_server_t* s, Slot& sl, const uint8_t* d_query, int nq, 
   };
   mark(0);
   launch_unpack_query(d_query, nq, (int)rows, sl.y_body, sl.z_body, sl.rows, sl.bodies, st);
+  // The operand builds of the GEMV and the packing depend on this query alone
+  // and write this slot's buffers: they run before the wait below, beside the
+  // previous request's packing.  (With nq > 2 the GEMV pairs share one scratch,
+  // so each pair builds its operands right before its stream.)
+  const bool mfma_gemv = s->gemv_kind == INSPIRE_GEMV_MFMA, mfma_pack = s->packing_kind == INSPIRE_PACKING_MFMA;
+  if (mfma_pack && !s->tiled_valid) return set_error(INSPIRE_ERR_STATE, "tiled key-switch material not built");
+  if (mfma_gemv && nq <= 2 && launch_gemv_mfma_prep(sl.rows, nq, (int)rows, sl.gemv_scratch, st))
+    return set_error(INSPIRE_ERR_INVALID, "unsupported GEMV shape");
+  if (mfma_pack) launch_packing_mfma_prep(sl.keys, sl.y_body, (int64_t)kTExp * kN, 1, st);
   mark(1);
   // GEMV, then packing: the HBM-streaming stages run back to back (after the
   // previous request's), and the post-process waits off that critical chain.
   HIP_TRY(hipStreamWaitEvent(st, s->pack_done, 0), "wait for the previous request's GEMV and packing");
-  if (s->gemv_kind == INSPIRE_GEMV_MFMA) {
+  if (mfma_gemv && nq <= 2) {
+    if (launch_gemv_mfma_main(sl.cvec, nq, s->db, s->dbfrag, (int)rows, s->db_cols, sl.gemv_scratch, st))
+      return set_error(INSPIRE_ERR_INVALID, "unsupported GEMV shape");
+  } else if (mfma_gemv) {
     // two hash queries per MFMA tile; larger nq runs in pairs
     for (int h = 0; h < nq; h += 2) {
       const int m = nq - h < 2 ? nq - h : 2;
@@ -1131,9 +1143,8 @@ int run_pipeline(inspire_server_t* s, Slot& sl, const uint8_t* d_query, int nq, 
     return set_error(INSPIRE_ERR_INVALID, "unsupported GEMV shape");
   }
   mark(2);
-  if (s->packing_kind == INSPIRE_PACKING_MFMA) {
-    if (!s->tiled_valid) return set_error(INSPIRE_ERR_STATE, "tiled key-switch material not built");
-    if (launch_packing_mfma(sl.S, inst * 2 * kN, s->tiled, sl.keys, sl.y_body, (int64_t)kTExp * kN, 1, inst, st))
+  if (mfma_pack) {
+    if (launch_packing_mfma_main(sl.S, inst * 2 * kN, s->tiled, sl.keys, 1, inst, st))
       return set_error(INSPIRE_ERR_INVALID, "packing: shape beyond the kernels' 32-bit fragment offsets");
     HIP_TRY(hipEventRecord(s->pack_done, st), "GEMV and packing done");
     mark(3);
