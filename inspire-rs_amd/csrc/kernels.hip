@@ -4,8 +4,10 @@
 //   k_unpack_query   wire.rs:618-705   bit-unpack of the serialized query
 //   k_gemv           gemv_kernel.cu:46-103 / gpu/kernel.rs:139-237
 //   k_packing        packing_online_kernel.cu:18-92 + rotation_kernel.cu:21-59
-//   k_post           packing_online.rs:359-398 (INTT + CRT)
-//   k_fold           keyword.rs:543-581 + modulus_switch.rs:45-73
+//   k_post           packing_online.rs:359-398 (INTT + CRT; or n^-1 NTT(P) for a server's fold)
+//   k_post_ntt       packing_online_kernel.cu:70-88 + packing_online.rs:359-398 (n^-1 NTT(P) after the MFMA packing)
+//   k_ahat_ntt       the server fold's n^-1 NTT(a_hat) table
+//   k_fold           keyword.rs:543-581 + modulus_switch.rs:45-73 (k_fold_seq: the same, primes in turn)
 //   k_rotation_legacy / k_packing (legacy form) for the native/cuda shims
 #include <hip/hip_runtime.h>
 
