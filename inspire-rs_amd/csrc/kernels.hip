@@ -804,6 +804,7 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
 constexpr int kFold2Threads = 512;
 constexpr int kFoldSeqMinBlocks = 512;
 constexpr int kFold2Slots = kN / kFold2Threads;
+static_assert(kFold2Slots == 4, "a thread's slots are one radix-4 unit of the first and last passes");
 
 template <bool kNttAdd>
 __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_fold_seq(FoldArgs args) {
@@ -870,20 +871,23 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
     uint32_t res0[2][kFold2Slots];
 #pragma unroll 1  // one prime at a time keeps the NTT within 80 VGPRs (6 waves per SIMD)
     for (int c = 0; c < 2; ++c) {
-      // G^-1(sum): digit k of row r is gadget row r + 2k (gadget.rs:35-61).
-      // Each thread rewrites only its own slots, which it alone read last.
+      // G^-1(sum): digit k of row r is gadget row r + 2k (gadget.rs:35-61),
+      // and the first radix-4 pass of its transform in registers: the
+      // thread's slots tid + 512 s are that pass's unit.  Each thread rewrites
+      // only its own slots, which it alone read last.
 #pragma unroll
-      for (int s = 0; s < kFold2Slots; ++s) {
-        const int lz = lds_index(tid + s * kFold2Threads);
+      for (int p = 0; p < kDig; ++p) {
+        const int row = kNttAdd && p == 4 ? 5 : p;  // gadget row held in LDS row p (digit_row)
+        uint32_t a[kFold2Slots];
 #pragma unroll
-        for (int r = 0; r < 2; ++r)
+        for (int s = 0; s < kFold2Slots; ++s)
+          a[s] = (uint32_t)((sum[row & 1][s] >> (kGadgetBits * (row >> 1))) & mask);
+        ntt_forward_first_pass(a, c, tw);
 #pragma unroll
-          for (int k = 0; k < 3; ++k)
-            if (!kNttAdd || r + 2 * k != 4)
-              dig[digit_row<kNttAdd>(r + 2 * k)][lz] = (uint32_t)((sum[r][s] >> (kGadgetBits * k)) & mask);
+        for (int s = 0; s < kFold2Slots; ++s) dig[p][lds_index(tid + s * kFold2Threads)] = a[s];
       }
       __syncthreads();
-      ntt_forward_r4_1p<kFold2Threads, kDig>(dpolys, c, tw);
+      ntt_forward_r4_1p<kFold2Threads, kDig, 2>(dpolys, c, tw);
       const uint32_t q = prime(c);
 #pragma unroll
       for (int s = 0; s < kFold2Slots; ++s) {
@@ -924,24 +928,33 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
         dig[1][lz] = p1;
       }
       __syncthreads();
-      ntt_inverse_r4_1p_unscaled<kFold2Threads, 2>(ipolys, c, args.fold);
+      ntt_inverse_r4_1p_unscaled<kFold2Threads, 2, 3>(ipolys, c, args.fold);
+      // the last inverse pass in registers, on the thread's own slots
+      uint32_t y0[kFold2Slots], y1[kFold2Slots];
+#pragma unroll
+      for (int s = 0; s < kFold2Slots; ++s) {
+        const int lz = lds_index(tid + s * kFold2Threads);
+        y0[s] = dig[0][lz];
+        y1[s] = dig[1][lz];
+      }
+      ntt_inverse_last_pass_unscaled(y0, c, args.fold);
+      ntt_inverse_last_pass_unscaled(y1, c, args.fold);
       if (c == 0) {
 #pragma unroll
         for (int s = 0; s < kFold2Slots; ++s) {
-          const int lz = lds_index(tid + s * kFold2Threads);
-          res0[0][s] = dig[0][lz];
-          res0[1][s] = dig[1][lz];
+          res0[0][s] = y0[s];
+          res0[1][s] = y1[s];
+        }
+      } else {
+#pragma unroll
+        for (int s = 0; s < kFold2Slots; ++s) {
+          const int z = tid + s * kFold2Threads;
+          const uint64_t x0 = crt_compose(res0[0][s], y0[s]);
+          const uint64_t x1 = crt_compose(res0[1][s], y1[s]);
+          sum[0][s] = kNttAdd ? x0 : reduce_small_q(x0 + args.a_hat[j * kN + z]);
+          sum[1][s] = reduce_small_q(x1 + cv[j * kN + z] + (kNttAdd ? 0 : Pc[j * kN + z]));
         }
       }
-    }
-#pragma unroll
-    for (int s = 0; s < kFold2Slots; ++s) {
-      const int z = tid + s * kFold2Threads;
-      const int lz = lds_index(z);
-      const uint64_t x0 = crt_compose(res0[0][s], dig[0][lz]);
-      const uint64_t x1 = crt_compose(res0[1][s], dig[1][lz]);
-      sum[0][s] = kNttAdd ? x0 : reduce_small_q(x0 + args.a_hat[j * kN + z]);
-      sum[1][s] = reduce_small_q(x1 + cv[j * kN + z] + (kNttAdd ? 0 : Pc[j * kN + z]));
     }
   }
 

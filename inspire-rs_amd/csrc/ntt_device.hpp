@@ -187,15 +187,17 @@ __device__ inline void ntt_inverse_r4(uint32_t* const (&polys)[NP], const int (&
 }
 
 // Single-prime forms: NP polynomials, all over prime c (twiddles of c only).
-// Same transforms, layouts and sync contract as the forms above.
-template <int NT, int NP>
+// Same transforms, layouts and sync contract as the forms above.  With
+// kFirstMM = 2 the forward transform starts at its second radix-4 pass: the
+// caller ran the first one (ntt_forward_first_pass) and synced.
+template <int NT, int NP, int kFirstMM = 0>
 __device__ inline void ntt_forward_r4_1p(uint32_t* const (&polys)[NP], int c, const NttTables& tw) {
   const int tid = threadIdx.x;
   const uint32_t q = prime(c), q2 = 2 * q;
   const uint32_t* fw = tw.fw(c);
   const uint32_t* fwp = tw.fwp(c);
   auto bf = [q](uint32_t& x, uint32_t& y, uint32_t w, uint32_t wp) { fwd_bf(x, y, w, wp, q); };
-  for (int mm = 0; mm + 1 < kLogN; mm += 2) {
+  for (int mm = kFirstMM; mm + 1 < kLogN; mm += 2) {
     const int m = 1 << mm, t = (kN / 2) >> mm, th = t >> 1;
     for (int u = tid; u < kN / 4; u += NT) {
       int i, j;
@@ -320,7 +322,7 @@ __device__ inline void inv_bf_unscaled(uint32_t& x, uint32_t& y, uint32_t w, uin
   y = w * tt - mulhi32(tt, wp) * q;
 }
 
-template <int NT, int NP, bool OnePrime>
+template <int NT, int NP, bool OnePrime, int kLastMM = 1>
 __device__ inline void ntt_inverse_unscaled_impl(uint32_t* const (&polys)[NP], const int* prime_of, int c1,
                                                  const uint32_t* __restrict__ fold_tab) {
   const int tid = threadIdx.x;
@@ -339,7 +341,7 @@ __device__ inline void ntt_inverse_unscaled_impl(uint32_t* const (&polys)[NP], c
     }
   }
   __syncthreads();
-  for (int mm = kLogN - 2; mm >= 1; mm -= 2) {
+  for (int mm = kLogN - 2; mm >= kLastMM; mm -= 2) {
     const int h = 1 << mm, t = (kN / 2) >> mm, ni = (kN / 4) / t;
     for (int u = tid; u < kN / 4; u += NT) {
       int i, j;
@@ -538,11 +540,43 @@ __device__ inline uint32_t third_gadget_digit_ntt(uint32_t S, uint32_t g0, uint3
   return x >= q ? x - q : x;
 }
 
-// NP polynomials over prime c.
-template <int NT, int NP>
+// NP polynomials over prime c.  With kLastMM = 3 the transform stops before
+// its last radix-4 pass (after a sync); the caller runs that pass on its own
+// units (ntt_inverse_last_pass_unscaled).
+template <int NT, int NP, int kLastMM = 1>
 __device__ inline void ntt_inverse_r4_1p_unscaled(uint32_t* const (&polys)[NP], int c,
                                                   const uint32_t* __restrict__ fold_tab) {
-  ntt_inverse_unscaled_impl<NT, NP, true>(polys, nullptr, c, fold_tab);
+  ntt_inverse_unscaled_impl<NT, NP, true, kLastMM>(polys, nullptr, c, fold_tab);
+}
+
+// The first radix-4 pass of the forward transform (stages 0 and 1) on one
+// unit in registers: coefficients u, u + 512, u + 1024, u + 1536 of a
+// polynomial over prime c, i.e. the unit of thread u in a 512-thread block.
+__device__ inline void ntt_forward_first_pass(uint32_t (&a)[4], int c, const NttTables& tw) {
+  const uint32_t q = prime(c);
+  const uint32_t* fw = tw.fw(c);
+  const uint32_t* fwp = tw.fwp(c);
+  fwd_bf(a[0], a[2], fw[1], fwp[1], q);
+  fwd_bf(a[1], a[3], fw[1], fwp[1], q);
+  fwd_bf(a[0], a[1], fw[2], fwp[2], q);
+  fwd_bf(a[2], a[3], fw[3], fwp[3], q);
+}
+
+// The last radix-4 pass of the unscaled inverse (stages 1 and 0) on the same
+// unit in registers; outputs canonical.
+__device__ inline void ntt_inverse_last_pass_unscaled(uint32_t (&a)[4], int c, const uint32_t* __restrict__ fold_tab) {
+  const uint32_t q = prime(c), q2 = 2 * q;
+  const uint32_t* iw = fold_tab + c * 3 * kN;
+  const uint32_t* iwp = iw + kN;
+  inv_bf_unscaled(a[0], a[1], iw[2], iwp[2], q);
+  inv_bf_unscaled(a[2], a[3], iw[3], iwp[3], q);
+  inv_bf_unscaled(a[0], a[2], iw[1], iwp[1], q);
+  inv_bf_unscaled(a[1], a[3], iw[1], iwp[1], q);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    a[i] = a[i] >= q2 ? a[i] - q2 : a[i];
+    a[i] = a[i] >= q ? a[i] - q : a[i];
+  }
 }
 
 // NP polynomials with per-polynomial primes.
