@@ -823,9 +823,11 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
   const uint32_t* P32 = reinterpret_cast<const uint32_t*>(Pc);  // server form: n^-1 NTT_c(P_j) rows
   const uint32_t* body = args.bodies + ((size_t)b * args.nq + h) * 6 * 2 * kN;
 
-  // Server form: n^-1 NTT_c(sum row 0) at the thread's slots, prime 0's in
-  // S0 and prime 1's in LDS row 5, which this form does not transform (each
-  // thread reads and writes only its own slots there).
+  // sum[r][s] for slots z = tid + 512 s.  The product stage works on the slot
+  // pairs (2v, 2v + 1), v = tid + 512 h, instead; in the server form S0 holds
+  // n^-1 NTT_0(sum row 0) at those slots (index 2h + e for slot 2v + e), and
+  // prime 1's value sits in LDS row 5, which this form does not transform
+  // (each thread reads and writes only its own slots there).
   uint64_t sum[2][kFold2Slots];
   uint32_t S0[kFold2Slots];
   {
@@ -836,7 +838,8 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
       sum[0][s] = args.a_hat[j * kN + z];
       sum[1][s] = kNttAdd ? cv[j * kN + z] : reduce_small_q(cv[j * kN + z] + Pc[j * kN + z]);
       if constexpr (kNttAdd) {
-        S0[s] = args.ahat_ntt[(j * 2 + 0) * kN + z];
+        const int zp = 2 * (tid + (s >> 1) * kFold2Threads) + (s & 1);  // the pair slot of index s
+        S0[s] = args.ahat_ntt[(j * 2 + 0) * kN + zp];
         dig[5][lds_index(z)] = args.ahat_ntt[(j * 2 + 1) * kN + z];
       }
     }
@@ -887,48 +890,102 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
         for (int s = 0; s < kFold2Slots; ++s) dig[p][lds_index(tid + s * kFold2Threads)] = a[s];
       }
       __syncthreads();
-      ntt_forward_r4_1p<kFold2Threads, kDig, 2>(dpolys, c, tw);
-      const uint32_t q = prime(c);
+      ntt_forward_r4_1p<kFold2Threads, kDig, 2, false>(dpolys, c, tw);
+      // The last forward stage, the products (ct_gsw * G: row 0 = NTT(-a) *
+      // sum_k G_k, row 1 = sum_k body_k * G_k) and the first inverse stage all
+      // work on slot pairs (2v, 2v + 1): each thread runs them for its pairs
+      // v = tid + 512 h in registers.
+      const uint32_t q = prime(c), q2 = 2 * q;
+      const uint32_t* fw = tw.fw(c);
+      const uint32_t* fwp = tw.fwp(c);
+      const uint32_t* iw = args.fold + c * 3 * kN;
+      const uint32_t* iwp = iw + kN;
+#pragma unroll 1  // one pair at a time keeps its loads and sums within the register budget
+      for (int hh = 0; hh < 2; ++hh) {
+        const int v = tid + hh * kFold2Threads;
+        const int l0 = lds_index(2 * v), l1 = lds_index(2 * v + 1);
+        // Each transformed digit row goes straight into the two slots' sums;
+        // rows 0 and 2 stay for the derived row 4 (server form).  The pair's
+        // per-slot operands are 8-byte loads.
+        uint64_t acc[2] = {0, 0};
+        uint32_t gsum[2] = {0, 0}, g0[2] = {0, 0}, g2[2] = {0, 0};
+        auto add_row = [&](int row, uint32_t x, uint32_t y) {
+          const uint2 bv = *reinterpret_cast<const uint2*>(body + (row * 2 + c) * kN + 2 * v);
+          gsum[0] += x;
+          gsum[1] += y;
+          acc[0] += (uint64_t)bv.x * x;
+          acc[1] += (uint64_t)bv.y * y;
+        };
 #pragma unroll
-      for (int s = 0; s < kFold2Slots; ++s) {
-        const int z = tid + s * kFold2Threads;
-        const int lz = lds_index(z);
-        uint32_t g[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k)
-          if (!kNttAdd || k != 4) g[k] = dig[digit_row<kNttAdd>(k)][lz];
-        if constexpr (kNttAdd) g[4] = third_gadget_digit_ntt<kLogN>(c == 0 ? S0[s] : dig[5][lz], g[0], g[2], c);
-        uint32_t gsum = 0;
-        uint64_t acc = 0;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-          gsum += g[k];
-          acc += (uint64_t)body[(k * 2 + c) * kN + z] * g[k];
+        for (int p = 0; p < kDig; ++p) {
+          uint32_t x = dig[p][l0], y = dig[p][l1];
+          fwd_bf(x, y, fw[1024 + v], fwp[1024 + v], q);
+          x = x >= q2 ? x - q2 : x;
+          x = x >= q ? x - q : x;
+          y = y >= q2 ? y - q2 : y;
+          y = y >= q ? y - q : y;
+          const int row = kNttAdd && p == 4 ? 5 : p;  // gadget row held in LDS row p (digit_row)
+          add_row(row, x, y);
+          if (row == 0) {
+            g0[0] = x;
+            g0[1] = y;
+          } else if (row == 2) {
+            g2[0] = x;
+            g2[1] = y;
+          }
+        }
+        if constexpr (kNttAdd) {
+          // constant indices into S0 (hh is 0 or 1) keep it in registers
+          const uint32_t s0 = c == 0 ? (hh ? S0[2] : S0[0]) : dig[5][l0];
+          const uint32_t s1 = c == 0 ? (hh ? S0[3] : S0[1]) : dig[5][l1];
+          add_row(4, third_gadget_digit_ntt<kLogN>(s0, g0[0], g2[0], c),
+                  third_gadget_digit_ntt<kLogN>(s1, g0[1], g2[1], c));
         }
         // both rows carry n^-1 (see k_fold)
-        // six canonical NTT values: gsum < 6q
-        uint32_t gs = gsum >= 4 * q ? gsum - 4 * q : gsum;
-        gs = gs >= 2 * q ? gs - 2 * q : gs;
-        gs = gs >= q ? gs - q : gs;
-        const uint32_t na = args.fold[(c * 3 + 2) * kN + z];
-        const uint32_t r1 = (uint32_t)barrett64(acc, cr1(c), q);
-        uint32_t p0 = c == 0 ? mulmod<0>(na, gs) : mulmod<1>(na, gs);
-        uint32_t p1 = c == 0 ? mulmod<0>(r1, kNInvQ0) : mulmod<1>(r1, kNInvQ1);
-        if constexpr (kNttAdd) {
-          p0 += args.ahat_ntt[(j * 2 + c) * kN + z];
-          p0 = p0 >= q ? p0 - q : p0;
-          if (c == 0)
-            S0[s] = p0;
-          else
-            dig[5][lz] = p0;
-          p1 += P32[(j * 2 + c) * kN + z];
-          p1 = p1 >= q ? p1 - q : p1;
+        const uint2 na = *reinterpret_cast<const uint2*>(args.fold + (c * 3 + 2) * kN + 2 * v);
+        uint32_t p0[2], p1[2];
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          // six canonical NTT values: gsum < 6q
+          uint32_t gs = gsum[e] >= 4 * q ? gsum[e] - 4 * q : gsum[e];
+          gs = gs >= 2 * q ? gs - 2 * q : gs;
+          gs = gs >= q ? gs - q : gs;
+          const uint32_t r1 = (uint32_t)barrett64(acc[e], cr1(c), q);
+          p0[e] = c == 0 ? mulmod<0>(e ? na.y : na.x, gs) : mulmod<1>(e ? na.y : na.x, gs);
+          p1[e] = c == 0 ? mulmod<0>(r1, kNInvQ0) : mulmod<1>(r1, kNInvQ1);
         }
-        dig[0][lz] = p0;
-        dig[1][lz] = p1;
+        if constexpr (kNttAdd) {
+          const uint2 ah = *reinterpret_cast<const uint2*>(args.ahat_ntt + (j * 2 + c) * kN + 2 * v);
+          const uint2 pn = *reinterpret_cast<const uint2*>(P32 + (j * 2 + c) * kN + 2 * v);
+          p0[0] += ah.x;
+          p0[1] += ah.y;
+          p1[0] += pn.x;
+          p1[1] += pn.y;
+#pragma unroll
+          for (int e = 0; e < 2; ++e) {
+            p0[e] = p0[e] >= q ? p0[e] - q : p0[e];
+            p1[e] = p1[e] >= q ? p1[e] - q : p1[e];
+          }
+          if (c == 0 && hh == 0) {
+            S0[0] = p0[0];
+            S0[1] = p0[1];
+          } else if (c == 0) {
+            S0[2] = p0[0];
+            S0[3] = p0[1];
+          } else {
+            dig[5][l0] = p0[0];
+            dig[5][l1] = p0[1];
+          }
+        }
+        inv_bf_unscaled(p0[0], p0[1], iw[1024 + v], iwp[1024 + v], q);
+        inv_bf_unscaled(p1[0], p1[1], iw[1024 + v], iwp[1024 + v], q);
+        dig[0][l0] = p0[0];
+        dig[0][l1] = p0[1];
+        dig[1][l0] = p1[0];
+        dig[1][l1] = p1[1];
       }
       __syncthreads();
-      ntt_inverse_r4_1p_unscaled<kFold2Threads, 2, 3>(ipolys, c, args.fold);
+      ntt_inverse_r4_1p_unscaled<kFold2Threads, 2, 3, false>(ipolys, c, args.fold);
       // the last inverse pass in registers, on the thread's own slots
       uint32_t y0[kFold2Slots], y1[kFold2Slots];
 #pragma unroll

@@ -189,8 +189,10 @@ __device__ inline void ntt_inverse_r4(uint32_t* const (&polys)[NP], const int (&
 // Single-prime forms: NP polynomials, all over prime c (twiddles of c only).
 // Same transforms, layouts and sync contract as the forms above.  With
 // kFirstMM = 2 the forward transform starts at its second radix-4 pass: the
-// caller ran the first one (ntt_forward_first_pass) and synced.
-template <int NT, int NP, int kFirstMM = 0>
+// caller ran the first one (ntt_forward_first_pass) and synced.  Without
+// kLastStage it returns (after a sync) before the last, radix-2 stage and
+// its final reduction, which the caller runs on its own slot pairs.
+template <int NT, int NP, int kFirstMM = 0, bool kLastStage = true>
 __device__ inline void ntt_forward_r4_1p(uint32_t* const (&polys)[NP], int c, const NttTables& tw) {
   const int tid = threadIdx.x;
   const uint32_t q = prime(c), q2 = 2 * q;
@@ -229,6 +231,7 @@ __device__ inline void ntt_forward_r4_1p(uint32_t* const (&polys)[NP], int c, co
     }
     __syncthreads();
   }
+  if constexpr (!kLastStage) return;
   for (int u = tid; u < kN / 2; u += NT) {
     const uint32_t w = fw[1024 + u], wp = fwp[1024 + u];
     const int l0 = lds_index(2 * u), l1 = lds_index(2 * u + 1);
@@ -322,12 +325,12 @@ __device__ inline void inv_bf_unscaled(uint32_t& x, uint32_t& y, uint32_t w, uin
   y = w * tt - mulhi32(tt, wp) * q;
 }
 
-template <int NT, int NP, bool OnePrime, int kLastMM = 1>
+template <int NT, int NP, bool OnePrime, int kLastMM = 1, bool kFirstStage = true>
 __device__ inline void ntt_inverse_unscaled_impl(uint32_t* const (&polys)[NP], const int* prime_of, int c1,
                                                  const uint32_t* __restrict__ fold_tab) {
   const int tid = threadIdx.x;
   auto prime_at = [&](int p) { return OnePrime ? c1 : prime_of[p]; };
-  for (int u = tid; u < kN / 2; u += NT) {
+  for (int u = tid; kFirstStage && u < kN / 2; u += NT) {
     const int l0 = lds_index(2 * u), l1 = lds_index(2 * u + 1);
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
@@ -340,7 +343,7 @@ __device__ inline void ntt_inverse_unscaled_impl(uint32_t* const (&polys)[NP], c
       a[l1] = y;
     }
   }
-  __syncthreads();
+  if constexpr (kFirstStage) __syncthreads();
   for (int mm = kLogN - 2; mm >= kLastMM; mm -= 2) {
     const int h = 1 << mm, t = (kN / 2) >> mm, ni = (kN / 4) / t;
     for (int u = tid; u < kN / 4; u += NT) {
@@ -542,11 +545,13 @@ __device__ inline uint32_t third_gadget_digit_ntt(uint32_t S, uint32_t g0, uint3
 
 // NP polynomials over prime c.  With kLastMM = 3 the transform stops before
 // its last radix-4 pass (after a sync); the caller runs that pass on its own
-// units (ntt_inverse_last_pass_unscaled).
-template <int NT, int NP, int kLastMM = 1>
+// units (ntt_inverse_last_pass_unscaled).  Without kFirstStage it starts at
+// its first radix-4 pass: the caller ran the first, radix-2 stage on its slot
+// pairs and synced.
+template <int NT, int NP, int kLastMM = 1, bool kFirstStage = true>
 __device__ inline void ntt_inverse_r4_1p_unscaled(uint32_t* const (&polys)[NP], int c,
                                                   const uint32_t* __restrict__ fold_tab) {
-  ntt_inverse_unscaled_impl<NT, NP, true, kLastMM>(polys, nullptr, c, fold_tab);
+  ntt_inverse_unscaled_impl<NT, NP, true, kLastMM, kFirstStage>(polys, nullptr, c, fold_tab);
 }
 
 // The first radix-4 pass of the forward transform (stages 0 and 1) on one
