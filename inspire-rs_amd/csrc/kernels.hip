@@ -891,6 +891,8 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
       }
       __syncthreads();
       ntt_forward_r4_1p<kFold2Threads, kDig, 2, false>(dpolys, c, tw);
+      const uint32_t* ah_row = args.ahat_ntt + (j * 2 + c) * kN;  // server form: this step's table row
+      const uint32_t* pn_row = P32 + (j * 2 + c) * kN;            // and P row
       // The last forward stage, the products (ct_gsw * G: row 0 = NTT(-a) *
       // sum_k G_k, row 1 = sum_k body_k * G_k) and the first inverse stage all
       // work on slot pairs (2v, 2v + 1): each thread runs them for its pairs
@@ -904,13 +906,17 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
       for (int hh = 0; hh < 2; ++hh) {
         const int v = tid + hh * kFold2Threads;
         const int l0 = lds_index(2 * v), l1 = lds_index(2 * v + 1);
+        // wave-uniform row bases plus a 32-bit lane offset, opaque so that the
+        // compiler does not hoist 64-bit lane addresses out of the loops
+        uint32_t off = 2u * (uint32_t)v;
+        asm volatile("" : "+v"(off));
         // Each transformed digit row goes straight into the two slots' sums;
         // rows 0 and 2 stay for the derived row 4 (server form).  The pair's
         // per-slot operands are 8-byte loads.
         uint64_t acc[2] = {0, 0};
         uint32_t gsum[2] = {0, 0}, g0[2] = {0, 0}, g2[2] = {0, 0};
         auto add_row = [&](int row, uint32_t x, uint32_t y) {
-          const uint2 bv = *reinterpret_cast<const uint2*>(body + (row * 2 + c) * kN + 2 * v);
+          const uint2 bv = *reinterpret_cast<const uint2*>(body + (row * 2 + c) * kN + off);
           gsum[0] += x;
           gsum[1] += y;
           acc[0] += (uint64_t)bv.x * x;
@@ -942,7 +948,7 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
                   third_gadget_digit_ntt<kLogN>(s1, g0[1], g2[1], c));
         }
         // both rows carry n^-1 (see k_fold)
-        const uint2 na = *reinterpret_cast<const uint2*>(args.fold + (c * 3 + 2) * kN + 2 * v);
+        const uint2 na = *reinterpret_cast<const uint2*>(args.fold + (c * 3 + 2) * kN + off);
         uint32_t p0[2], p1[2];
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
@@ -955,8 +961,8 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
           p1[e] = c == 0 ? mulmod<0>(r1, kNInvQ0) : mulmod<1>(r1, kNInvQ1);
         }
         if constexpr (kNttAdd) {
-          const uint2 ah = *reinterpret_cast<const uint2*>(args.ahat_ntt + (j * 2 + c) * kN + 2 * v);
-          const uint2 pn = *reinterpret_cast<const uint2*>(P32 + (j * 2 + c) * kN + 2 * v);
+          const uint2 ah = *reinterpret_cast<const uint2*>(ah_row + off);
+          const uint2 pn = *reinterpret_cast<const uint2*>(pn_row + off);
           p0[0] += ah.x;
           p0[1] += ah.y;
           p1[0] += pn.x;
@@ -1003,13 +1009,17 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
           res0[1][s] = y1[s];
         }
       } else {
+        const uint64_t* cv_row = cv + j * kN;
+        const uint64_t* ahat_row = args.a_hat + j * kN;
+        const uint64_t* p_row = Pc + j * kN;
 #pragma unroll
         for (int s = 0; s < kFold2Slots; ++s) {
-          const int z = tid + s * kFold2Threads;
+          uint32_t z = (uint32_t)(tid + s * kFold2Threads);
+          asm volatile("" : "+v"(z));  // as off above
           const uint64_t x0 = crt_compose(res0[0][s], y0[s]);
           const uint64_t x1 = crt_compose(res0[1][s], y1[s]);
-          sum[0][s] = kNttAdd ? x0 : reduce_small_q(x0 + args.a_hat[j * kN + z]);
-          sum[1][s] = reduce_small_q(x1 + cv[j * kN + z] + (kNttAdd ? 0 : Pc[j * kN + z]));
+          sum[0][s] = kNttAdd ? x0 : reduce_small_q(x0 + ahat_row[z]);
+          sum[1][s] = reduce_small_q(x1 + cv_row[z] + (kNttAdd ? 0 : p_row[z]));
         }
       }
     }
