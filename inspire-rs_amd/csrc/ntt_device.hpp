@@ -3,9 +3,12 @@
 // Same transform and slot order as core/ntt.rs (forward: Cooley-Tukey with the
 // bit-reversed power table of the minimal primitive 2n-th root, so slot k holds
 // the evaluation at psi^(2*rev(k)+1); inverse: Gentleman-Sande with the
-// halved inverse powers, so the result is scaled by n^-1).  Values stay in
-// [0, 4q) during the forward pass and [0, 2q) during the inverse pass, as in
-// Harvey's lazy butterflies; both passes end with a canonical reduction.
+// halved inverse powers, so the result is scaled by n^-1).  The inverse pass
+// keeps values in [0, 2q), as in Harvey's lazy butterflies.  The forward pass
+// is lazier still (fwd_stage_reduction): from inputs below 2q it skips the
+// reduction of x in stages 1-7, where values stay below 16q < 2^32, restores
+// x < 2q in stage 8 and runs Harvey's [0, 4q) butterflies after that.  Both
+// passes end with a canonical reduction.
 //
 // Radix-4: two butterfly stages per pass with the four values of a unit in
 // registers.  LDS layout: coefficient x of a polynomial lives at word
@@ -15,6 +18,8 @@
 // coefficients are 4*stride apart: with the pad every 32 words both mappings
 // put the 32 lanes of a half-wave on 32 different banks.
 #pragma once
+#include <type_traits>
+
 #include "modarith.hpp"
 
 namespace inspire {
@@ -25,15 +30,49 @@ __host__ __device__ constexpr int lds_index(int x) { return x + (x >> 5); }
 
 // Forward Cooley-Tukey butterfly with Harvey's lazy reduction: x, y in [0, 4q)
 // -> x + w y, x - w y + 2q in [0, 4q), wp = floor(w 2^32 / q).  The Shoup
-// product is formed negated, nqn = qt q - w y = -(w y mod q in [0, 2q)) mod
-// 2^32, so that y' = xr + 2q + nqn is one three-input add (8 VALU operations
-// per butterfly instead of 9).
-__device__ inline void fwd_bf(uint32_t& x, uint32_t& y, uint32_t w, uint32_t wp, uint32_t q) {
-  const uint32_t q2 = 2 * q;
-  const uint32_t xr = x >= q2 ? x - q2 : x;
+// product w y mod q lies in [0, 2q) for any y < 2^32.  It is formed negated,
+// nqn = qt q - w y = -(w y mod q) mod 2^32, so that y' = xr + 2q + nqn is one
+// three-input add (8 VALU operations per butterfly instead of 9).
+//
+// kRed sets how x is reduced first: 1 is Harvey's (x < 4q -> x < 2q); 0 skips
+// it, so the outputs exceed x's bound by at most 2q (6 operations); 3
+// subtracts 8q, 4q and 2q conditionally, taking any x < 16q below 2q.
+template <int kRed>
+__device__ inline void fwd_bf_r(uint32_t& x, uint32_t& y, uint32_t w, uint32_t wp, uint32_t q) {
+  static_assert(kRed == 0 || kRed == 1 || kRed == 3, "reduction of x: none, 2q, or 8q, 4q, 2q");
+  uint32_t xr = x;
+  if constexpr (kRed == 3) {
+    xr = xr >= 8 * q ? xr - 8 * q : xr;
+    xr = xr >= 4 * q ? xr - 4 * q : xr;
+  }
+  if constexpr (kRed != 0) xr = xr >= 2 * q ? xr - 2 * q : xr;
   const uint32_t nqn = mulhi32(y, wp) * q - w * y;
   x = xr - nqn;
-  y = xr + q2 + nqn;
+  y = xr + 2 * q + nqn;
+}
+
+__device__ inline void fwd_bf(uint32_t& x, uint32_t& y, uint32_t w, uint32_t wp, uint32_t q) {
+  fwd_bf_r<1>(x, y, w, wp, q);
+}
+
+template <int V>
+using IntC = std::integral_constant<int, V>;
+
+// The reductions of a forward radix-4 pass over stages (mm + 1, mm + 2), 1-based,
+// for inputs below 2q: stage s leaves values below 2q + 2sq without reducing x,
+// so stages 1-7 skip it (below 16q after stage 7: 16 q0 = 4,293,918,736 <
+// 2^32, and q1 < q0), stage 8 takes x from below 16q to below 2q, and stages
+// 9-11 are Harvey's.  Calls f(IntC<kRed of stage mm + 1>, IntC<kRed of stage
+// mm + 2>); mm is even and uniform across the workgroup.
+template <class F>
+__device__ inline void fwd_stage_reduction(int mm, F&& f) {
+  static_assert(16ull * kQ0 < (1ull << 32) && kQ1 < kQ0, "stages 1-7 stay below 2^32");
+  if (mm <= 4)
+    f(IntC<0>{}, IntC<0>{});
+  else if (mm == 6)
+    f(IntC<0>{}, IntC<3>{});
+  else
+    f(IntC<1>{}, IntC<1>{});
 }
 
 // Twiddle tables per prime: [fwd w][fwd w'][inv w][inv w'] x 2048, u32.
@@ -76,21 +115,24 @@ __device__ inline void ntt_forward_r4m(uint32_t* const (&polys)[NP], const int (
         w[c][2] = tw.fw(c)[2 * m + 2 * i + 1];
         wp[c][2] = tw.fwp(c)[2 * m + 2 * i + 1];
       }
+      fwd_stage_reduction(mm, [&](auto r1, auto r2) {
+        constexpr int kR1 = decltype(r1)::value, kR2 = decltype(r2)::value;
 #pragma unroll
-      for (int p = 0; p < NP; ++p) {
-        const int c = prime_of[p];
-        const uint32_t q = prime(c);
-        uint32_t* a = polys[p];
-        uint32_t a0 = a[l0], a1 = a[l1], a2 = a[l2], a3 = a[l3];
-        bf(a0, a2, w[c][0], wp[c][0], q);
-        bf(a1, a3, w[c][0], wp[c][0], q);
-        bf(a0, a1, w[c][1], wp[c][1], q);
-        bf(a2, a3, w[c][2], wp[c][2], q);
-        a[l0] = a0;
-        a[l1] = a1;
-        a[l2] = a2;
-        a[l3] = a3;
-      }
+        for (int p = 0; p < NP; ++p) {
+          const int c = prime_of[p];
+          const uint32_t q = prime(c);
+          uint32_t* a = polys[p];
+          uint32_t a0 = a[l0], a1 = a[l1], a2 = a[l2], a3 = a[l3];
+          fwd_bf_r<kR1>(a0, a2, w[c][0], wp[c][0], q);
+          fwd_bf_r<kR1>(a1, a3, w[c][0], wp[c][0], q);
+          fwd_bf_r<kR2>(a0, a1, w[c][1], wp[c][1], q);
+          fwd_bf_r<kR2>(a2, a3, w[c][2], wp[c][2], q);
+          a[l0] = a0;
+          a[l1] = a1;
+          a[l2] = a2;
+          a[l3] = a3;
+        }
+      });
     }
     __syncthreads();
   }
@@ -215,19 +257,22 @@ __device__ inline void ntt_forward_r4_1p(uint32_t* const (&polys)[NP], int c, co
       const uint32_t w0 = fw[m + i], wp0 = fwp[m + i];
       const uint32_t w1 = fw[2 * m + 2 * i], wp1 = fwp[2 * m + 2 * i];
       const uint32_t w2 = fw[2 * m + 2 * i + 1], wp2 = fwp[2 * m + 2 * i + 1];
+      fwd_stage_reduction(mm, [&](auto r1, auto r2) {
+        constexpr int kR1 = decltype(r1)::value, kR2 = decltype(r2)::value;
 #pragma unroll
-      for (int p = 0; p < NP; ++p) {
-        uint32_t* a = polys[p];
-        uint32_t a0 = a[l0], a1 = a[l1], a2 = a[l2], a3 = a[l3];
-        bf(a0, a2, w0, wp0);
-        bf(a1, a3, w0, wp0);
-        bf(a0, a1, w1, wp1);
-        bf(a2, a3, w2, wp2);
-        a[l0] = a0;
-        a[l1] = a1;
-        a[l2] = a2;
-        a[l3] = a3;
-      }
+        for (int p = 0; p < NP; ++p) {
+          uint32_t* a = polys[p];
+          uint32_t a0 = a[l0], a1 = a[l1], a2 = a[l2], a3 = a[l3];
+          fwd_bf_r<kR1>(a0, a2, w0, wp0, q);
+          fwd_bf_r<kR1>(a1, a3, w0, wp0, q);
+          fwd_bf_r<kR2>(a0, a1, w1, wp1, q);
+          fwd_bf_r<kR2>(a2, a3, w2, wp2, q);
+          a[l0] = a0;
+          a[l1] = a1;
+          a[l2] = a2;
+          a[l3] = a3;
+        }
+      });
     }
     __syncthreads();
   }
@@ -554,17 +599,19 @@ __device__ inline void ntt_inverse_r4_1p_unscaled(uint32_t* const (&polys)[NP], 
   ntt_inverse_unscaled_impl<NT, NP, true, kLastMM, kFirstStage>(polys, nullptr, c, fold_tab);
 }
 
-// The first radix-4 pass of the forward transform (stages 0 and 1) on one
-// unit in registers: coefficients u, u + 512, u + 1024, u + 1536 of a
-// polynomial over prime c, i.e. the unit of thread u in a 512-thread block.
+// The first radix-4 pass of the forward transform (stages 1 and 2 in the
+// numbering of fwd_stage_reduction, inputs below 2q) on one unit in registers:
+// coefficients u, u + 512, u + 1024, u + 1536 of a polynomial over prime c,
+// i.e. the unit of thread u in a 512-thread block.
 __device__ inline void ntt_forward_first_pass(uint32_t (&a)[4], int c, const NttTables& tw) {
   const uint32_t q = prime(c);
   const uint32_t* fw = tw.fw(c);
   const uint32_t* fwp = tw.fwp(c);
-  fwd_bf(a[0], a[2], fw[1], fwp[1], q);
-  fwd_bf(a[1], a[3], fw[1], fwp[1], q);
-  fwd_bf(a[0], a[1], fw[2], fwp[2], q);
-  fwd_bf(a[2], a[3], fw[3], fwp[3], q);
+  // fwd_stage_reduction(0, ..): no reduction of x
+  fwd_bf_r<0>(a[0], a[2], fw[1], fwp[1], q);
+  fwd_bf_r<0>(a[1], a[3], fw[1], fwp[1], q);
+  fwd_bf_r<0>(a[0], a[1], fw[2], fwp[2], q);
+  fwd_bf_r<0>(a[2], a[3], fw[3], fwp[3], q);
 }
 
 // The last radix-4 pass of the unscaled inverse (stages 1 and 0) on the same
