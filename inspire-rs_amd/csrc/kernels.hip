@@ -805,6 +805,10 @@ constexpr int kFold2Threads = 512;
 constexpr int kFoldSeqMinBlocks = 512;
 constexpr int kFold2Slots = kN / kFold2Threads;
 static_assert(kFold2Slots == 4, "a thread's slots are one radix-4 unit of the first and last passes");
+static_assert(12ull * kQ0 < (1ull << 32) && 4ull * kQ0 * kQ0 < (1ull << 58) && kQ1 < kQ0,
+              "pair stage bounds: six digits below 2q sum below 2^32; q times 4q fits reduce58");
+static_assert(12ull * kQ0 * kQ0 < (1ull << 60) && (1ull << 56) + (1ull << 32) < (1ull << 57),
+              "the row-1 sum splits into hi < 2^28 and lo, and hi (2^32 mod q) + lo < 2^57");
 
 template <bool kNttAdd>
 __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_fold_seq(FoldArgs args) {
@@ -897,7 +901,8 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
       // sum_k G_k, row 1 = sum_k body_k * G_k) and the first inverse stage all
       // work on slot pairs (2v, 2v + 1): each thread runs them for its pairs
       // v = tid + 512 h in registers.
-      const uint32_t q = prime(c), q2 = 2 * q;
+      const uint32_t q = prime(c), q2 = 2 * q, mu = mu58(c);
+      const uint32_t c32 = c ? kC32Q1 : kC32Q0, ninv = c ? kNInvQ1 : kNInvQ0;
       const uint32_t* fw = tw.fw(c);
       const uint32_t* fwp = tw.fwp(c);
       const uint32_t* iw = args.fold + c * 3 * kN;
@@ -922,14 +927,14 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
           acc[0] += (uint64_t)bv.x * x;
           acc[1] += (uint64_t)bv.y * y;
         };
+        // The products take the transformed digits below 2q, not canonical:
+        // six of them sum below 12q < 2^32, and acc stays below 12 q^2 < 2^60.
 #pragma unroll
         for (int p = 0; p < kDig; ++p) {
           uint32_t x = dig[p][l0], y = dig[p][l1];
           fwd_bf(x, y, fw[1024 + v], fwp[1024 + v], q);
           x = x >= q2 ? x - q2 : x;
-          x = x >= q ? x - q : x;
           y = y >= q2 ? y - q2 : y;
-          y = y >= q ? y - q : y;
           const int row = kNttAdd && p == 4 ? 5 : p;  // gadget row held in LDS row p (digit_row)
           add_row(row, x, y);
           if (row == 0) {
@@ -944,34 +949,35 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
           // constant indices into S0 (hh is 0 or 1) keep it in registers
           const uint32_t s0 = c == 0 ? (hh ? S0[2] : S0[0]) : dig[5][l0];
           const uint32_t s1 = c == 0 ? (hh ? S0[3] : S0[1]) : dig[5][l1];
-          add_row(4, third_gadget_digit_ntt<kLogN>(s0, g0[0], g2[0], c),
-                  third_gadget_digit_ntt<kLogN>(s1, g0[1], g2[1], c));
+          add_row(4, third_gadget_digit_ntt<kLogN, false>(s0, g0[0], g2[0], c),
+                  third_gadget_digit_ntt<kLogN, false>(s1, g0[1], g2[1], c));
         }
         // both rows carry n^-1 (see k_fold)
         const uint2 na = *reinterpret_cast<const uint2*>(args.fold + (c * 3 + 2) * kN + off);
         uint32_t p0[2], p1[2];
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
-          // six canonical NTT values: gsum < 6q
-          uint32_t gs = gsum[e] >= 4 * q ? gsum[e] - 4 * q : gsum[e];
-          gs = gs >= 2 * q ? gs - 2 * q : gs;
-          gs = gs >= q ? gs - q : gs;
-          const uint32_t r1 = (uint32_t)barrett64(acc[e], cr1(c), q);
-          p0[e] = c == 0 ? mulmod<0>(e ? na.y : na.x, gs) : mulmod<1>(e ? na.y : na.x, gs);
-          p1[e] = c == 0 ? mulmod<0>(r1, kNInvQ0) : mulmod<1>(r1, kNInvQ1);
+          // six values below 2q: gsum < 12q; below 4q, the product with
+          // na < q stays below 4 q^2 < 2^58, the range of reduce58
+          uint32_t gs = gsum[e] >= 8 * q ? gsum[e] - 8 * q : gsum[e];
+          gs = gs >= 4 * q ? gs - 4 * q : gs;
+          // acc < 12 q^2 < 2^60 is 2^32 hi + lo with hi < 2^28, so hi (2^32
+          // mod q) + lo < 2^57; its residue below 3q times n^-1 < q stays
+          // below 2^58
+          const uint32_t r1 =
+              reduce58_lazy_rt((uint64_t)(uint32_t)(acc[e] >> 32) * c32 + (uint32_t)acc[e], q, mu);
+          p0[e] = reduce58_rt((uint64_t)(e ? na.y : na.x) * gs, q, mu);
+          p1[e] = reduce58_rt((uint64_t)r1 * ninv, q, mu);
         }
         if constexpr (kNttAdd) {
           const uint2 ah = *reinterpret_cast<const uint2*>(ah_row + off);
           const uint2 pn = *reinterpret_cast<const uint2*>(pn_row + off);
+          // below 2q, which the inverse butterfly and the next step's
+          // derivation both take
           p0[0] += ah.x;
           p0[1] += ah.y;
           p1[0] += pn.x;
           p1[1] += pn.y;
-#pragma unroll
-          for (int e = 0; e < 2; ++e) {
-            p0[e] = p0[e] >= q ? p0[e] - q : p0[e];
-            p1[e] = p1[e] >= q ? p1[e] - q : p1[e];
-          }
           if (c == 0 && hh == 0) {
             S0[0] = p0[0];
             S0[1] = p0[1];
@@ -1018,8 +1024,14 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
           asm volatile("" : "+v"(z));  // as off above
           const uint64_t x0 = crt_compose(res0[0][s], y0[s]);
           const uint64_t x1 = crt_compose(res0[1][s], y1[s]);
-          sum[0][s] = kNttAdd ? x0 : reduce_small_q(x0 + ahat_row[z]);
-          sum[1][s] = reduce_small_q(x1 + cv_row[z] + (kNttAdd ? 0 : p_row[z]));
+          if constexpr (kNttAdd) {
+            sum[0][s] = x0;
+            const uint64_t v1 = x1 + cv_row[z];  // < 2q
+            sum[1][s] = v1 >= kQ ? v1 - kQ : v1;
+          } else {
+            sum[0][s] = reduce_small_q(x0 + ahat_row[z]);
+            sum[1][s] = reduce_small_q(x1 + cv_row[z] + p_row[z]);
+          }
         }
       }
     }

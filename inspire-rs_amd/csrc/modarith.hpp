@@ -82,10 +82,15 @@ __host__ __device__ inline uint32_t reduce58(uint64_t x) {
   return r;
 }
 
-// reduce58 with the prime as data: x mod q for x < 2^58, mu = floor(2^58/q).
+// reduce58 with the prime as data, without the final reduction: a
+// representative of x mod q below 3q for x < 2^58, mu = floor(2^58/q).
+__host__ __device__ inline uint32_t reduce58_lazy_rt(uint64_t x, uint32_t q, uint32_t mu) {
+  return (uint32_t)x - mulhi32((uint32_t)(x >> 26), mu) * q;
+}
+
+// reduce58 with the prime as data: x mod q for x < 2^58.
 __host__ __device__ inline uint32_t reduce58_rt(uint64_t x, uint32_t q, uint32_t mu) {
-  const uint32_t t = mulhi32((uint32_t)(x >> 26), mu);
-  uint32_t r = (uint32_t)x - t * q;  // < 3q
+  uint32_t r = reduce58_lazy_rt(x, q, mu);  // < 3q
   r = r >= 2 * q ? r - 2 * q : r;
   return r >= q ? r - q : r;
 }
@@ -113,11 +118,15 @@ __host__ __device__ constexpr uint32_t shoup_companion(uint32_t w, uint32_t q) {
 }
 
 // CRT compose (core/params.rs:203-215): the unique x < q with x = a mod q0 and
-// x = b mod q1, by Garner's formula x = a + q0 * ((b - a) * q0^{-1} mod q1).
+// x = b mod q1 (a < q0, b < q1), by Garner's formula
+// x = a + q0 * ((b - a) * q0^{-1} mod q1).  Since a < q0 < 2 q1, b + 2 q1 - a
+// lies in (0, 3 q1), and one Shoup product by the constant q0^{-1} brings it
+// to [0, 2 q1).
 __host__ __device__ inline uint64_t crt_compose(uint32_t a, uint32_t b) {
-  uint32_t am = a >= kQ1 ? a - kQ1 : a;      // a < q0 < 2*q1
-  uint32_t diff = b >= am ? b - am : b + kQ1 - am;
-  uint32_t t = mulmod<1>(diff, kQ0InvModQ1);
+  static_assert(kQ0 < 2 * kQ1, "b + 2 q1 - a > 0");
+  constexpr uint32_t kWp = shoup_companion(kQ0InvModQ1, kQ1);
+  uint32_t t = shoup_lazy(b + 2 * kQ1 - a, kQ0InvModQ1, kWp, kQ1);
+  t = t >= kQ1 ? t - kQ1 : t;
   return (uint64_t)a + (uint64_t)kQ0 * t;
 }
 

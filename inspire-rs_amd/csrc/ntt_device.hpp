@@ -570,8 +570,9 @@ __device__ inline void ntt_inverse_r4_halves_unscaled(uint32_t* base, const uint
 // 2^38 d2 = x exactly, from S = 2^-kM NTT_c(x) and the other two digits'
 // transforms G0, G1, by linearity:
 //     NTT_c(d2) = 2^(kM-38) S - 2^-38 G0 - 2^-19 G1   (mod q_c).
-// Inputs canonical; returns the canonical value.
-template <int kM>
+// Inputs any representatives below 2^32 (each enters one Shoup product);
+// returns the canonical value, or one below 2q without kCanonical.
+template <int kM, bool kCanonical = true>
 __device__ inline uint32_t third_gadget_digit_ntt(uint32_t S, uint32_t g0, uint32_t g1, int c) {
   static_assert(3 * kGadgetBits >= 57, "three digits must cover q");
   constexpr int kE = 2 * kGadgetBits - kM;
@@ -585,6 +586,7 @@ __device__ inline uint32_t third_gadget_digit_ntt(uint32_t S, uint32_t g0, uint3
   uint32_t x = a + 4 * q - b - e;  // (0, 6q)
   x = x >= 4 * q ? x - 4 * q : x;
   x = x >= 2 * q ? x - 2 * q : x;
+  if constexpr (!kCanonical) return x;  // [0, 2q)
   return x >= q ? x - q : x;
 }
 
