@@ -194,11 +194,13 @@ void build_fold_tables(uint32_t* out, const uint32_t* tables, const uint32_t* ne
     const uint64_t ninv = invm(kN, m);
     const uint32_t* iw = tables + (c * 4 + 2) * kN;  // inverse powers, halved
     uint32_t* t = out + c * 3 * kN;
+    uint32_t* tp = out + 6 * kN + c * kN;
     for (int i = 0; i < kN; ++i) {
       const uint64_t w = (2 * (uint64_t)iw[i]) % m;  // the inverse power itself
       t[0 * kN + i] = (uint32_t)w;
       t[1 * kN + i] = (uint32_t)((w << 32) / m);
       t[2 * kN + i] = (uint32_t)mulm(neg_a[c * kN + i], ninv, m);
+      tp[i] = (uint32_t)(((uint64_t)t[2 * kN + i] << 32) / m);
     }
   }
 }

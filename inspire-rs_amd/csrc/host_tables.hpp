@@ -29,8 +29,11 @@ void build_rgsw_neg_a(uint32_t* out, const uint32_t* tables);
 
 // out: [2 primes][3][kN] u32 for the RGSW fold: the inverse NTT powers (not
 // halved), their w' = floor(w*2^32/q), and n^-1 * NTT(-a) (neg_a as built by
-// build_rgsw_neg_a).  With them the fold's inverse NTTs skip the per-stage
-// halving: n^-1 is applied once, to the product rows before the transform.
+// build_rgsw_neg_a); then [2 primes][kN]: the w' of n^-1 * NTT(-a), for its
+// Shoup products (kFoldTableWords words in all).  With them the fold's
+// inverse NTTs skip the per-stage halving: n^-1 is applied once, to the
+// product rows before the transform.
+constexpr int kFoldTableWords = 2 * 4 * kN;
 void build_fold_tables(uint32_t* out, const uint32_t* tables, const uint32_t* neg_a);
 
 // out: [N] discrete logs of the odd exponents mod 2N, (-1)^w 5^i -> i | w << 15.

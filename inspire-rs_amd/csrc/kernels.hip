@@ -596,7 +596,7 @@ struct FoldArgs {
   const uint64_t* cvec;      // [clients][nq][instances*N]
   const uint64_t* P;         // [clients][instances][N]
   const uint32_t* bodies;    // [clients][nq][6][2][N]
-  const uint32_t* fold;      // [2][3][N]: inverse powers, their w', n^-1 NTT(-a)
+  const uint32_t* fold;      // [2][3][N]: inverse powers, their w', n^-1 NTT(-a); then [2][N]: its w
   const uint32_t* tw;        // NTT tables
   // [instances][2][N]: n^-1 NTT_c(a_hat_j) per prime, or nullptr (fold row 0
   // from a_hat in the coefficient domain)
@@ -807,8 +807,9 @@ constexpr int kFold2Slots = kN / kFold2Threads;
 static_assert(kFold2Slots == 4, "a thread's slots are one radix-4 unit of the first and last passes");
 static_assert(12ull * kQ0 < (1ull << 32) && 4ull * kQ0 * kQ0 < (1ull << 58) && kQ1 < kQ0,
               "pair stage bounds: six digits below 2q sum below 2^32; q times 4q fits reduce58");
-static_assert(12ull * kQ0 * kQ0 < (1ull << 60) && (1ull << 56) + (1ull << 32) < (1ull << 57),
-              "the row-1 sum splits into hi < 2^28 and lo, and hi (2^32 mod q) + lo < 2^57");
+static_assert(6ull * (1ull << 28) * 4 * kQ0 < (1ull << 61) && (1ull << 57) + (1ull << 32) < (1ull << 58),
+              "the row-1 sum of six 28-bit body fields times values below 4q splits into hi < 2^29 and lo, "
+              "and hi (2^32 mod q) + lo < 2^58");
 
 template <bool kNttAdd>
 __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_fold_seq(FoldArgs args) {
@@ -903,6 +904,7 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
       // v = tid + 512 h in registers.
       const uint32_t q = prime(c), q2 = 2 * q, mu = mu58(c);
       const uint32_t c32 = c ? kC32Q1 : kC32Q0, ninv = c ? kNInvQ1 : kNInvQ0;
+      const uint32_t ninvp = c ? shoup_companion(kNInvQ1, kQ1) : shoup_companion(kNInvQ0, kQ0);
       const uint32_t* fw = tw.fw(c);
       const uint32_t* fwp = tw.fwp(c);
       const uint32_t* iw = args.fold + c * 3 * kN;
@@ -919,22 +921,24 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
         // rows 0 and 2 stay for the derived row 4 (server form).  The pair's
         // per-slot operands are 8-byte loads.
         uint64_t acc[2] = {0, 0};
-        uint32_t gsum[2] = {0, 0}, g0[2] = {0, 0}, g2[2] = {0, 0};
+        uint32_t gsum[2] = {0, 0}, gsum_lo[2] = {0, 0}, g0[2] = {0, 0}, g2[2] = {0, 0};
         auto add_row = [&](int row, uint32_t x, uint32_t y) {
           const uint2 bv = *reinterpret_cast<const uint2*>(body + (row * 2 + c) * kN + off);
-          gsum[0] += x;
-          gsum[1] += y;
+          uint32_t(&gs)[2] = row < 3 ? gsum_lo : gsum;
+          gs[0] += x;
+          gs[1] += y;
           acc[0] += (uint64_t)bv.x * x;
           acc[1] += (uint64_t)bv.y * y;
         };
-        // The products take the transformed digits below 2q, not canonical:
-        // six of them sum below 12q < 2^32, and acc stays below 12 q^2 < 2^60.
+        // The products take the transformed digits as the last stage leaves
+        // them, below 4q: acc stays below 6 * 2^28 * 4q < 2^61 (the bodies are
+        // 28-bit fields of the query), and the digit sum is
+        // formed in two halves of three (each below 12q < 2^32), each brought
+        // below 4q (gsum_lo, gsum).
 #pragma unroll
         for (int p = 0; p < kDig; ++p) {
           uint32_t x = dig[p][l0], y = dig[p][l1];
           fwd_bf(x, y, fw[1024 + v], fwp[1024 + v], q);
-          x = x >= q2 ? x - q2 : x;
-          y = y >= q2 ? y - q2 : y;
           const int row = kNttAdd && p == 4 ? 5 : p;  // gadget row held in LDS row p (digit_row)
           add_row(row, x, y);
           if (row == 0) {
@@ -954,20 +958,25 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
         }
         // both rows carry n^-1 (see k_fold)
         const uint2 na = *reinterpret_cast<const uint2*>(args.fold + (c * 3 + 2) * kN + off);
+        const uint2 nap = *reinterpret_cast<const uint2*>(args.fold + (6 + c) * kN + off);
         uint32_t p0[2], p1[2];
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
-          // six values below 2q: gsum < 12q; below 4q, the product with
-          // na < q stays below 4 q^2 < 2^58, the range of reduce58
-          uint32_t gs = gsum[e] >= 8 * q ? gsum[e] - 8 * q : gsum[e];
-          gs = gs >= 4 * q ? gs - 4 * q : gs;
-          // acc < 12 q^2 < 2^60 is 2^32 hi + lo with hi < 2^28, so hi (2^32
-          // mod q) + lo < 2^57; its residue below 3q times n^-1 < q stays
-          // below 2^58
+          // each half of the digit sum from below 12q to below 4q: the sum
+          // of the halves (< 8q < 2^32) enters one Shoup product
+          uint32_t lo = gsum_lo[e] >= 8 * q ? gsum_lo[e] - 8 * q : gsum_lo[e];
+          lo = lo >= 4 * q ? lo - 4 * q : lo;
+          uint32_t hi = gsum[e] >= 8 * q ? gsum[e] - 8 * q : gsum[e];
+          hi = hi >= 4 * q ? hi - 4 * q : hi;
+          uint32_t r0 = shoup_lazy(lo + hi, e ? na.y : na.x, e ? nap.y : nap.x, q);  // [0, 2q)
+          p0[e] = r0 >= q ? r0 - q : r0;
+          // acc < 2^61 is 2^32 hi + lo with hi < 2^29, so hi (2^32
+          // mod q) + lo < 2^58; its residue below 3q enters a Shoup product
+          // by n^-1
           const uint32_t r1 =
               reduce58_lazy_rt((uint64_t)(uint32_t)(acc[e] >> 32) * c32 + (uint32_t)acc[e], q, mu);
-          p0[e] = reduce58_rt((uint64_t)(e ? na.y : na.x) * gs, q, mu);
-          p1[e] = reduce58_rt((uint64_t)r1 * ninv, q, mu);
+          const uint32_t s1 = shoup_lazy(r1, ninv, ninvp, q);  // [0, 2q)
+          p1[e] = s1 >= q ? s1 - q : s1;
         }
         if constexpr (kNttAdd) {
           const uint2 ah = *reinterpret_cast<const uint2*>(ah_row + off);

@@ -28,6 +28,13 @@ constexpr int kNPad = kN + kN / 32;  // 2112 words per polynomial in LDS
 
 __host__ __device__ constexpr int lds_index(int x) { return x + (x >> 5); }
 
+// t[i] for a workgroup-uniform table t and an unsigned 32-bit index: the byte
+// offset stays 32-bit, so the load addresses from the scalar base (no 64-bit
+// lane address arithmetic).
+__device__ inline uint32_t tab_at(const uint32_t* __restrict__ t, uint32_t i) {
+  return *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(t) + (i << 2));
+}
+
 // Forward Cooley-Tukey butterfly with Harvey's lazy reduction: x, y in [0, 4q)
 // -> x + w y, x - w y + 2q in [0, 4q), wp = floor(w 2^32 / q).  The Shoup
 // product w y mod q lies in [0, 2q) for any y < 2^32.  It is formed negated,
@@ -242,9 +249,9 @@ __device__ inline void ntt_forward_r4_1p(uint32_t* const (&polys)[NP], int c, co
   const uint32_t* fwp = tw.fwp(c);
   auto bf = [q](uint32_t& x, uint32_t& y, uint32_t w, uint32_t wp) { fwd_bf(x, y, w, wp, q); };
   for (int mm = kFirstMM; mm + 1 < kLogN; mm += 2) {
-    const int m = 1 << mm, t = (kN / 2) >> mm, th = t >> 1;
-    for (int u = tid; u < kN / 4; u += NT) {
-      int i, j;
+    const uint32_t m = 1u << mm, t = (kN / 2) >> mm, th = t >> 1;
+    for (uint32_t u = tid; u < kN / 4; u += NT) {
+      uint32_t i, j;
       if (th >= 32) {
         i = u / th;
         j = u % th;
@@ -252,11 +259,11 @@ __device__ inline void ntt_forward_r4_1p(uint32_t* const (&polys)[NP], int c, co
         i = u % m;
         j = u / m;
       }
-      const int x0 = 2 * t * i + j;
-      const int l0 = lds_index(x0), l1 = lds_index(x0 + th), l2 = lds_index(x0 + t), l3 = lds_index(x0 + t + th);
-      const uint32_t w0 = fw[m + i], wp0 = fwp[m + i];
-      const uint32_t w1 = fw[2 * m + 2 * i], wp1 = fwp[2 * m + 2 * i];
-      const uint32_t w2 = fw[2 * m + 2 * i + 1], wp2 = fwp[2 * m + 2 * i + 1];
+      const int x0 = (int)(2 * t * i + j), ti = (int)t, thi = (int)th;
+      const int l0 = lds_index(x0), l1 = lds_index(x0 + thi), l2 = lds_index(x0 + ti), l3 = lds_index(x0 + ti + thi);
+      const uint32_t w0 = tab_at(fw, m + i), wp0 = tab_at(fwp, m + i);
+      const uint32_t w1 = tab_at(fw, 2 * m + 2 * i), wp1 = tab_at(fwp, 2 * m + 2 * i);
+      const uint32_t w2 = tab_at(fw, 2 * m + 2 * i + 1), wp2 = tab_at(fwp, 2 * m + 2 * i + 1);
       fwd_stage_reduction(mm, [&](auto r1, auto r2) {
         constexpr int kR1 = decltype(r1)::value, kR2 = decltype(r2)::value;
 #pragma unroll
@@ -390,9 +397,9 @@ __device__ inline void ntt_inverse_unscaled_impl(uint32_t* const (&polys)[NP], c
   }
   if constexpr (kFirstStage) __syncthreads();
   for (int mm = kLogN - 2; mm >= kLastMM; mm -= 2) {
-    const int h = 1 << mm, t = (kN / 2) >> mm, ni = (kN / 4) / t;
-    for (int u = tid; u < kN / 4; u += NT) {
-      int i, j;
+    const uint32_t h = 1u << mm, t = (kN / 2) >> mm, ni = (kN / 4) / t;
+    for (uint32_t u = tid; u < kN / 4; u += NT) {
+      uint32_t i, j;
       if (t >= 32) {
         i = u / t;
         j = u % t;
@@ -400,8 +407,8 @@ __device__ inline void ntt_inverse_unscaled_impl(uint32_t* const (&polys)[NP], c
         i = u % ni;
         j = u / ni;
       }
-      const int x0 = 4 * t * i + j;
-      const int l0 = lds_index(x0), l1 = lds_index(x0 + t), l2 = lds_index(x0 + 2 * t), l3 = lds_index(x0 + 3 * t);
+      const int x0 = (int)(4 * t * i + j), ti = (int)t;
+      const int l0 = lds_index(x0), l1 = lds_index(x0 + ti), l2 = lds_index(x0 + 2 * ti), l3 = lds_index(x0 + 3 * ti);
 #pragma unroll
       for (int p = 0; p < NP; ++p) {
         const int c = prime_at(p);
@@ -410,10 +417,10 @@ __device__ inline void ntt_inverse_unscaled_impl(uint32_t* const (&polys)[NP], c
         const uint32_t* iwp = iw + kN;
         uint32_t* a = polys[p];
         uint32_t a0 = a[l0], a1 = a[l1], a2 = a[l2], a3 = a[l3];
-        inv_bf_unscaled(a0, a1, iw[h + 2 * i], iwp[h + 2 * i], q);
-        inv_bf_unscaled(a2, a3, iw[h + 2 * i + 1], iwp[h + 2 * i + 1], q);
-        inv_bf_unscaled(a0, a2, iw[(h >> 1) + i], iwp[(h >> 1) + i], q);
-        inv_bf_unscaled(a1, a3, iw[(h >> 1) + i], iwp[(h >> 1) + i], q);
+        inv_bf_unscaled(a0, a1, tab_at(iw, h + 2 * i), tab_at(iwp, h + 2 * i), q);
+        inv_bf_unscaled(a2, a3, tab_at(iw, h + 2 * i + 1), tab_at(iwp, h + 2 * i + 1), q);
+        inv_bf_unscaled(a0, a2, tab_at(iw, (h >> 1) + i), tab_at(iwp, (h >> 1) + i), q);
+        inv_bf_unscaled(a1, a3, tab_at(iw, (h >> 1) + i), tab_at(iwp, (h >> 1) + i), q);
         if (mm == 1) {  // last pass: canonical
           a0 = a0 >= q2 ? a0 - q2 : a0;
           a1 = a1 >= q2 ? a1 - q2 : a1;

@@ -15,8 +15,8 @@ int hip_error(hipError_t e, const char* what);
 // Device-resident constants built once per device: NTT twiddle tables
 // ([2][4][2048] u32), NTT(-a) for the RGSW row 0 ([2][2048] u32), the fold's
 // tables ([2][3][2048] u32: unhalved inverse powers, their Shoup constants,
-// n^-1 NTT(-a)) and the exponent discrete logs of the slot-ordered packing
-// GEMM ([2048] u16).
+// n^-1 NTT(-a); then [2][2048] u32: the Shoup constants of n^-1 NTT(-a)) and
+// the exponent discrete logs of the slot-ordered packing GEMM ([2048] u16).
 struct DeviceConstants {
   uint32_t* tw = nullptr;
   uint32_t* neg_a = nullptr;

@@ -58,7 +58,7 @@ const DeviceConstants* device_constants() {
   std::lock_guard<std::mutex> lk(g_const_mu);
   DeviceConstants& dc = g_consts[dev];
   if (dc.tw) return &dc;
-  std::vector<uint32_t> tw(2 * 4 * kN), na(2 * kN), fo(2 * 3 * kN);
+  std::vector<uint32_t> tw(2 * 4 * kN), na(2 * kN), fo(kFoldTableWords);
   std::vector<uint16_t> ts(kN);
   build_ntt_tables(tw.data());
   build_rgsw_neg_a(na.data(), tw.data());
