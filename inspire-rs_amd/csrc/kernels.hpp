@@ -130,7 +130,8 @@ size_t setup_offline_bytes_per_instance();
 // a HIP failure).
 int64_t setup_offline_resident_instances();
 int launch_setup_offline(const uint16_t* db, int64_t db_rows, int64_t instances, uint64_t* bt, uint64_t* btb,
-                         uint64_t* bth, uint64_t* a_hat, const uint32_t* tw, int64_t group, hipStream_t s);
+                         uint64_t* bth, uint64_t* a_hat, const uint32_t* tw, const uint32_t* fold_tab, int64_t group,
+                         hipStream_t s);
 
 size_t fold_lds_bytes(bool table);
 // out: [nq][instances/d][12288] response blobs.

@@ -94,14 +94,18 @@ struct NttTables {
 // Forward radix-4 over NP polynomials (padded LDS layout) with per-polynomial
 // primes.  Caller syncs before (inputs canonical); returns after a sync with
 // outputs canonical.  Every thread of the block participates.
-template <int NT, int NP>
+// kFirstMM and kLastStage as in ntt_forward_r4_1p below.  With kOpaque the
+// unit index is opaque to the compiler, so a caller's loop does not hoist the
+// passes' lane addresses (and spill them) across its iterations.
+template <int NT, int NP, int kFirstMM = 0, bool kLastStage = true, bool kOpaque = false>
 __device__ inline void ntt_forward_r4m(uint32_t* const (&polys)[NP], const int (&prime_of)[NP], const NttTables& tw) {
   const int tid = threadIdx.x;
   auto bf = [](uint32_t& x, uint32_t& y, uint32_t w, uint32_t wp, uint32_t q) { fwd_bf(x, y, w, wp, q); };
   // Stages (mm, mm+1): t = N/2 >> mm, units (i < m, j < t/2) at base 2t*i + j.
-  for (int mm = 0; mm + 1 < kLogN; mm += 2) {
+  for (int mm = kFirstMM; mm + 1 < kLogN; mm += 2) {
     const int m = 1 << mm, t = (kN / 2) >> mm, th = t >> 1;
     for (int u = tid; u < kN / 4; u += NT) {
+      if constexpr (kOpaque) asm volatile("" : "+v"(u));
       int i, j;
       if (th >= 32) {
         i = u / th;
@@ -143,6 +147,7 @@ __device__ inline void ntt_forward_r4m(uint32_t* const (&polys)[NP], const int (
     }
     __syncthreads();
   }
+  if constexpr (!kLastStage) return;
   // last stage (mm = 10, t = 1) and the final reduction to [0, q)
   for (int u = tid; u < kN / 2; u += NT) {
     uint32_t w[2], wp[2];
@@ -377,7 +382,7 @@ __device__ inline void inv_bf_unscaled(uint32_t& x, uint32_t& y, uint32_t w, uin
   y = w * tt - mulhi32(tt, wp) * q;
 }
 
-template <int NT, int NP, bool OnePrime, int kLastMM = 1, bool kFirstStage = true>
+template <int NT, int NP, bool OnePrime, int kLastMM = 1, bool kFirstStage = true, bool kOpaque = false>
 __device__ inline void ntt_inverse_unscaled_impl(uint32_t* const (&polys)[NP], const int* prime_of, int c1,
                                                  const uint32_t* __restrict__ fold_tab) {
   const int tid = threadIdx.x;
@@ -399,6 +404,7 @@ __device__ inline void ntt_inverse_unscaled_impl(uint32_t* const (&polys)[NP], c
   for (int mm = kLogN - 2; mm >= kLastMM; mm -= 2) {
     const uint32_t h = 1u << mm, t = (kN / 2) >> mm, ni = (kN / 4) / t;
     for (uint32_t u = tid; u < kN / 4; u += NT) {
+      if constexpr (kOpaque) asm volatile("" : "+v"(u));  // as in ntt_forward_r4m
       uint32_t i, j;
       if (t >= 32) {
         i = u / t;

@@ -970,7 +970,7 @@ extern "C" int inspire_setup_offline_device(const uint16_t* d_db, int64_t db_row
   int64_t group = 0;
   int rc = setup_group(instances, &group);
   if (rc) return rc;
-  return launch_setup_offline(d_db, db_rows, instances, d_bt, d_btb, d_bth, d_a_hat, dc->tw, group,
+  return launch_setup_offline(d_db, db_rows, instances, d_bt, d_btb, d_bth, d_a_hat, dc->tw, dc->fold, group,
                               (hipStream_t)stream);
 }
 
@@ -983,7 +983,7 @@ extern "C" int inspire_server_setup_offline(inspire_server_t* s) {
   rc = acquire_condensed(s);
   if (rc) return rc;
   rc = launch_setup_offline(s->db, s->shape.db_rows, s->shape.instances, s->bt, s->btb, s->bth, s->a_hat,
-                            s->consts->tw, group, s->stream);
+                            s->consts->tw, s->consts->fold, group, s->stream);
   if (!rc) {
     launch_ahat_ntt(s->ahat_ntt, s->a_hat, s->consts->tw, s->shape.instances, s->stream);
     hipError_t e = hipGetLastError();

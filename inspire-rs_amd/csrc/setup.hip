@@ -43,6 +43,10 @@ constexpr int kSlotBlock = 16;                      // NTT slots per k_setup_slo
 constexpr int kColThreads = 512;
 constexpr int kColSlots = kN / kColThreads;         // slots per thread in the collapse
 constexpr uint32_t kFiveInv = 3277;                 // 5^-1 mod 4096
+// n^-2 mod q_c: the scale of R in the collapse (n^-1 of r_i times the n^-1
+// the chain carries)
+constexpr uint32_t kNInv2Q0 = (uint32_t)((uint64_t)kNInvQ0 * kNInvQ0 % kQ0);
+constexpr uint32_t kNInv2Q1 = (uint32_t)((uint64_t)kNInvQ1 * kNInvQ1 % kQ1);
 
 __device__ inline uint32_t rev11d(uint32_t x) { return __brev(x) >> (32 - kLogN); }
 
@@ -123,8 +127,9 @@ __global__ __launch_bounds__(kSlotThreads) void k_setup_slot_ntt(uint32_t* __res
 #pragma unroll
   for (int p = 0; p < kSlotBlock; ++p) polys[p] = lds + p * kNPad;
   ntt_forward_r4_1p<kSlotThreads, kSlotBlock>(polys, c, T);
-  // Step order: the collapse of chain w reads, at step i and slot k, n^-1 R_z[k]
-  // with e_z = u e_k (mod 2n), u = (-1)^w 5^i; it is stored at R[g][c][w n/2 +
+  // Step order: the collapse of chain w reads, at step i and slot k, n^-2 R_z[k]
+  // (r_i's n^-1 times the n^-1 its chain carries) with e_z = u e_k (mod 2n),
+  // u = (-1)^w 5^i; it is stored at R[g][c][w n/2 +
   // i][k].  This workgroup's 16 slots z0.. are the exponents e = a (mod 256), and
   // multiplying by u maps that coset to the 16 consecutive slots k0(u).. with
   // e_k0 = a u^-1 (mod 256), so every (w, i) row gets one contiguous 64-byte run
@@ -154,7 +159,7 @@ __global__ __launch_bounds__(kSlotThreads) void k_setup_slot_ntt(uint32_t* __res
       const int k = (int)(__brev((b - 1u) >> 1) >> 25) * kSlotBlock + m;
       const int zz = auto_slot(u, k) - z0;
       const uint32_t v = lds[zz * kNPad + lds_index(k)];
-      dst[((int64_t)w * (kN / 2) + i) * kN + k] = c ? mulmod<1>(v, kNInvQ1) : mulmod<0>(v, kNInvQ0);
+      dst[((int64_t)w * (kN / 2) + i) * kN + k] = c ? mulmod<1>(v, kNInv2Q1) : mulmod<0>(v, kNInv2Q0);
       u5 = (u5 * step) & kMask2n;
       u5inv = (u5inv * step_inv) & kMask2n;
     }
@@ -166,44 +171,34 @@ struct CollapseArgs {
   uint64_t* btb;
   uint64_t* bth;        // [instances][3][N] condensed
   uint32_t* r0;         // [G][2][N] updated r_0 (NTT), read by k_setup_ahat
-  const uint32_t* R;    // [G][2][N][N]
-  const uint32_t* w;    // W mask [3][2][N] NTT residues
+  const uint32_t* R;    // [G][2][N][N]: n^-2 R, collapse step order
+  const uint32_t* w;    // n^-1 W mask [3][2][N] NTT residues
   const uint32_t* tw;
+  const uint32_t* fold;  // unhalved inverse powers (build_fold_tables)
   uint32_t g_top;       // 5^(n/2 - 1) mod 2n
 };
 
-// G^-1 of the canonical NTT-domain polynomial x in dig[0], dig[1] (both
-// primes): INTT, CRT compose, the first two 19-bit digits into dig[2k + c]
-// and their forward NTTs.  The third digit's transform follows from x and
-// those two (third_gadget_digit_ntt<0>, with the copy of x the caller keeps in
-// dig[4], dig[5]), so it is never transformed.  Every thread touches only its
-// own slots between the two transforms.
-__device__ inline void gadget_ntt(uint32_t (&dig)[6][kNPad], const NttTables& T) {
-  uint32_t* const ip[2] = {dig[0], dig[1]};
-  const int ipr[2] = {0, 1};
-  uint32_t* const fp[4] = {dig[0], dig[1], dig[2], dig[3]};
-  const int fpr[4] = {0, 1, 0, 1};
-  ntt_inverse_r4<kColThreads, 2>(ip, ipr, T);
-  const uint64_t mask = (1ull << kGadgetBits) - 1;
-#pragma unroll
-  for (int s = 0; s < kColSlots; ++s) {
-    const int lz = lds_index(threadIdx.x + s * kColThreads);
-    const uint64_t raw = crt_compose(dig[0][lz], dig[1][lz]);
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      const uint32_t d = (uint32_t)((raw >> (kGadgetBits * k)) & mask);  // < 2^19: canonical for both primes
-      dig[2 * k][lz] = d;
-      dig[2 * k + 1][lz] = d;
-    }
-  }
-  __syncthreads();
-  ntt_forward_r4m<kColThreads, 4>(fp, fpr, T);
-}
-
+// One workgroup per (instance, chain r or rbar): the 1023-step collapse
+// (packing.rs:2694-2731).  A step of chain r computes
+//   bold_t[i][k] = NTT(G^-1(raw(r_(i+1))))[k],  r_i += sum_k w[i][k] bold_t[i][k]
+// with r_i's own term gathered from R.  The chain carries S = n^-1 NTT_c(r)
+// (R and W arrive scaled by n^-2 and n^-1), so the inverse transform runs
+// without scaling, and the transform of the third gadget digit follows from S
+// and the other two (third_gadget_digit_ntt<11>): 2 inverse and 4 forward
+// transforms per step.
+//
+// Per step: the inverse passes of stages 2-9 over both primes in LDS; the
+// last inverse pass, the CRT composition, the two digits and the first forward
+// pass in registers (a thread's slots tid + 512 s are one unit of those
+// passes); the forward passes of stages 3-10 in LDS; then, on slot pairs
+// (2v, 2v + 1), v = tid + 512 h, in registers: the last forward stage, the
+// third digit, the bold_t writes, the W products, and the first inverse stage
+// of the next step.
 __global__ __launch_bounds__(kColThreads) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_setup_collapse(
     CollapseArgs a) {
-  // rows 0-3: the transformed digits; rows 4, 5: r_(i+1) per prime at each
-  // thread's own slots (read back by the same thread for the third digit)
+  // row 2k + c: digit k over prime c (forward); rows 0 and 1 also hold the
+  // inverse transform of prime 0 and 1; row 4 + c: S over prime c (each
+  // thread reads and writes only its own pair slots there)
   __shared__ uint32_t dig[6][kNPad];
   const NttTables T{a.tw};
   const int tid = threadIdx.x;
@@ -211,97 +206,185 @@ __global__ __launch_bounds__(kColThreads) __attribute__((amdgpu_waves_per_eu(6, 
   const int which = blockIdx.x & 1;  // 0: r chain (bold_t), 1: rbar chain (bold_t_bar)
   const uint32_t* Rg = a.R + g * 2 * (int64_t)kN * kN;
   uint64_t* out = (which ? a.btb : a.bt) + g * (int64_t)kNInner * kN;
+  uint32_t* const fpolys[4] = {dig[0], dig[1], dig[2], dig[3]};
+  const int fprime[4] = {0, 1, 0, 1};
+  uint32_t* const ipolys[2] = {dig[0], dig[1]};
+  const int iprime[2] = {0, 1};
+  const uint64_t mask = (1ull << kGadgetBits) - 1;
 
-  // r_i (or rbar_i) at this thread's slots, in step order in R (see
-  // k_setup_slot_ntt).
-  auto load_r = [&](int i, uint32_t (&rr)[2][kColSlots]) {
+  // rr[c][2h + e] at slot 2(tid + 512 h) + e (indices constant or selected by
+  // h, so it stays in registers).
+  uint32_t rr[2][4];
+  auto load_r = [&](int i) {  // n^-2 R of step i at the pair slots
     const int64_t row = (int64_t)which * (kN / 2) + i;
 #pragma unroll
-    for (int s = 0; s < kColSlots; ++s) {
-      const int z = tid + s * kColThreads;
-      rr[0][s] = Rg[row * kN + z];
-      rr[1][s] = Rg[((int64_t)kN + row) * kN + z];
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const uint2 v2 = *reinterpret_cast<const uint2*>(Rg + ((int64_t)c * kN + row) * kN + 2 * (tid + h * kColThreads));
+        rr[c][2 * h] = v2.x;
+        rr[c][2 * h + 1] = v2.y;
+      }
+  };
+  // First inverse stage of S at pair h: rows 0 and 1 (Harvey, inputs < 2q).
+  auto first_inverse = [&](int h, const uint32_t (&s0)[2], const uint32_t (&s1)[2]) {
+    int v = tid + h * kColThreads;
+    asm volatile("" : "+v"(v));  // as in the passes (kOpaque)
+    const int l0 = lds_index(2 * v), l1 = lds_index(2 * v + 1);
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const uint32_t* iw = a.fold + c * 3 * kN;
+      uint32_t x = c ? s1[0] : s0[0], y = c ? s1[1] : s0[1];
+      inv_bf_unscaled(x, y, iw[1024 + v], iw[kN + 1024 + v], prime(c));
+      dig[c][l0] = x;
+      dig[c][l1] = y;
     }
   };
-
-  uint32_t cur[2][kColSlots], rr[2][kColSlots];
-  uint32_t gi = a.g_top;  // g_(n/2 - 1)
-  load_r(kN / 2 - 1, rr);
-#pragma unroll
-  for (int s = 0; s < kColSlots; ++s) {
-    cur[0][s] = rr[0][s];
-    cur[1][s] = rr[1][s];
-  }
-  for (int i = kN / 2 - 2; i >= 0; --i) {
-    gi = (gi * kFiveInv) & (2u * kN - 1u);  // g_i = 5^i mod 2n
-    const uint32_t t = which ? 2u * kN - gi : gi;
-    load_r(i, rr);  // independent of the chain: in flight during the transforms
+  // From rows 0 and 1 after the first inverse stage to the forward digit
+  // transforms before their last stage (rows 0-3).
+  auto transform = [&]() {
+    __syncthreads();
+    ntt_inverse_unscaled_impl<kColThreads, 2, false, 3, false, true>(ipolys, iprime, 0, a.fold);
+    uint32_t y0[kColSlots], y1[kColSlots];
 #pragma unroll
     for (int s = 0; s < kColSlots; ++s) {
       const int lz = lds_index(tid + s * kColThreads);
-      dig[0][lz] = dig[4][lz] = cur[0][s];
-      dig[1][lz] = dig[5][lz] = cur[1][s];
+      y0[s] = dig[0][lz];
+      y1[s] = dig[1][lz];
     }
-    __syncthreads();
-    gadget_ntt(dig, T);
-    // bold_t[i][k] = NTT(G^-1(raw(r_{i+1}))); r_i += sum_k w_all[i][k] * bold_t[i][k]
+    ntt_inverse_last_pass_unscaled(y0, 0, a.fold);
+    ntt_inverse_last_pass_unscaled(y1, 1, a.fold);
+    // G^-1: the first two unsigned 19-bit digits (< 2^19, canonical for both
+    // primes); each thread rewrites only its own slots here
+    uint32_t d[2][kColSlots];
 #pragma unroll
     for (int s = 0; s < kColSlots; ++s) {
-      const int z = tid + s * kColThreads;
-      const int lz = lds_index(z);
-      const int zs = auto_slot(t, z);  // the W mask rotated by t_i (cache-resident)
-      uint64_t acc0 = rr[0][s], acc1 = rr[1][s];
-      uint32_t dk[3][2];
+      const uint64_t raw = crt_compose(y0[s], y1[s]);
+      d[0][s] = (uint32_t)(raw & mask);
+      d[1][s] = (uint32_t)((raw >> kGadgetBits) & mask);
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        uint32_t u[kColSlots];
+#pragma unroll
+        for (int s = 0; s < kColSlots; ++s) u[s] = d[k][s];
+        ntt_forward_first_pass(u, c, T);
+#pragma unroll
+        for (int s = 0; s < kColSlots; ++s) dig[2 * k + c][lds_index(tid + s * kColThreads)] = u[s];
+      }
+    __syncthreads();
+    ntt_forward_r4m<kColThreads, 4, 2, false, true>(fpolys, fprime, T);
+  };
+  // Last forward stage at pair h: the three digit transforms gk[k][c][e],
+  // canonical (the third derived from S).
+  auto digits_at = [&](int h, uint32_t (&gk)[3][2][2]) {
+    int v = tid + h * kColThreads;
+    asm volatile("" : "+v"(v));
+    const int l0 = lds_index(2 * v), l1 = lds_index(2 * v + 1);
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const uint32_t q = prime(c), q2 = 2 * q;
+      const uint32_t w = T.fw(c)[1024 + v], wp = T.fwp(c)[1024 + v];
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
-        dk[k][0] = dig[2 * k][lz];
-        dk[k][1] = dig[2 * k + 1][lz];
+        uint32_t x = dig[2 * k + c][l0], y = dig[2 * k + c][l1];
+        fwd_bf(x, y, w, wp, q);
+        x = x >= q2 ? x - q2 : x;
+        x = x >= q ? x - q : x;
+        y = y >= q2 ? y - q2 : y;
+        y = y >= q ? y - q : y;
+        gk[k][c][0] = x;
+        gk[k][c][1] = y;
       }
 #pragma unroll
-      for (int c = 0; c < 2; ++c) dk[2][c] = third_gadget_digit_ntt<0>(dig[4 + c][lz], dk[0][c], dk[1][c], c);
+      for (int e = 0; e < 2; ++e)
+        gk[2][c][e] = third_gadget_digit_ntt<kLogN>(dig[4 + c][e ? l1 : l0], gk[0][c][e], gk[1][c][e], c);
+    }
+  };
+
+  load_r(kN / 2 - 1);
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int v = tid + h * kColThreads;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      dig[4 + c][lds_index(2 * v)] = rr[c][2 * h];
+      dig[4 + c][lds_index(2 * v + 1)] = rr[c][2 * h + 1];
+    }
+    first_inverse(h, {rr[0][2 * h], rr[0][2 * h + 1]}, {rr[1][2 * h], rr[1][2 * h + 1]});
+  }
+  uint32_t gi = a.g_top;  // g_(n/2 - 1)
+  for (int i = kN / 2 - 2; i >= 0; --i) {
+    gi = (gi * kFiveInv) & (2u * kN - 1u);  // g_i = 5^i mod 2n
+    const uint32_t t = which ? 2u * kN - gi : gi;
+    load_r(i);  // independent of the chain: in flight during the transforms
+    transform();
+#pragma unroll 1
+    for (int h = 0; h < 2; ++h) {
+      const int v = tid + h * kColThreads;
+      uint32_t off = 2u * (uint32_t)v;
+      asm volatile("" : "+v"(off));  // keeps the lane offset 32-bit
+      uint32_t gk[3][2][2];
+      digits_at(h, gk);
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        out[((int64_t)i * kTExp + k) * kN + z] = (uint64_t)dk[k][0] | ((uint64_t)dk[k][1] << 32);
-        acc0 += (uint64_t)a.w[(k * 2 + 0) * kN + zs] * dk[k][0];
-        acc1 += (uint64_t)a.w[(k * 2 + 1) * kN + zs] * dk[k][1];
+        ulonglong2 pr;
+        pr.x = (uint64_t)gk[k][0][0] | ((uint64_t)gk[k][1][0] << 32);
+        pr.y = (uint64_t)gk[k][0][1] | ((uint64_t)gk[k][1][1] << 32);
+        *reinterpret_cast<ulonglong2*>(out + ((int64_t)i * kTExp + k) * kN + off) = pr;
       }
-      cur[0][s] = reduce58<0>(acc0);
-      cur[1][s] = reduce58<1>(acc1);
+      // S <- n^-1 NTT(r_i) = n^-2 R + sum_k n^-1 W(t_i) bold_t[i][k] (the W mask
+      // rotated by t_i is a cache-resident gather)
+      uint32_t sn[2][2];
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const int zs = auto_slot(t, 2 * v + e);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          uint64_t acc = h ? rr[c][2 + e] : rr[c][e];
+#pragma unroll
+          for (int k = 0; k < 3; ++k) acc += (uint64_t)a.w[(k * 2 + c) * kN + zs] * gk[k][c][e];
+          sn[c][e] = c ? reduce58<1>(acc) : reduce58<0>(acc);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        dig[4 + c][lds_index(2 * v)] = sn[c][0];
+        dig[4 + c][lds_index(2 * v + 1)] = sn[c][1];
+      }
+      first_inverse(h, sn[0], sn[1]);
     }
   }
   if (which == 0) {
+    // r_0 = n S
 #pragma unroll
-    for (int s = 0; s < kColSlots; ++s) {
-      const int z = tid + s * kColThreads;
-      a.r0[(g * 2 + 0) * kN + z] = cur[0][s];
-      a.r0[(g * 2 + 1) * kN + z] = cur[1][s];
-    }
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const int z = 2 * (tid + h * kColThreads) + e;
+          const uint32_t sv = dig[4 + c][lds_index(z)];
+          a.r0[(g * 2 + c) * kN + z] = c ? mulmod<1>(sv, (uint32_t)kN) : mulmod<0>(sv, (uint32_t)kN);
+        }
     return;
   }
   // bold_t_hat = NTT(G^-1(raw(rbar_0)))  (packing.rs:2733-2734)
+  transform();
 #pragma unroll
-  for (int s = 0; s < kColSlots; ++s) {
-    const int lz = lds_index(tid + s * kColThreads);
-    dig[0][lz] = dig[4][lz] = cur[0][s];
-    dig[1][lz] = dig[5][lz] = cur[1][s];
-  }
-  __syncthreads();
-  gadget_ntt(dig, T);
+  for (int h = 0; h < 2; ++h) {
+    const int v = tid + h * kColThreads;
+    uint32_t gk[3][2][2];
+    digits_at(h, gk);
 #pragma unroll
-  for (int s = 0; s < kColSlots; ++s) {
-    const int z = tid + s * kColThreads;
-    const int lz = lds_index(z);
-    uint32_t dk[3][2];
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      dk[k][0] = dig[2 * k][lz];
-      dk[k][1] = dig[2 * k + 1][lz];
+    for (int k = 0; k < 3; ++k) {
+      ulonglong2 pr;
+      pr.x = (uint64_t)gk[k][0][0] | ((uint64_t)gk[k][1][0] << 32);
+      pr.y = (uint64_t)gk[k][0][1] | ((uint64_t)gk[k][1][1] << 32);
+      *reinterpret_cast<ulonglong2*>(a.bth + (g * kTExp + k) * kN + 2 * v) = pr;
     }
-#pragma unroll
-    for (int c = 0; c < 2; ++c) dk[2][c] = third_gadget_digit_ntt<0>(dig[4 + c][lz], dk[0][c], dk[1][c], c);
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-      a.bth[(g * kTExp + k) * kN + z] = (uint64_t)dk[k][0] | ((uint64_t)dk[k][1] << 32);
   }
 }
 
@@ -361,7 +444,8 @@ int64_t setup_offline_resident_instances() {
 }
 
 int launch_setup_offline(const uint16_t* db, int64_t db_rows, int64_t instances, uint64_t* bt, uint64_t* btb,
-                         uint64_t* bth, uint64_t* a_hat, const uint32_t* tw, int64_t group, hipStream_t s) {
+                         uint64_t* bth, uint64_t* a_hat, const uint32_t* tw, const uint32_t* fold_tab, int64_t group,
+                         hipStream_t s) {
   if (db_rows <= 0 || db_rows % kN || instances <= 0 || group <= 0) return set_error(INSPIRE_ERR_INVALID, "setup shape: db_rows % 2048 == 0, instances > 0");
   const int nrlwe = (int)(db_rows / kN);
   // Public masks (host): SEED_0 query masks abar_k = -a_k (client/online.rs:164-168,
@@ -393,6 +477,11 @@ int launch_setup_offline(const uint16_t* db, int64_t db_rows, int64_t instances,
     chacha20_u64(seed, r.data(), r.size());  // PolyMatrixNTT::random_rng, poly.rs:418-442
     for (size_t i = 0; i < r.size(); ++i) masks[(size_t)m * 3 * 2 * kN + i] = (uint32_t)(r[i] % prime((int)((i / kN) & 1)));
   }
+  // the collapse takes W scaled by n^-1 (the scale its chain carries)
+  for (size_t i = 0; i < (size_t)3 * 2 * kN; ++i) {
+    const int c = (int)((i / kN) & 1);
+    masks[i] = (uint32_t)((uint64_t)masks[i] * (c ? kNInvQ1 : kNInvQ0) % prime(c));
+  }
   const int64_t G = std::min(group, instances);
   uint32_t *d_abar = nullptr, *d_masks = nullptr, *d_at = nullptr, *d_R = nullptr, *d_r0 = nullptr;
   auto cleanup = [&]() {
@@ -423,7 +512,7 @@ int launch_setup_offline(const uint16_t* db, int64_t db_rows, int64_t instances,
     hipLaunchKernelGGL(k_setup_slot_ntt, dim3((unsigned)(gn * 2 * (kN / kSlotBlock))), dim3(kSlotThreads),
                        kSlotBlock * kNPad * sizeof(uint32_t), s, d_R, d_at, tw);
     CollapseArgs ca{bt + t0 * (int64_t)kNInner * kN, btb + t0 * (int64_t)kNInner * kN, bth + t0 * (int64_t)kTExp * kN,
-                    d_r0, d_R, d_masks, tw, pow5_mod_2n(kN / 2 - 1)};
+                    d_r0, d_R, d_masks, tw, fold_tab, pow5_mod_2n(kN / 2 - 1)};
     hipLaunchKernelGGL(k_setup_collapse, dim3((unsigned)(2 * gn)), dim3(kColThreads), 0, s, ca);
     hipLaunchKernelGGL(k_setup_ahat, dim3((unsigned)gn), dim3(kColThreads), 0, s, a_hat + t0 * kN, d_r0,
                        bth + t0 * (int64_t)kTExp * kN, d_masks + 3 * 2 * kN, tw);
