@@ -117,10 +117,17 @@ __global__ __launch_bounds__(kSlotThreads) void k_setup_slot_ntt(uint32_t* __res
   const int c = (blockIdx.x / kBlocksPerPrime) & 1;
   const int64_t g = blockIdx.x / (2 * kBlocksPerPrime);
   const uint32_t* src = at + (g * kN * 2 + c) * kN + zb * kSlotBlock;
-  // 16 consecutive slots (64 bytes) of each column j
-  for (int e = tid; e < kN * kSlotBlock; e += kSlotThreads) {
-    const int zz = e % kSlotBlock, j = e / kSlotBlock;
-    lds[zz * kNPad + lds_index(j)] = src[(int64_t)j * 2 * kN + zz];
+  // 16 consecutive slots (64 bytes) of each column j, as four 16-byte loads
+  constexpr int kQuads = kSlotBlock / 4;
+#pragma unroll 4
+  for (int e = tid; e < kN * kQuads; e += kSlotThreads) {
+    const int zq = (e % kQuads) * 4, j = e / kQuads;
+    const uint4 v = *reinterpret_cast<const uint4*>(src + (int64_t)j * 2 * kN + zq);
+    const int lj = lds_index(j);
+    lds[(zq + 0) * kNPad + lj] = v.x;
+    lds[(zq + 1) * kNPad + lj] = v.y;
+    lds[(zq + 2) * kNPad + lj] = v.z;
+    lds[(zq + 3) * kNPad + lj] = v.w;
   }
   __syncthreads();
   uint32_t* polys[kSlotBlock];
@@ -136,30 +143,35 @@ __global__ __launch_bounds__(kSlotThreads) void k_setup_slot_ntt(uint32_t* __res
   // from here and the collapse reads each step's row in order.
   const int z0 = zb * kSlotBlock;
   const uint32_t a = (2u * rev11d((uint32_t)z0) + 1u) & 255u;
-  const int m = tid % kSlotBlock, rb = tid / kSlotBlock;  // rows rb + 32 it of each chain
-  constexpr int kRowStep = kSlotThreads / kSlotBlock;     // 32
+  // each thread stores four consecutive slots (16 bytes) of rows rb + 128 it
+  const int m4 = (tid % kQuads) * 4, rb = tid / kQuads;
+  constexpr int kRowStep = kSlotThreads / kQuads;  // 128
   constexpr uint32_t kMask2n = 2u * kN - 1u;
-  uint32_t p = 1, pinv = 1;
-  for (int x = 0; x < rb; ++x) {
-    p = (p * 5u) & kMask2n;
-    pinv = (pinv * kFiveInv) & kMask2n;
-  }
-  uint32_t step = 1, step_inv = 1;
-  for (int x = 0; x < kRowStep; ++x) {
-    step = (step * 5u) & kMask2n;
-    step_inv = (step_inv * kFiveInv) & kMask2n;
-  }
+  auto pow_mod_2n = [](uint32_t base, int e) {  // base^e mod 2n by squaring
+    uint32_t r = 1;
+    for (; e; e >>= 1, base = (base * base) & kMask2n)
+      if (e & 1) r = (r * base) & kMask2n;
+    return r;
+  };
+  const uint32_t p = pow_mod_2n(5u, rb), pinv = pow_mod_2n(kFiveInv, rb);
+  const uint32_t step = pow_mod_2n(5u, kRowStep), step_inv = pow_mod_2n(kFiveInv, kRowStep);
   uint32_t* dst = R + (g * 2 + c) * (int64_t)kN * kN;
   for (int w = 0; w < 2; ++w) {
-    uint32_t u5 = p, u5inv = pinv;  // 5^i and 5^-i for i = rb + 32 it
+    uint32_t u5 = p, u5inv = pinv;  // 5^i and 5^-i for i = rb + 128 it
     for (int i = rb; i < kN / 2; i += kRowStep) {
       const uint32_t u = w ? (2u * kN - u5) & kMask2n : u5;
       const uint32_t uinv = w ? (2u * kN - u5inv) & kMask2n : u5inv;
       const uint32_t b = (a * uinv) & 255u;  // e_k0 mod 256 = 2 rev7(k0 / 16) + 1
-      const int k = (int)(__brev((b - 1u) >> 1) >> 25) * kSlotBlock + m;
-      const int zz = auto_slot(u, k) - z0;
-      const uint32_t v = lds[zz * kNPad + lds_index(k)];
-      dst[((int64_t)w * (kN / 2) + i) * kN + k] = c ? mulmod<1>(v, kNInv2Q1) : mulmod<0>(v, kNInv2Q0);
+      const int k0 = (int)(__brev((b - 1u) >> 1) >> 25) * kSlotBlock + m4;
+      uint32_t o[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int k = k0 + t;
+        const int zz = auto_slot(u, k) - z0;
+        const uint32_t v = lds[zz * kNPad + lds_index(k)];
+        o[t] = c ? mulmod<1>(v, kNInv2Q1) : mulmod<0>(v, kNInv2Q0);
+      }
+      *reinterpret_cast<uint4*>(dst + ((int64_t)w * (kN / 2) + i) * kN + k0) = uint4{o[0], o[1], o[2], o[3]};
       u5 = (u5 * step) & kMask2n;
       u5inv = (u5inv * step_inv) & kMask2n;
     }
