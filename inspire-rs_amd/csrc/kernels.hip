@@ -818,9 +818,11 @@ __global__ __launch_bounds__(kFold2Threads) __attribute__((amdgpu_waves_per_eu(6
   const NttTables tw{args.tw};
   const int per_client = args.nq * args.c;
   const int chain = xcd_chunk((int)blockIdx.x, (int)gridDim.x);
+  // the hash query varies fastest, so the chains that read the same P rows
+  // (one per hash query) run side by side and share them in L2
   const int b = chain / per_client;
-  const int h = (chain % per_client) / args.c;
-  const int w = chain % args.c;
+  const int h = chain % args.nq;
+  const int w = (chain % per_client) / args.nq;
   const int tid = threadIdx.x;
   const int d = args.d;
   const uint64_t* cv = args.cvec + ((size_t)b * args.nq + h) * args.instances * kN;
