@@ -647,8 +647,8 @@ __device__ inline void fold_write_response(const FoldArgs& args, int b, int h, i
 // Ah_j = n^-1 NTT_c(a_hat_j), the fold adds Ah_j to its row-0 product before
 // the inverse transform, so that product row S (n^-1 scaled) satisfies
 // NTT_c(sum row 0) = n S = 2^11 S, and the transform of row 0's third gadget
-// digit follows from S and the other two (third_gadget_digit_ntt<11>): one of
-// the six digit transforms of a step per prime.
+// digit follows from S and the other two (third_gadget_digit_ntt<11>): it
+// replaces one of the six digit transforms of a step per prime.
 
 // LDS row of gadget digit row k (row r + 2 k' for digit k' of row r) among
 // the transformed ones: in the server form, row 4 (row 0's third digit) is
@@ -797,8 +797,9 @@ __global__ __launch_bounds__(kFoldThreads) void k_fold(FoldArgs args) {
 }
 
 // The same fold with 512 threads and the two CRT primes taken one after the
-// other: per Horner step and prime, the six gadget digits are transformed, the
-// two product rows are formed in place of digits 0 and 1 and transformed back;
+// other: per Horner step and prime, the gadget digits are transformed (five in
+// the server form, whose sixth follows from S), the two product rows are
+// formed in place of digits 0 and 1 and transformed back;
 // prime 0's result waits in registers for the CRT.  50 KiB of LDS, so three
 // workgroups share a CU and every thread has a radix-4 unit in each pass.
 constexpr int kFold2Threads = 512;
