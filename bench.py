@@ -185,31 +185,42 @@ def bench_32m(IA, torch, rates=(100, 300, 600, 700), serve_seconds=2.0):
     dq, qlen, stride, blobs = make_queries(IA, torch, db_rows, 4, 31, return_host=True)
     blob = 2 * plan["c"] * IA.RESP_BYTES
     out = torch.empty(64 * blob, dtype=torch.uint8, device="cuda")
-    st = torch.cuda.Stream()
-    for k in range(2):
-        srv.respond_device(dq.data_ptr() + (k % 4) * stride, qlen, 2, out, st.cuda_stream)
-    torch.cuda.synchronize()
+    outs = [out[:blob], out[blob:2 * blob]]
+    sts = [torch.cuda.Stream() for _ in range(2)]
     steps = 10
-    t0 = time.perf_counter()
-    for k in range(steps):
-        srv.respond_device(dq.data_ptr() + (k % 4) * stride, qlen, 2, out, st.cuda_stream)
-    torch.cuda.synchronize()
-    single_ms = (time.perf_counter() - t0) * 1e3 / steps
+
+    def timed(in_flight):
+        """ms per single-query respond with `in_flight` requests on their own streams."""
+        for k in range(2):
+            srv.respond_device(dq.data_ptr() + (k % 4) * stride, qlen, 2, outs[k % in_flight],
+                               sts[k % in_flight].cuda_stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            srv.respond_device(dq.data_ptr() + (k % 4) * stride, qlen, 2, outs[k % in_flight],
+                               sts[k % in_flight].cuda_stream)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    serial_ms = timed(1)
+    single_ms = timed(2)
     res = {"workload": "32M ML-KEM-768 keys, dim0 65536, lean server",
            "db_rows": db_rows, "db_cols": plan["db_cols"], "instances": inst, "db_bytes": plan["db_bytes"],
            "bold_t_bytes": plan["bold_t_bytes"] * 56 // 64,  # packed fragments, 28 bits per residue
-           "single_query": {"ms": single_ms, "qps": 1e3 / single_ms},
+           "single_query": {"ms": single_ms, "qps": 1e3 / single_ms, "in_flight": 2,
+                            "serial_ms": serial_ms, "serial_qps": 1e3 / serial_ms},
            "batched": {}, "serving": []}
-    log("32m: single query %.2f ms (%.1f q/s)" % (single_ms, 1e3 / single_ms))
+    log("32m: single query %.2f ms (%.1f q/s) with two in flight, %.2f ms serial" %
+        (single_ms, 1e3 / single_ms, serial_ms))
     srv.set_max_batch(64)
     for B in (8, 32, 64):
         bq = dq.repeat(B // 4 + 1)[:B * stride].contiguous()
-        srv.respond_batch_device(bq, stride, qlen, 2, B, out, blob, st.cuda_stream)
+        srv.respond_batch_device(bq, stride, qlen, 2, B, out, blob, sts[0].cuda_stream)
         torch.cuda.synchronize()
         reps = 3
         t0 = time.perf_counter()
         for _ in range(reps):
-            srv.respond_batch_device(bq, stride, qlen, 2, B, out, blob, st.cuda_stream)
+            srv.respond_batch_device(bq, stride, qlen, 2, B, out, blob, sts[0].cuda_stream)
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t0) * 1e3 / reps
         res["batched"][str(B)] = {"ms_per_batch": ms, "qps": B * 1e3 / ms, "speedup_vs_single_query": B * single_ms / ms}
