@@ -62,9 +62,12 @@ __device__ inline uint32_t neg_mod(uint32_t v) {
   return v ? prime(c) - v : 0u;
 }
 
-__global__ __launch_bounds__(kAtThreads) void k_setup_atilde(uint32_t* __restrict__ at, const uint16_t* __restrict__ db,
-                                                             int db_rows, int64_t col0,
-                                                             const uint32_t* __restrict__ abar, const uint32_t* tw) {
+// kOne: one query RLWE row per column (db_rows = n), so the product is the
+// output and no accumulator lives across the transform (8 waves per SIMD).
+template <bool kOne>
+__global__ __launch_bounds__(kAtThreads) __attribute__((amdgpu_waves_per_eu(kOne ? 8 : 4, 8))) void k_setup_atilde(
+    uint32_t* __restrict__ at, const uint16_t* __restrict__ db, int db_rows, int64_t col0,
+    const uint32_t* __restrict__ abar, const uint32_t* tw) {
   __shared__ uint32_t buf[2][kNPad];
   constexpr int kS = kN / kAtThreads;
   const NttTables T{tw};
@@ -76,7 +79,7 @@ __global__ __launch_bounds__(kAtThreads) void k_setup_atilde(uint32_t* __restric
   uint32_t acc[2][kS];
 #pragma unroll
   for (int s = 0; s < kS; ++s) acc[0][s] = acc[1][s] = 0;
-  const int nrlwe = db_rows / kN;
+  const int nrlwe = kOne ? 1 : db_rows / kN;
   for (int k = 0; k < nrlwe; ++k) {
     // Dt(X) = sum_i v_i X^-i: coefficient 0 is v_0, coefficient m >= 1 is -v_(n-m).
 #pragma unroll
@@ -89,6 +92,16 @@ __global__ __launch_bounds__(kAtThreads) void k_setup_atilde(uint32_t* __restric
     __syncthreads();
     ntt_forward_r4m<kAtThreads, 2>(polys, primes, T);
     const uint32_t* ab = abar + (size_t)k * 2 * kN;
+    if constexpr (kOne) {
+      uint32_t* o = at + jl * 2 * kN;
+#pragma unroll
+      for (int s = 0; s < kS; ++s) {
+        const int x = tid + s * kAtThreads;
+        o[x] = mulmod<0>(buf[0][lds_index(x)], ab[x]);
+        o[kN + x] = mulmod<1>(buf[1][lds_index(x)], ab[kN + x]);
+      }
+      return;
+    }
 #pragma unroll
     for (int s = 0; s < kS; ++s) {
       const int x = tid + s * kAtThreads;
@@ -519,8 +532,12 @@ int launch_setup_offline(const uint16_t* db, int64_t db_rows, int64_t instances,
   }
   for (int64_t t0 = 0; t0 < instances && e == hipSuccess; t0 += G) {
     const int64_t gn = std::min(G, instances - t0);
-    hipLaunchKernelGGL(k_setup_atilde, dim3((unsigned)(gn * kN)), dim3(kAtThreads), 0, s, d_at, db, (int)db_rows,
-                       t0 * kN, d_abar, tw);
+    if (nrlwe == 1)
+      hipLaunchKernelGGL(k_setup_atilde<true>, dim3((unsigned)(gn * kN)), dim3(kAtThreads), 0, s, d_at, db,
+                         (int)db_rows, t0 * kN, d_abar, tw);
+    else
+      hipLaunchKernelGGL(k_setup_atilde<false>, dim3((unsigned)(gn * kN)), dim3(kAtThreads), 0, s, d_at, db,
+                         (int)db_rows, t0 * kN, d_abar, tw);
     hipLaunchKernelGGL(k_setup_slot_ntt, dim3((unsigned)(gn * 2 * (kN / kSlotBlock))), dim3(kSlotThreads),
                        kSlotBlock * kNPad * sizeof(uint32_t), s, d_R, d_at, tw);
     CollapseArgs ca{bt + t0 * (int64_t)kNInner * kN, btb + t0 * (int64_t)kNInner * kN, bth + t0 * (int64_t)kTExp * kN,
